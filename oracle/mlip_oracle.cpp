@@ -1,0 +1,1038 @@
+// TEST INFRASTRUCTURE ONLY -- the CPU oracle.  Only tests/, the smoke check
+// and bench.py's cpu_baseline leg may load this library; the product path in
+// lammps-mlip-package_amd/ never links or calls it.
+//
+// FP64 C++/OpenMP restatement of the reference CPU pair style
+// `pair_style mlip_gtinv` (src/USER-MLIP/pair_mlip_gtinv.cpp) together with
+// the model/data classes it calls.  The loop structure, the unique-partial-
+// product tables and the `omp atomic` accumulation of the reference are kept
+// so that timing it measures the reference algorithm.  Boost is absent here,
+// so:
+//   * boost::math::spherical_harmonic is restated from its textbook
+//     definition (associated Legendre recurrence with Condon-Shortley phase,
+//     normalisation sqrt((2l+1)/(4 pi) (l-m)!/(l+m)!));
+//   * boost::geometry cartesian->spherical is acos(z/r), atan2(y,x) in long
+//     double, as the reference requests long double points;
+//   * boost::multi_array is replaced by flat std::vector storage.
+//
+// Function-by-function map (reference file:line -> here):
+//   DataMLIPBase::initialize           mlip_model.h:35-120          -> Potential::load
+//   Readgtinv::screening               mlip_read_gtinv.cpp:35-76    -> Potential::screening
+//   ModelParams ctor                   mlip_model_params.cpp:27-82  -> Potential::model_params
+//   ModelParams::uniq_gtinv_type       mlip_model_params.cpp:86-126 -> Potential::uniq_gtinv_type
+//   combination2/3_gtinv, check_type   mlip_model_params.cpp:128-170
+//   Polynomial ctor (model 1 combs)    mlip_polynomial.cpp:27-55
+//   PolynomialGtinv::set_uniq_lmt_prod mlip_polynomial_gtinv.cpp:60-84
+//   PolynomialGtinv::set_gtinv         mlip_polynomial_gtinv.cpp:86-123
+//   set_uniq_comb/set_polynomial_array mlip_polynomial_gtinv.cpp:125-174
+//   gtinv_extract                      mlip_polynomial_gtinv.cpp:176-194
+//   get_fn / gauss / cosine cutoff     mlip_features.cpp:26-52, mlip_basis_function.cpp:28-59
+//   get_lm_info_table                  mlip_features.cpp:54-62
+//   get_ylm / spherical_harmonic_dtheta mlip_features.cpp:64-119
+//   PairMLIPGtinv::compute and callees pair_mlip_gtinv.cpp:63-521
+//
+// Parity status of each part is recorded in the test suite headers; the
+// parts are pinned by the reference's unit-test known answers, the regression
+// log goldens, closed forms and invariants.
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <complex>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <set>
+#include <sstream>
+#include <string>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+using dc = std::complex<double>;
+using vec1i = std::vector<int>;
+using vec2i = std::vector<vec1i>;
+
+struct GtEntry {  // GtinvLammps, mlip_polynomial.h:53-60
+    int reg_i, lmt_pi;
+    double coeff;
+    int order, tc0, lm0;
+};
+struct PolyEntry {  // GtinvPolynomialLammps, mlip_polynomial.h:62-69
+    int c0, reg_i, comb_i, lmt_pi;
+    double coeff;
+    int order;
+};
+struct LinearTerm {  // LinearTermGtinv, mlip_model_params.h:88-92
+    int lmindex;
+    vec1i tcomb_index;
+    vec1i type1;
+};
+
+struct Irrep {
+    vec1i l;
+    vec2i m;
+    std::vector<double> c;
+};
+
+std::vector<Irrep> load_gtinv_table(const std::string& path, std::string& err) {
+    std::vector<Irrep> out;
+    std::ifstream in(path);
+    if (!in) {
+        err = "cannot open gtinv table " + path;
+        return out;
+    }
+    std::string tag;
+    int n = 0;
+    in >> tag >> n;
+    if (tag != "gtinv_table" || n <= 0) {
+        err = "bad gtinv table header";
+        return out;
+    }
+    out.resize(n);
+    for (int i = 0; i < n; ++i) {
+        int idx, q, nt;
+        in >> tag >> idx >> q;
+        if (tag != "irrep" || idx != i) {
+            err = "bad gtinv irrep header";
+            out.clear();
+            return out;
+        }
+        out[i].l.resize(q);
+        for (int k = 0; k < q; ++k) in >> out[i].l[k];
+        in >> nt;
+        out[i].m.assign(nt, vec1i(q));
+        out[i].c.resize(nt);
+        for (int t = 0; t < nt; ++t) {
+            for (int k = 0; k < q; ++k) in >> out[i].m[t][k];
+            std::string cs;
+            in >> cs;
+            out[i].c[t] = std::strtod(cs.c_str(), nullptr);
+        }
+    }
+    if (!in) {
+        err = "truncated gtinv table";
+        out.clear();
+    }
+    return out;
+}
+
+// ---------------------------------------------------------------- basis
+const double kPi = 3.141592653589793;  // mlip_basis_function.cpp:26
+
+double cosine_cutoff(double dis, double rc) {
+    return dis < rc ? 0.5 * (std::cos(kPi * dis / rc) + 1.0) : 0.0;
+}
+double cosine_cutoff_d(double dis, double rc) {
+    return dis < rc ? -0.5 * kPi / rc * std::sin(kPi * dis / rc) : 0.0;
+}
+double gauss(double x, double beta, double mu) {
+    return std::exp(-beta * std::pow(std::fabs(x - mu), 2.0));
+}
+
+// Textbook spherical harmonic with Condon-Shortley phase; negative m via
+// Y_l^{-m} = (-1)^m conj(Y_l^m).  Same conventions as the Boost routine the
+// reference calls at mlip_features.cpp:67,86.
+dc sph_harm(int l, int m, double theta, double phi) {
+    const int am = std::abs(m);
+    if (am > l) return 0.0;
+    const double x = std::cos(theta);
+    const double s = std::pow(std::fabs(std::sin(theta)), am);
+    // P_am^am = (-1)^am (2am-1)!! s
+    double pmm = 1.0;
+    for (int k = 1; k <= am; ++k) pmm *= -(2.0 * k - 1.0);
+    pmm *= s;
+    double p = pmm;
+    if (l > am) {
+        double p1 = x * (2.0 * am + 1.0) * pmm;
+        double p0 = pmm;
+        for (int ll = am + 2; ll <= l; ++ll) {
+            double p2 = ((2.0 * ll - 1.0) * x * p1 - (ll + am - 1.0) * p0) / (ll - am);
+            p0 = p1;
+            p1 = p2;
+        }
+        p = p1;
+    }
+    // (l-am)!/(l+am)!
+    double ratio = 1.0;
+    for (int k = l - am + 1; k <= l + am; ++k) ratio /= k;
+    const double pref = std::sqrt((2.0 * l + 1.0) / (4.0 * kPi) * ratio) * p;
+    dc y(pref * std::cos(am * phi), pref * std::sin(am * phi));
+    if (m < 0) {
+        y = std::conj(y);
+        if (am & 1) y = -y;
+    }
+    return y;
+}
+
+// boost::geometry::transform(cartesian point<long double,3> -> spherical
+// polar<radian>) as used at mlip_features.cpp:97-103: theta = acos(z/r),
+// phi = atan2(y, x), evaluated in long double.
+void cart_to_sph(double x, double y, double z, double& theta, double& phi) {
+    long double lx = x, ly = y, lz = z;
+    long double r = sqrtl(lx * lx + ly * ly + lz * lz);
+    phi = static_cast<double>(atan2l(ly, lx));
+    theta = (r > 0.0L || r < 0.0L) ? static_cast<double>(acosl(lz / r)) : 0.0;
+}
+
+// ---------------------------------------------------------------- model
+struct Potential {
+    std::vector<std::string> ele;
+    std::vector<double> mass;
+    int n_type = 0, n_fn = 0, model_type = 0, maxp = 0, maxl = 0;
+    double cutoff = 0.0;
+    std::string pair_type, des_type;
+    std::vector<std::array<double, 2>> params;
+    std::vector<double> reg_coeffs;  // x 2/scale (mlip_model.h:90)
+    // gtinv screening
+    vec2i l_comb;
+    std::vector<vec2i> lm_array;
+    std::vector<std::vector<double>> lm_coeffs;
+    vec2i lm_info;  // (l, m, lm1, lm2), m <= 0
+    // ModelParams
+    std::vector<std::vector<vec1i>> type_comb_pair;
+    std::vector<LinearTerm> linear;
+    int n_des = 0, n_coeff_all = 0;
+    vec2i comb2, comb3;
+    vec2i type_comb;  // [type1][type2] -> tc
+    // PolynomialGtinv
+    int n_tc = 0, n_lm_all = 0, n_gtinv = 0;
+    vec2i lmtc_map;
+    std::set<vec1i> uniq_lmt_prod_set, uniq_comb_set;
+    vec2i uniq_lmt_prod, uniq_comb;
+    std::vector<std::vector<std::vector<GtEntry>>> gtinv_array, gtinv_array_poly;
+    std::vector<std::vector<GtEntry>> gtinv_all;
+    std::vector<std::vector<std::vector<std::vector<PolyEntry>>>> poly_array;
+
+    bool check_type_comb_pair(const vec1i& idx, int type1) const {
+        size_t mn = SIZE_MAX;
+        for (int p : idx) mn = std::min(mn, type_comb_pair[p][type1].size());
+        return mn > 0;
+    }
+    bool check_type(const vec2i& type1_array) const {
+        for (int t = 0; t < n_type; ++t) {
+            bool ok = true;
+            for (const auto& t1 : type1_array)
+                if (std::find(t1.begin(), t1.end(), t) == t1.end()) {
+                    ok = false;
+                    break;
+                }
+            if (ok) return true;
+        }
+        return false;
+    }
+    int lmtc2seq(int lm, int tc) const { return lm * n_tc + tc; }
+
+    std::string load(const char* file, const std::vector<Irrep>& table);
+    void screening(int order, const vec1i& gmaxl, const std::vector<int>& gsym,
+                   const std::vector<Irrep>& table);
+    void uniq_gtinv_type();
+    void model_params();
+    void polynomial_gtinv();
+};
+
+template <typename T>
+bool read_one(std::ifstream& in, T& v) {
+    std::string line;
+    if (!std::getline(in, line)) return false;
+    std::stringstream ss(line);
+    return static_cast<bool>(ss >> v);
+}
+template <typename T>
+bool read_arr(std::ifstream& in, std::vector<T>& v, int n) {
+    std::string line;
+    if (!std::getline(in, line)) return false;
+    std::stringstream ss(line);
+    v.resize(n);
+    for (int i = 0; i < n; ++i)
+        if (!(ss >> v[i])) return false;
+    return true;
+}
+
+std::string Potential::load(const char* file, const std::vector<Irrep>& table) {
+    std::ifstream in(file);
+    if (!in) return std::string("cannot open potential file ") + file;
+    std::string line;
+    std::getline(in, line);
+    {
+        std::stringstream ss(line);
+        std::string tok;
+        while (ss >> tok) ele.push_back(tok);
+        if (ele.size() < 3) return "element line must end with two comment tokens";
+        ele.resize(ele.size() - 2);
+    }
+    n_type = static_cast<int>(ele.size());
+    if (!read_one(in, cutoff) || !read_one(in, pair_type) || !read_one(in, des_type) ||
+        !read_one(in, model_type) || !read_one(in, maxp) || !read_one(in, maxl))
+        return "truncated potential header";
+    if (pair_type != "gaussian") return "unsupported pair_type " + pair_type;
+    if (des_type != "gtinv") return "des_type must be gtinv";
+    int gorder;
+    std::vector<int> gmaxl, gsym;
+    if (!read_one(in, gorder) || gorder < 1) return "bad gtinv_order";
+    if (!read_arr(in, gmaxl, gorder - 1) || !read_arr(in, gsym, gorder - 1))
+        return "bad gtinv_maxl/gtinv_sym";
+    screening(gorder, gmaxl, gsym, table);
+    lm_info.clear();
+    for (int l = 0; l <= maxl; ++l)
+        for (int m = -l; m < 1; ++m) lm_info.push_back({l, m, l * l + l + m, l * l + l - m});
+    int nreg;
+    if (!read_one(in, nreg) || nreg <= 0) return "bad coefficient count";
+    std::vector<double> scale;
+    if (!read_arr(in, reg_coeffs, nreg) || !read_arr(in, scale, nreg))
+        return "truncated coefficient/scale lines";
+    for (int i = 0; i < nreg; ++i) reg_coeffs[i] *= 2.0 / scale[i];
+    int npar;
+    if (!read_one(in, npar) || npar <= 0) return "bad radial parameter count";
+    params.resize(npar);
+    for (int i = 0; i < npar; ++i) {
+        std::vector<double> p;
+        if (!read_arr(in, p, 2)) return "truncated radial parameters";
+        params[i] = {p[0], p[1]};
+    }
+    if (!read_arr(in, mass, n_type)) return "truncated masses";
+    n_fn = npar;
+    if (n_type > 2) return "n_type > 2 is not supported by the reference model";
+    model_params();
+    if (n_coeff_all != nreg) {
+        char buf[160];
+        std::snprintf(buf, sizeof buf, "coefficient count %d does not match model (%d)", nreg,
+                      n_coeff_all);
+        return buf;
+    }
+    polynomial_gtinv();
+    type_comb.assign(n_type, vec1i(n_type, 0));
+    for (int t1 = 0; t1 < n_type; ++t1)
+        for (int t2 = 0; t2 < n_type; ++t2)
+            for (size_t i = 0; i < type_comb_pair.size(); ++i) {
+                const auto& tc = type_comb_pair[i];
+                if (!tc[t1].empty() && tc[t1][0] == t2) {
+                    type_comb[t1][t2] = static_cast<int>(i);
+                    break;
+                }
+            }
+    return "";
+}
+
+void Potential::screening(int order, const vec1i& gmaxl, const std::vector<int>& gsym,
+                          const std::vector<Irrep>& table) {
+    for (const auto& irr : table) {
+        const int q = static_cast<int>(irr.l.size());
+        bool req = true;
+        if (q > 1) {
+            if (q > order || irr.l.back() > gmaxl[q - 2]) req = false;
+            // the reference also reads gtinv_sym[q-2] when q > order; the
+            // irrep is already rejected then, so only q <= order matters.
+            if (q <= order && gsym[q - 2]) {
+                int ne = static_cast<int>(std::count(irr.l.begin(), irr.l.end(), irr.l[0]));
+                if (ne != q) req = false;
+            }
+        }
+        if (!req) continue;
+        vec2i v(irr.m.size(), vec1i(q));
+        for (size_t j = 0; j < irr.m.size(); ++j)
+            for (int k = 0; k < q; ++k) v[j][k] = irr.l[k] * irr.l[k] + irr.l[k] + irr.m[j][k];
+        l_comb.push_back(irr.l);
+        lm_array.push_back(v);
+        lm_coeffs.push_back(irr.c);
+    }
+}
+
+void permutenr(const vec1i& input, vec1i output, std::vector<vec1i>& all, size_t r) {
+    if (output.size() == r) {
+        all.push_back(output);
+        return;
+    }
+    for (int v : input) {
+        vec1i t = output;
+        t.push_back(v);
+        permutenr(input, t, all, r);
+    }
+}
+
+void Potential::uniq_gtinv_type() {
+    vec1i pinput(type_comb_pair.size());
+    for (size_t i = 0; i < pinput.size(); ++i) pinput[i] = static_cast<int>(i);
+    const int gorder = static_cast<int>(l_comb.back().size());
+    std::vector<std::vector<vec1i>> perm_array(gorder);
+    for (int i = 0; i < gorder; ++i) {
+        std::vector<vec1i> perm;
+        permutenr(pinput, {}, perm, i + 1);
+        for (const auto& p1 : perm)
+            for (int t1 = 0; t1 < n_type; ++t1)
+                if (check_type_comb_pair(p1, t1)) {
+                    perm_array[i].push_back(p1);
+                    break;
+                }
+    }
+    for (size_t i = 0; i < l_comb.size(); ++i) {
+        const vec1i& lc = l_comb[i];
+        std::set<std::multiset<std::pair<int, int>>> uniq;
+        for (const auto& p : perm_array[lc.size() - 1]) {
+            std::multiset<std::pair<int, int>> tmp;
+            for (size_t j = 0; j < p.size(); ++j) tmp.insert({lc[j], p[j]});
+            uniq.insert(tmp);
+        }
+        for (const auto& lt : uniq) {
+            vec1i tc, t1a;
+            for (const auto& x : lt) tc.push_back(x.second);
+            for (int t1 = 0; t1 < n_type; ++t1)
+                if (check_type_comb_pair(tc, t1)) t1a.push_back(t1);
+            linear.push_back({static_cast<int>(i), tc, t1a});
+        }
+    }
+}
+
+void Potential::model_params() {
+    if (n_type == 1)
+        type_comb_pair = {{{0}}};
+    else
+        type_comb_pair = {{{0}, {}}, {{1}, {0}}, {{}, {1}}};
+    uniq_gtinv_type();
+    n_gtinv = static_cast<int>(linear.size());
+    n_des = n_fn * n_gtinv;
+    vec1i pidx;
+    if (model_type == 2) {
+        for (int n = 0; n < n_des; ++n) pidx.push_back(n);
+    } else if (model_type == 3 || model_type == 4) {
+        for (int i = 0; i < n_gtinv; ++i) {
+            const size_t q = linear[i].tcomb_index.size();
+            if ((model_type == 3 && q == 1) || (model_type == 4 && q < 3))
+                for (int n = 0; n < n_fn; ++n) pidx.push_back(n * n_gtinv + i);
+        }
+        std::sort(pidx.begin(), pidx.end());
+    }
+    if (model_type == 1) {
+        n_coeff_all = n_des * maxp;
+        // Polynomial ctor, mlip_polynomial.cpp:36-40
+        for (int n = 0; n < n_des; ++n) {
+            if (maxp > 1) comb2.push_back({n, n});
+            if (maxp > 2) comb3.push_back({n, n, n});
+        }
+        return;
+    }
+    auto seq2ig = [&](int s) { return s % n_gtinv; };
+    if (maxp > 1)
+        for (size_t i1 = 0; i1 < pidx.size(); ++i1)
+            for (size_t i2 = 0; i2 <= i1; ++i2)
+                if (check_type({linear[seq2ig(pidx[i1])].type1, linear[seq2ig(pidx[i2])].type1}))
+                    comb2.push_back({pidx[i2], pidx[i1]});
+    if (maxp > 2)
+        for (size_t i1 = 0; i1 < pidx.size(); ++i1)
+            for (size_t i2 = 0; i2 <= i1; ++i2)
+                for (size_t i3 = 0; i3 <= i2; ++i3)
+                    if (check_type({linear[seq2ig(pidx[i1])].type1, linear[seq2ig(pidx[i2])].type1,
+                                    linear[seq2ig(pidx[i3])].type1}))
+                        comb3.push_back({pidx[i3], pidx[i2], pidx[i1]});
+    n_coeff_all = n_des + static_cast<int>(comb2.size() + comb3.size());
+}
+
+vec2i permutation(int order) {  // Polynomial::permutation, mlip_polynomial.cpp:122-131
+    vec2i s(order);
+    for (int i = 0; i < order; ++i) {
+        s[i].push_back(i);
+        for (int j = 0; j < order; ++j)
+            if (j != i) s[i].push_back(j);
+    }
+    return s;
+}
+
+int find_in_set(const std::set<vec1i>& st, vec1i v) {
+    std::sort(v.begin(), v.end());
+    auto it = st.find(v);
+    return static_cast<int>(std::distance(st.begin(), it));
+}
+
+void Potential::polynomial_gtinv() {
+    n_tc = static_cast<int>(type_comb_pair.size());
+    n_lm_all = 2 * static_cast<int>(lm_info.size()) - maxl - 1;
+    for (int i = 0; i < n_lm_all; ++i)
+        for (int j = 0; j < n_tc; ++j) lmtc_map.push_back({i, j});
+    // set_uniq_lmt_prod
+    for (const auto& lin : linear) {
+        const vec1i& tc = lin.tcomb_index;
+        const vec2i& lma = lm_array[lin.lmindex];
+        const int order = static_cast<int>(tc.size());
+        const vec2i seq = permutation(order);
+        for (int j = 0; j < order; ++j)
+            for (const auto& row : lma) {
+                vec1i info;
+                for (int k2 = 1; k2 < order; ++k2) {
+                    int s = seq[j][k2];
+                    info.push_back(lmtc2seq(row[s], tc[s]));
+                }
+                std::sort(info.begin(), info.end());
+                if (!info.empty()) uniq_lmt_prod_set.insert(info);
+            }
+    }
+    // set_gtinv
+    gtinv_array.assign(n_tc, std::vector<std::vector<GtEntry>>(n_lm_all));
+    gtinv_all.assign(linear.size(), {});
+    for (size_t i1 = 0; i1 < linear.size(); ++i1) {
+        const vec1i& tc = linear[i1].tcomb_index;
+        const vec2i& lma = lm_array[linear[i1].lmindex];
+        const auto& coeffs = lm_coeffs[linear[i1].lmindex];
+        const int order = static_cast<int>(tc.size());
+        const vec2i seq = permutation(order);
+        for (int j = 0; j < order; ++j) {
+            const int s0 = seq[j][0], tc0 = tc[s0];
+            for (size_t k1 = 0; k1 < lma.size(); ++k1) {
+                const int lm0 = lma[k1][s0];
+                vec1i info;
+                for (int k2 = 1; k2 < order; ++k2) {
+                    int s = seq[j][k2];
+                    info.push_back(lmtc2seq(lma[k1][s], tc[s]));
+                }
+                const int lmt_pi = info.empty() ? -1 : find_in_set(uniq_lmt_prod_set, info);
+                GtEntry gt{static_cast<int>(i1), lmt_pi, coeffs[k1], order, tc0, lm0};
+                gtinv_array[tc0][lm0].push_back(gt);
+                gtinv_all[i1].push_back(gt);
+            }
+        }
+    }
+    uniq_lmt_prod.assign(uniq_lmt_prod_set.begin(), uniq_lmt_prod_set.end());
+    if (maxp <= 1) return;
+    poly_array.assign(n_tc, std::vector<std::vector<std::vector<PolyEntry>>>(
+                                n_fn, std::vector<std::vector<PolyEntry>>(n_lm_all)));
+    gtinv_array_poly.assign(n_tc, std::vector<std::vector<GtEntry>>(n_lm_all));
+    auto uniq_comb_add = [&](const vec2i& comb) {
+        if (comb.empty()) return;
+        const int order = static_cast<int>(comb[0].size());
+        const vec2i seq = permutation(order);
+        vec1i info(order - 1);
+        for (const auto& c : comb)
+            for (const auto& s : seq) {
+                for (int i = 1; i < order; ++i) info[i - 1] = c[s[i]];
+                uniq_comb_set.insert(info);
+            }
+    };
+    uniq_comb_add(comb2);
+    uniq_comb_add(comb3);
+    auto set_poly = [&](const vec2i& comb, int sindex) {
+        if (comb.empty()) return;
+        const int order = static_cast<int>(comb[0].size());
+        const vec2i seq = permutation(order);
+        int reg_i = sindex;
+        vec1i info(order - 1);
+        for (const auto& c : comb) {
+            int poly_order = 0;
+            for (int ci : c) poly_order += gtinv_all[ci % n_gtinv][0].order;
+            for (const auto& s : seq) {
+                for (int i = 1; i < order; ++i) info[i - 1] = c[s[i]];
+                const int cindex = find_in_set(uniq_comb_set, info);
+                const int c0 = c[s[0]];
+                const int n0 = c0 / n_gtinv, ig0 = c0 % n_gtinv;
+                for (const auto& gt : gtinv_all[ig0])
+                    poly_array[gt.tc0][n0][gt.lm0].push_back(
+                        {c0, reg_i, cindex, gt.lmt_pi, gt.coeff, poly_order});
+            }
+            ++reg_i;
+        }
+    };
+    set_poly(comb2, n_des);
+    set_poly(comb3, n_des + static_cast<int>(comb2.size()));
+    // gtinv_extract
+    std::set<int> uniq_ig;
+    for (const auto& u : uniq_comb_set)
+        for (int c : u) uniq_ig.insert(c % n_gtinv);
+    for (int tc0 = 0; tc0 < n_tc; ++tc0)
+        for (int lm0 = 0; lm0 < n_lm_all; ++lm0)
+            for (const auto& gt : gtinv_array[tc0][lm0])
+                if (uniq_ig.count(gt.reg_i)) gtinv_array_poly[tc0][lm0].push_back(gt);
+    uniq_comb.assign(uniq_comb_set.begin(), uniq_comb_set.end());
+}
+
+// ---------------------------------------------------------------- compute
+struct Oracle {
+    Potential pot;
+    std::vector<int> types_by_tag;  // element index of tag-1 (pair_mlip_gtinv.cpp:590-592)
+};
+
+void get_fn(const Potential& p, double dis, std::vector<double>& fn) {
+    const double fc = cosine_cutoff(dis, p.cutoff);
+    fn.resize(p.n_fn);
+    for (int n = 0; n < p.n_fn; ++n) fn[n] = gauss(dis, p.params[n][0], p.params[n][1]) * fc;
+}
+void get_fn_d(const Potential& p, double dis, std::vector<double>& fn, std::vector<double>& fnd) {
+    const double fc = cosine_cutoff(dis, p.cutoff), fcd = cosine_cutoff_d(dis, p.cutoff);
+    fn.resize(p.n_fn);
+    fnd.resize(p.n_fn);
+    for (int n = 0; n < p.n_fn; ++n) {
+        const double b = gauss(dis, p.params[n][0], p.params[n][1]);
+        const double bd = -2.0 * p.params[n][0] * (dis - p.params[n][1]) * b;
+        fn[n] = b * fc;
+        fnd[n] = bd * fc + b * fcd;
+    }
+}
+void get_ylm(const Potential& p, double th, double ph, std::vector<dc>& ylm) {
+    ylm.resize(p.lm_info.size());
+    for (size_t k = 0; k < p.lm_info.size(); ++k) ylm[k] = sph_harm(p.lm_info[k][0], p.lm_info[k][1], th, ph);
+}
+void get_ylm_d(const Potential& p, double th, double ph, std::vector<dc>& ylm, std::vector<dc>& yd) {
+    get_ylm(p, th, ph, ylm);
+    const double tan_theta = static_cast<double>(tanl(th));
+    const double tinv = 1.0 / tan_theta;
+    const dc eip = std::exp(-dc(0.0, 1.0) * ph);
+    yd.resize(ylm.size());
+    for (size_t k = 0; k < p.lm_info.size(); ++k) {
+        const int l = p.lm_info[k][0], m = p.lm_info[k][1];
+        if (m < 0) {
+            dc v = std::sqrt(double((l - m) * (l + m + 1))) * ylm[k + 1] * eip;
+            if (!std::isinf(tinv)) v += double(m) * ylm[k] * tinv;
+            yd[k] = v;
+        } else if (l > 0) {
+            yd[k] = -std::sqrt(double(l * (l + 1))) * ylm[k - 1] * std::conj(eip);
+        } else {
+            yd[k] = 0.0;
+        }
+    }
+}
+inline double prod_real(const dc& a, const dc& b) { return a.real() * b.real() - a.imag() * b.imag(); }
+
+struct List {
+    int inum;
+    const int* ilist;
+    const int* off;  // firstneigh[ilist[ii]] == idx + off[ii]
+    const int* idx;
+    const int* numneigh;  // by local index i
+};
+
+struct ComputeOut {
+    double* f;
+    double eng;
+    double virial[6];
+    double* eatom;
+    double* anlm_out;   // optional [inum][n_type][n_fn][n_lm_all][2]
+    double* feat_out;   // optional [inum][n_des]
+};
+
+void run_compute(const Oracle& o, int nlocal, const double* x, const int64_t* tag, const List& L,
+                 ComputeOut& out) {
+    const Potential& p = o.pot;
+    const int inum = L.inum;
+    const int n_fn = p.n_fn, n_lm = static_cast<int>(p.lm_info.size());
+    const int n_lm_all = 2 * n_lm - p.maxl - 1, n_type = p.n_type, n_tc = p.n_tc;
+    auto T = [&](int i) { return o.types_by_tag[tag[i] - 1]; };
+    auto X = [&](int i, int k) { return x[3 * i + k]; };
+    auto nb = [&](int ii) { return L.idx + L.off[ii]; };
+
+    // ---- compute_anlm (pair_mlip_gtinv.cpp:349-447)
+    std::vector<double> ar(size_t(inum) * n_type * n_fn * n_lm, 0.0), ai(ar.size(), 0.0);
+    auto AI = [&](int t, int s, int n, int lm) {
+        return ((size_t(t) * n_type + s) * n_fn + n) * n_lm + lm;
+    };
+#pragma omp parallel for schedule(guided)
+    for (int ii = 0; ii < inum; ++ii) {
+        const int i = L.ilist[ii];
+        const int t1 = T(i);
+        const int jnum = L.numneigh[i];
+        const int* jl = nb(ii);
+        std::vector<double> fn;
+        std::vector<dc> ylm;
+        for (int jj = 0; jj < jnum; ++jj) {
+            const int j = jl[jj];
+            const double dx = X(i, 0) - X(j, 0), dy = X(i, 1) - X(j, 1), dz = X(i, 2) - X(j, 2);
+            const double dis = std::sqrt(dx * dx + dy * dy + dz * dz);
+            if (!(dis < p.cutoff)) continue;
+            const int t2 = T(j);
+            double th, ph;
+            cart_to_sph(dx, dy, dz, th, ph);
+            get_fn(p, dis, fn);
+            get_ylm(p, th, ph, ylm);
+            const int ti = static_cast<int>(tag[i] - 1), tj = static_cast<int>(tag[j] - 1);
+            for (int n = 0; n < n_fn; ++n)
+                for (int lm = 0; lm < n_lm; ++lm) {
+                    const double sc = (p.lm_info[lm][0] % 2 == 0) ? 1.0 : -1.0;
+                    const dc v = fn[n] * ylm[lm];
+#pragma omp atomic
+                    ar[AI(ti, t2, n, lm)] += v.real();
+#pragma omp atomic
+                    ar[AI(tj, t1, n, lm)] += v.real() * sc;
+#pragma omp atomic
+                    ai[AI(ti, t2, n, lm)] += v.imag();
+#pragma omp atomic
+                    ai[AI(tj, t1, n, lm)] += v.imag() * sc;
+                }
+        }
+    }
+    std::vector<dc> anlm(size_t(inum) * n_type * n_fn * n_lm_all);
+    auto AA = [&](int t, int s, int n, int lm) {
+        return ((size_t(t) * n_type + s) * n_fn + n) * n_lm_all + lm;
+    };
+#pragma omp parallel for schedule(guided)
+    for (int ii = 0; ii < inum; ++ii)
+        for (int s = 0; s < n_type; ++s)
+            for (int n = 0; n < n_fn; ++n)
+                for (int lm = 0; lm < n_lm; ++lm) {
+                    const int m = p.lm_info[lm][1], lm1 = p.lm_info[lm][2], lm2 = p.lm_info[lm][3];
+                    const double r = ar[AI(ii, s, n, lm)], im = ai[AI(ii, s, n, lm)];
+                    anlm[AA(ii, s, n, lm1)] = dc(r, im);
+                    const double cc = std::pow(-1.0, m);
+                    anlm[AA(ii, s, n, lm2)] = dc(cc * r, -cc * im);
+                }
+    if (out.anlm_out)
+        for (size_t k = 0; k < anlm.size(); ++k) {
+            out.anlm_out[2 * k] = anlm[k].real();
+            out.anlm_out[2 * k + 1] = anlm[k].imag();
+        }
+
+    // ---- compute_partial_anlm_product_for_each_atom (:260-296)
+    const size_t pstride = size_t(n_fn) * n_lm_all;
+    std::vector<dc> prod_f(size_t(n_tc) * inum * pstride), prod_e(prod_f.size());
+    const int n_uniq = static_cast<int>(p.uniq_lmt_prod.size());
+#pragma omp parallel for schedule(guided)
+    for (int ii = 0; ii < inum; ++ii) {
+        const int i = L.ilist[ii];
+        const int t1 = T(i);
+        const int ti = static_cast<int>(tag[i] - 1);
+        auto a_of = [&](int s, int n, int lm) { return anlm[AA(ti, s, n, lm)]; };
+        // compute_anlm_uniq_products (:451-478)
+        std::vector<dc> uniq(size_t(n_fn) * n_uniq, dc(1.0, 0.0));
+        for (int u = 0; u < n_uniq; ++u)
+            for (int seq : p.uniq_lmt_prod[u]) {
+                const int lm = p.lmtc_map[seq][0], tc = p.lmtc_map[seq][1];
+                if (!p.type_comb_pair[tc][t1].empty()) {
+                    const int t2 = p.type_comb_pair[tc][t1][0];
+                    for (int n = 0; n < n_fn; ++n) uniq[size_t(n) * n_uniq + u] *= a_of(t2, n, lm);
+                } else {
+                    for (int n = 0; n < n_fn; ++n) uniq[size_t(n) * n_uniq + u] = 0.0;
+                    break;
+                }
+            }
+        // compute_polynomial_model_uniq_products (:481-521)
+        std::vector<double> uniq_p;
+        std::vector<double> dn;
+        if (p.maxp > 1 || out.feat_out) {
+            dn.assign(p.n_des, 0.0);
+            for (int t2 = 0; t2 < n_type; ++t2) {
+                const int tc0 = p.type_comb[t1][t2];
+                for (int lm0 = 0; lm0 < n_lm_all; ++lm0) {
+                    const auto& lst = (p.maxp > 1) ? p.gtinv_array_poly[tc0][lm0] : p.gtinv_array[tc0][lm0];
+                    for (const auto& t : lst) {
+                        if (t.lmt_pi == -1) {
+                            for (int n = 0; n < n_fn; ++n)
+                                dn[n * p.n_gtinv + t.reg_i] += a_of(t2, n, 0).real();
+                        } else {
+                            for (int n = 0; n < n_fn; ++n)
+                                dn[n * p.n_gtinv + t.reg_i] +=
+                                    t.coeff / t.order *
+                                    prod_real(a_of(t2, n, lm0), uniq[size_t(n) * n_uniq + t.lmt_pi]);
+                        }
+                    }
+                }
+            }
+            if (p.maxp > 1) {
+                uniq_p.assign(p.uniq_comb.size(), 0.5);
+                for (size_t c = 0; c < p.uniq_comb.size(); ++c)
+                    for (int f : p.uniq_comb[c]) uniq_p[c] *= dn[f];
+            }
+        }
+        if (out.feat_out) {
+            // full feature vector for diagnostics (all itc, not only the
+            // polynomial subset): recompute with gtinv_array
+            std::vector<double> d(p.n_des, 0.0);
+            for (int t2 = 0; t2 < n_type; ++t2) {
+                const int tc0 = p.type_comb[t1][t2];
+                for (int lm0 = 0; lm0 < n_lm_all; ++lm0)
+                    for (const auto& t : p.gtinv_array[tc0][lm0]) {
+                        if (t.lmt_pi == -1) {
+                            for (int n = 0; n < n_fn; ++n) d[n * p.n_gtinv + t.reg_i] += a_of(t2, n, 0).real();
+                        } else {
+                            for (int n = 0; n < n_fn; ++n)
+                                d[n * p.n_gtinv + t.reg_i] +=
+                                    t.coeff / t.order *
+                                    prod_real(a_of(t2, n, lm0), uniq[size_t(n) * n_uniq + t.lmt_pi]);
+                        }
+                    }
+            }
+            std::copy(d.begin(), d.end(), out.feat_out + size_t(ii) * p.n_des);
+        }
+        for (int t2 = 0; t2 < n_type; ++t2) {
+            const int tc0 = p.type_comb[t1][t2];
+            for (int n = 0; n < n_fn; ++n)
+                for (int lm0 = 0; lm0 < n_lm_all; ++lm0) {
+                    dc sumf(0.0), sume(0.0);
+                    // compute_anlm_linear_term (:300-317)
+                    for (const auto& inv : p.gtinv_array[tc0][lm0]) {
+                        const double regc = 0.5 * p.reg_coeffs[n * p.n_gtinv + inv.reg_i];
+                        if (inv.lmt_pi != -1) {
+                            const dc v = regc * inv.coeff * uniq[size_t(n) * n_uniq + inv.lmt_pi];
+                            sumf += v;
+                            sume += dc(v.real() / inv.order, v.imag() / inv.order);
+                        } else {
+                            sumf += regc;
+                            sume += regc;
+                        }
+                    }
+                    // compute_anlm_polynomial_model_correction (:321-343)
+                    if (p.maxp > 1)
+                        for (const auto& pi : p.poly_array[tc0][n][lm0]) {
+                            const double regc = p.reg_coeffs[pi.reg_i] * uniq_p[pi.comb_i];
+                            if (pi.lmt_pi != -1) {
+                                const dc v = regc * pi.coeff * uniq[size_t(n) * n_uniq + pi.lmt_pi];
+                                sumf += v;
+                                sume += dc(v.real() / pi.order, v.imag() / pi.order);
+                            } else {
+                                sumf += regc;
+                                sume += regc / pi.order;
+                            }
+                        }
+                    const size_t k = (size_t(tc0) * inum + ti) * pstride + size_t(n) * n_lm_all + lm0;
+                    prod_f[k] = sumf;
+                    prod_e[k] = sume;
+                }
+        }
+    }
+
+    // ---- compute_energy_and_force_for_each_atom (:161-256)
+    std::vector<double> scales;
+    for (int l = 0; l <= p.maxl; ++l)
+        for (int m = -l; m <= l; ++m) scales.push_back(l % 2 == 0 ? 1.0 : -1.0);
+    std::vector<std::vector<double>> ev(inum), fxa(inum), fya(inum), fza(inum);
+    for (int ii = 0; ii < inum; ++ii) {
+        const int jn = L.numneigh[L.ilist[ii]];
+        ev[ii].assign(jn, 0.0);
+        fxa[ii].assign(jn, 0.0);
+        fya[ii].assign(jn, 0.0);
+        fza[ii].assign(jn, 0.0);
+    }
+#pragma omp parallel for schedule(guided)
+    for (int ii = 0; ii < inum; ++ii) {
+        const int i = L.ilist[ii];
+        const int t1 = T(i);
+        const int jnum = L.numneigh[i];
+        const int* jl = nb(ii);
+        std::vector<double> fn, fnd;
+        std::vector<dc> ylm, yd;
+        std::vector<dc> fy(size_t(n_fn) * n_lm_all), fdx(fy.size()), fdy(fy.size()), fdz(fy.size());
+        for (int jj = 0; jj < jnum; ++jj) {
+            const int j = jl[jj];
+            const double dx = X(i, 0) - X(j, 0), dy = X(i, 1) - X(j, 1), dz = X(i, 2) - X(j, 2);
+            const double dis = std::sqrt(dx * dx + dy * dy + dz * dz);
+            if (!(dis < p.cutoff)) continue;
+            const int t2 = T(j);
+            double th, ph;
+            cart_to_sph(dx, dy, dz, th, ph);
+            get_fn_d(p, dis, fn, fnd);
+            get_ylm_d(p, th, ph, ylm, yd);
+            const double ct = std::cos(th), st = std::sin(th), cp = std::cos(ph), sp = std::sin(ph);
+            const double coeff = std::fabs(st) > 1e-15 ? 1.0 / st : 0.0;
+            for (int lm = 0; lm < n_lm; ++lm) {
+                const int m = p.lm_info[lm][1], lm1 = p.lm_info[lm][2], lm2 = p.lm_info[lm][3];
+                const double cc = std::pow(-1.0, m);
+                const dc ydphi = dc(0.0, 1.0) * double(m) * ylm[lm];
+                const dc term1 = yd[lm] * ct, term2 = coeff * ydphi;
+                const dc d0 = term1 * cp - term2 * sp, d1 = term1 * sp + term2 * cp, d2 = -yd[lm] * st;
+                for (int n = 0; n < n_fn; ++n) {
+                    const size_t b = size_t(n) * n_lm_all;
+                    fy[b + lm1] = fn[n] * ylm[lm];
+                    fy[b + lm2] = cc * std::conj(fy[b + lm1]);
+                    const dc f1 = fnd[n] * ylm[lm];
+                    fdx[b + lm1] = -(f1 * dx + fn[n] * d0) / dis;
+                    fdx[b + lm2] = cc * std::conj(fdx[b + lm1]);
+                    fdy[b + lm1] = -(f1 * dy + fn[n] * d1) / dis;
+                    fdy[b + lm2] = cc * std::conj(fdy[b + lm1]);
+                    fdz[b + lm1] = -(f1 * dz + fn[n] * d2) / dis;
+                    fdz[b + lm2] = cc * std::conj(fdz[b + lm1]);
+                }
+            }
+            const int tc0 = p.type_comb[t1][t2];
+            const size_t bi = (size_t(tc0) * inum + (tag[i] - 1)) * pstride;
+            const size_t bj = (size_t(tc0) * inum + (tag[j] - 1)) * pstride;
+            double e = 0.0, fx = 0.0, fyy = 0.0, fz = 0.0;
+            for (int n = 0; n < n_fn; ++n)
+                for (int lm0 = 0; lm0 < n_lm_all; ++lm0) {
+                    const size_t k = size_t(n) * n_lm_all + lm0;
+                    const dc se = prod_e[bi + k] + prod_e[bj + k] * scales[lm0];
+                    const dc sf = prod_f[bi + k] + prod_f[bj + k] * scales[lm0];
+                    e += prod_real(fy[k], se);
+                    fx += prod_real(fdx[k], sf);
+                    fyy += prod_real(fdy[k], sf);
+                    fz += prod_real(fdz[k], sf);
+                }
+            ev[ii][jj] = e;
+            fxa[ii][jj] = fx;
+            fya[ii][jj] = fyy;
+            fza[ii][jj] = fz;
+        }
+    }
+    // ---- accumulate_energy_and_force_for_all_atom (:127-159) with
+    // ev_tally_xyz semantics for newton_pair = 1
+    out.eng = 0.0;
+    for (double& v : out.virial) v = 0.0;
+    for (int ii = 0; ii < inum; ++ii) {
+        const int i = L.ilist[ii];
+        const int jnum = L.numneigh[i];
+        const int* jl = nb(ii);
+        for (int jj = 0; jj < jnum; ++jj) {
+            const int j = jl[jj];
+            const double dx = X(i, 0) - X(j, 0), dy = X(i, 1) - X(j, 1), dz = X(i, 2) - X(j, 2);
+            const double dis = std::sqrt(dx * dx + dy * dy + dz * dz);
+            if (!(dis < p.cutoff)) continue;
+            const double e = ev[ii][jj], fx = fxa[ii][jj], fy = fya[ii][jj], fz = fza[ii][jj];
+            out.f[3 * i] += fx;
+            out.f[3 * i + 1] += fy;
+            out.f[3 * i + 2] += fz;
+            out.f[3 * j] -= fx;
+            out.f[3 * j + 1] -= fy;
+            out.f[3 * j + 2] -= fz;
+            out.eng += e;
+            if (out.eatom) {
+                out.eatom[i] += 0.5 * e;
+                out.eatom[j] += 0.5 * e;
+            }
+            out.virial[0] += dx * fx;
+            out.virial[1] += dy * fy;
+            out.virial[2] += dz * fz;
+            out.virial[3] += dx * fy;
+            out.virial[4] += dx * fz;
+            out.virial[5] += dy * fz;
+        }
+    }
+    (void)nlocal;
+}
+
+void set_err(char* err, int len, const std::string& msg) {
+    if (err && len > 0) {
+        std::snprintf(err, static_cast<size_t>(len), "%s", msg.c_str());
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+void* oracle_create(const char* potfile, const char* table_path, char* err, int errlen) {
+    std::string e;
+    auto table = load_gtinv_table(table_path, e);
+    if (!e.empty()) {
+        set_err(err, errlen, e);
+        return nullptr;
+    }
+    auto* o = new Oracle();
+    e = o->pot.load(potfile, table);
+    if (!e.empty()) {
+        set_err(err, errlen, e);
+        delete o;
+        return nullptr;
+    }
+    return o;
+}
+
+void oracle_destroy(void* h) { delete static_cast<Oracle*>(h); }
+
+// info: n_type, n_fn, n_gtinv, n_des, n_coeff_all, maxl, n_lm_all, n_uniq_prod,
+//       n_comb2, n_comb3, n_uniq_comb, model_type, maxp, n_irreps
+int oracle_info(void* h, int* info) {
+    const auto& p = static_cast<Oracle*>(h)->pot;
+    int v[14] = {p.n_type, p.n_fn, p.n_gtinv, p.n_des, p.n_coeff_all, p.maxl, p.n_lm_all,
+                 static_cast<int>(p.uniq_lmt_prod.size()), static_cast<int>(p.comb2.size()),
+                 static_cast<int>(p.comb3.size()), static_cast<int>(p.uniq_comb.size()), p.model_type,
+                 p.maxp, static_cast<int>(p.l_comb.size())};
+    std::memcpy(info, v, sizeof v);
+    return 0;
+}
+
+double oracle_cutoff(void* h) { return static_cast<Oracle*>(h)->pot.cutoff; }
+
+int oracle_masses(void* h, double* out) {
+    const auto& p = static_cast<Oracle*>(h)->pot;
+    for (int i = 0; i < p.n_type; ++i) out[i] = p.mass[i];
+    return p.n_type;
+}
+
+// Linear-term table: per itc, irreps index, order and the tc list (padded to
+// 6 with -1) and the type1 mask -> out[itc*9 + ...]
+int oracle_linear_terms(void* h, int* out) {
+    const auto& p = static_cast<Oracle*>(h)->pot;
+    for (size_t i = 0; i < p.linear.size(); ++i) {
+        int* r = out + 9 * i;
+        r[0] = p.linear[i].lmindex;
+        r[1] = static_cast<int>(p.linear[i].tcomb_index.size());
+        for (int k = 0; k < 6; ++k)
+            r[2 + k] = k < r[1] ? p.linear[i].tcomb_index[k] : -1;
+        int mask = 0;
+        for (int t : p.linear[i].type1) mask |= 1 << t;
+        r[8] = mask;
+    }
+    return static_cast<int>(p.linear.size());
+}
+
+// comb tables: order 2 -> out[2*k], order 3 -> out[3*k]
+int oracle_combs(void* h, int order, int* out) {
+    const auto& p = static_cast<Oracle*>(h)->pot;
+    const auto& c = order == 2 ? p.comb2 : p.comb3;
+    for (size_t k = 0; k < c.size(); ++k)
+        for (int q = 0; q < order; ++q) out[order * k + q] = c[k][q];
+    return static_cast<int>(c.size());
+}
+
+void oracle_set_types_by_tag(void* h, int n, const int* elem_of_tag) {
+    auto* o = static_cast<Oracle*>(h);
+    o->types_by_tag.assign(elem_of_tag, elem_of_tag + n);
+}
+
+// LAMMPS Pair::compute contract restated for a half (or any) neighbor list
+// given as ilist/numneigh plus a flattened firstneigh (per-ilist offsets).
+// f (nall*3) is accumulated; eng and virial (xx,yy,zz,xy,xz,yz) overwritten.
+int oracle_compute(void* h, int nlocal, int nall, const double* x, const int64_t* tag, int inum,
+                   const int* ilist, const int* numneigh, const int* neigh_off, const int* neigh_idx,
+                   double* f, double* eng, double* virial, double* eatom, double* anlm_out,
+                   double* feat_out) {
+    auto* o = static_cast<Oracle*>(h);
+    if (static_cast<int>(o->types_by_tag.size()) < nlocal) return -1;
+    (void)nall;
+    List L{inum, ilist, neigh_off, neigh_idx, numneigh};
+    ComputeOut out{f, 0.0, {0, 0, 0, 0, 0, 0}, eatom, anlm_out, feat_out};
+    run_compute(*o, nlocal, x, tag, L, out);
+    *eng = out.eng;
+    for (int k = 0; k < 6; ++k) virial[k] = out.virial[k];
+    return 0;
+}
+
+// Unit-level entry points for the reference's own known-answer tests
+// (test/test_mlip_features.cpp).
+void oracle_cart_to_sph(double x, double y, double z, double* th, double* ph) {
+    cart_to_sph(x, y, z, *th, *ph);
+}
+void oracle_ylm(int maxl, double th, double ph, double* out_ylm, double* out_dth) {
+    Potential p;
+    p.maxl = maxl;
+    for (int l = 0; l <= maxl; ++l)
+        for (int m = -l; m < 1; ++m) p.lm_info.push_back({l, m, l * l + l + m, l * l + l - m});
+    std::vector<dc> y, yd;
+    get_ylm_d(p, th, ph, y, yd);
+    for (size_t k = 0; k < y.size(); ++k) {
+        out_ylm[2 * k] = y[k].real();
+        out_ylm[2 * k + 1] = y[k].imag();
+        out_dth[2 * k] = yd[k].real();
+        out_dth[2 * k + 1] = yd[k].imag();
+    }
+}
+void oracle_fn(double cutoff, int n, const double* beta_mu, double dis, double* fn, double* fnd) {
+    Potential p;
+    p.cutoff = cutoff;
+    p.n_fn = n;
+    p.params.resize(n);
+    for (int i = 0; i < n; ++i) p.params[i] = {beta_mu[2 * i], beta_mu[2 * i + 1]};
+    std::vector<double> a, b;
+    get_fn_d(p, dis, a, b);
+    for (int i = 0; i < n; ++i) {
+        fn[i] = a[i];
+        fnd[i] = b[i];
+    }
+}
+int oracle_num_threads() {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+}  // extern "C"
