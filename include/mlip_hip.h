@@ -1,0 +1,130 @@
+/*
+ * C ABI of the MI355X-native gtinv energy/force engine.
+ *
+ * Drop-in boundary for the LAMMPS pair style `mlip_gtinv`
+ * (src/USER-MLIP/pair_mlip_gtinv.h:17, pair_mlip_gtinv.cpp).  Every entry
+ * point returns MLIP_OK or an error code and never calls exit(); the message
+ * of the last failure is available from mlip_last_error().  A context is used
+ * by one host thread at a time (like a Pair object) and owns one HIP stream.
+ */
+#ifndef MLIP_HIP_H
+#define MLIP_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mlip_ctx mlip_ctx;
+typedef struct mlip_nlist mlip_nlist;
+
+enum {
+    MLIP_OK = 0,
+    MLIP_ERR_ARG = 1,     /* bad argument (e.g. unknown element, newton off, n_type > 2) */
+    MLIP_ERR_FILE = 2,    /* potential file missing / malformed */
+    MLIP_ERR_MODEL = 3,   /* model outside what the HIP path evaluates */
+    MLIP_ERR_DEVICE = 4,  /* HIP or rocBLAS failure */
+    MLIP_ERR_STATE = 5    /* call order (e.g. resident compute before resident set) */
+};
+
+enum { MLIP_FP64 = 0, MLIP_MIXED = 1 };
+
+/* Replaces PairMLIPGtinv::coeff (pair_mlip_gtinv.cpp:555-593) together with
+ * DataMLIPBase::initialize (mlip_model.h:35-120): `pair_coeff * * potfile
+ * E1 E2 ...`.  elem_of_type[t] names the element of LAMMPS type t+1; NULL
+ * entries leave a type unmapped (atoms of that type must not appear).
+ * Errors that the reference reports with error->all / exit(8) / exit(1)
+ * come back as MLIP_ERR_* with the message copied into errbuf. */
+int mlip_create(const char* potfile, int ntypes, const char* const* elem_of_type, mlip_ctx** out,
+                char* errbuf, size_t errlen);
+
+void mlip_destroy(mlip_ctx* ctx);
+
+/* PairMLIPGtinv::init_one (pair_mlip_gtinv.cpp:599-603): cutoff radius. */
+double mlip_cutoff(const mlip_ctx* ctx);
+
+/* Masses per LAMMPS type from the potential file (pair_mlip_gtinv.cpp:584-588). */
+int mlip_masses(const mlip_ctx* ctx, double* mass_by_type);
+
+/* PairMLIPGtinv::settings (pair_mlip_gtinv.cpp:545-549) takes no argument and
+ * forces newton on; the HIP path has no counterpart beyond this check. */
+int mlip_settings(int narg);
+
+/* FP64 (default) or mixed precision (FP32 per-neighbor work, FP64 invariants). */
+int mlip_set_precision(mlip_ctx* ctx, int mode);
+
+const char* mlip_last_error(const mlip_ctx* ctx);
+
+/* Replaces PairMLIPGtinv::compute(eflag, vflag) (pair_mlip_gtinv.cpp:63-125).
+ * Reads x[nall][3] (nall = nlocal + nghost), 1-based type[nall], tag[nall]
+ * and the LAMMPS neighbor list (inum, ilist, numneigh, firstneigh).
+ * neigh_full = 0: half list with newton on, as the CPU style requests;
+ * neigh_full = 1: full list, as the Kokkos style requests.
+ * Accumulates into f[nall][3] (ghost entries are left for LAMMPS reverse
+ * communication); writes *eng_vdwl and virial6 (xx, yy, zz, xy, xz, yz).
+ * eatom (nullable) receives per-centre site energies; vatom6 (nullable)
+ * the per-centre virial of the pairs it owns. */
+int mlip_compute(mlip_ctx* ctx, int nlocal, int nghost, const double* x, const int* type,
+                 const int64_t* tag, int inum, const int* ilist, const int* numneigh,
+                 const int* const* firstneigh, int neigh_full, int eflag, int vflag, double* f,
+                 double* eng_vdwl, double* virial6, double* eatom, double* vatom6);
+
+/* Same contract with firstneigh flattened: the list of ilist[ii] is
+ * neigh_idx[neigh_off[ii] .. neigh_off[ii] + numneigh[ilist[ii]]). */
+int mlip_compute_flat(mlip_ctx* ctx, int nlocal, int nghost, const double* x, const int* type,
+                      const int64_t* tag, int inum, const int* ilist, const int* numneigh,
+                      const int* neigh_off, const int* neigh_idx, int neigh_full, int eflag, int vflag,
+                      double* f, double* eng_vdwl, double* virial6, double* eatom, double* vatom6);
+
+/* ---- Device-resident evaluation (no counterpart in the reference; the
+ * Kokkos style re-copies its inputs every step, pair_mlip_gtinv_kokkos_impl.h
+ * :411-634).  Positions, types and a full neighbor list of owned atoms
+ * with per-pair image shifts are uploaded once; mlip_resident_compute then
+ * only enqueues device work on the context's stream. */
+int mlip_resident_set(mlip_ctx* ctx, int natoms, const double* x, const int* type, int64_t npairs,
+                      const int64_t* nbr_off, const int* nbr_idx, const double* nbr_shift);
+int mlip_resident_update_positions(mlip_ctx* ctx, const double* x);
+int mlip_resident_compute(mlip_ctx* ctx);
+int mlip_resident_sync(mlip_ctx* ctx);
+int mlip_resident_get(mlip_ctx* ctx, double* f, double* energy, double* virial6, double* eatom);
+
+/* Kernel timing of the last resident evaluation, in milliseconds, measured
+ * with HIP events on the context's stream: names[i] / ms[i], returns count. */
+int mlip_profile_enable(mlip_ctx* ctx, int on);
+int mlip_profile_read(mlip_ctx* ctx, const char** names, double* ms, int cap);
+
+/* Diagnostics for parity tests: copy device intermediates of the last
+ * evaluation to host.  what = 0: a_nlm [slot order], 1: polynomial features,
+ * 2: per-centre energies; returns the number of doubles written. */
+int64_t mlip_debug_fetch(mlip_ctx* ctx, int what, double* out, int64_t cap);
+
+/* Model description: n_type, n_fn, n_gtinv, n_des, n_coeff, maxl, lmax3,
+ * n_poly_features, n_irreps, gtinv_order, model_type, maxp. */
+int mlip_model_info(const mlip_ctx* ctx, int* info12);
+
+/* Periodic full neighbor list of a single cell for the resident path (the
+ * step before the path; LAMMPS builds it for the pair style).  cell9 holds
+ * the rows a, b, c; pairs with 0 < r < rc including self-images, neighbor
+ * indices into the owned atoms and shifts with del = x_i - x_j + shift.
+ * Binning follows src/USER-MLIP/mlipkk_cell.cpp:218-307. */
+int mlip_nlist_build(const double* cell9, int natoms, const double* x, double rc, mlip_nlist** out,
+                     char* errbuf, size_t errlen);
+int64_t mlip_nlist_npairs(const mlip_nlist* nl);
+int mlip_nlist_copy(const mlip_nlist* nl, int64_t* off, int* idx, double* shift);
+void mlip_nlist_free(mlip_nlist* nl);
+
+/* Device-free model inspection for tools and tests (no reference
+ * counterpart): which = 0 model info (as mlip_model_info), 1 feature
+ * families (9 ints each: irrep, order, tc[6], type mask), 2 quadratic
+ * products (2 feature ids each), 3 cubic products (3 each), 4 the number
+ * of coefficients the model defines (the file's own count is not checked).  Returns the
+ * number of ints written or a negative value with errbuf set. */
+int mlip_model_inspect(const char* potfile, int which, int* out, int64_t cap, char* errbuf, size_t errlen);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MLIP_HIP_H */

@@ -1,0 +1,84 @@
+// Plain-data views of the model tables and of one structure's device
+// buffers, passed by value to the HIP kernels, plus the launch wrappers the
+// host engine calls.  All pointers are device pointers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "model.h"
+
+namespace mlip {
+
+constexpr int kBlock = 64;        // atoms per slot block (one wavefront of lanes)
+constexpr int kAnlmChunk = 32;    // neighbors staged per LDS round in the a_nlm kernel
+
+enum Precision : int { kFP64 = 0, kMixed = 1 };
+
+struct DevModel {
+    int n_type, n_fn, n_gtinv, n_lmh, n_lmh3, maxl, lmax3, n_pitc, n_p;
+    int has_q34;            // order-3/4 invariants present (k_q34 runs)
+    double cutoff;
+    double beta[kMaxFn], mu[kMaxFn];
+    const double* coef;     // dense coefficient blocks
+    const double* weight;   // coeff/scale, linear part first (n_des)
+    const VisitQ12* q12;    // all types
+    const int32_t* q12_off; // [type][maxl+2]
+    const VisitQ34* vadj;
+    const GroupQ34* gadj;
+    int32_t gadj_beg[2], gadj_end[2];
+    const VisitQ34* vfeat;
+    const GroupQ34* gfeat;
+    int32_t gfeat_beg[2], gfeat_end[2];
+    const double* lin_p;    // linear weights of polynomial features (n_p)
+    const int32_t* c3_off;  // cubic terms transposed by target: offsets (n_p+1)
+    const int32_t* c3_bc;   // 2 per entry
+    const double* c3_w;
+};
+
+struct DevSystem {
+    int nall;      // positions (owned atoms first; ghosts / images after)
+    int nown;      // owned atoms (centres)
+    int nblocks;   // slot blocks of kBlock same-type centres
+    int64_t npairs;
+    int max_neigh;
+    const double* x;          // [nall][3]
+    const int32_t* elem;      // [nall] element index
+    const int32_t* poff;      // [nown+1] directed pairs grouped by centre
+    const int32_t* pnbr;      // [npairs] neighbor position index
+    const int32_t* pcen;      // [npairs] centre position index, or null (= owner)
+    const double* pshift;     // [npairs][3] added to x_c - x_n, or null
+    const int32_t* atom_of_slot;  // [nblocks*kBlock], -1 for padding
+    const int32_t* slot_of_atom;  // [nown]
+    const int32_t* block_type;    // [nblocks]
+    double* A;       // a_nlm, block-interleaved [blk][n][s][lm][re,im][64]
+    double* G;       // adjoint dE/da, same layout
+    double* D;       // polynomial features [slot][n_p]
+    double* Hq;      // quadratic-form product [slot][n_p]
+    double* H;       // polynomial adjoint [slot][n_p]
+    double* Ep;      // [slot]
+    double* Eq12;    // [slot][n_fn]
+    double* Eq34;    // [slot][n_fn][kWavesPerWg]
+    double* gpair;   // [npairs][3] dE_centre/d(del)
+    double* vown;    // [nown][6]
+    const int32_t* goff;   // [nall+1] force gather CSR
+    const int32_t* gent;   // signed pair entries: p+1 (neighbor side) / -(p+1) (centre side)
+    double* f;       // [nall][3]
+    double* e_slot;  // [nblocks*kBlock]
+    double* red;     // scratch for reductions
+    double* out;     // [7]: energy, virial[6]
+};
+
+// Upload the constant recursion tables (once per process and device).
+void init_device_constants();
+
+void launch_anlm(const DevModel& m, const DevSystem& s, int precision, hipStream_t st);
+void launch_q12(const DevModel& m, const DevSystem& s, bool feat, hipStream_t st);
+void launch_q34(const DevModel& m, const DevSystem& s, bool feat, hipStream_t st);
+void launch_poly(const DevModel& m, const DevSystem& s, hipStream_t st);
+void launch_force(const DevModel& m, const DevSystem& s, int precision, hipStream_t st);
+void launch_gather(const DevModel& m, const DevSystem& s, hipStream_t st);
+void launch_energy_virial(const DevModel& m, const DevSystem& s, hipStream_t st);
+
+}  // namespace mlip

@@ -1,0 +1,832 @@
+// Host engine and C ABI of the gtinv evaluation path: owns the model tables
+// on the device, converts LAMMPS neighbor data into the device pair layout,
+// runs the kernel pipeline on one HIP stream and returns E, F, virial.
+#include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/mlip_hip.h"
+#include "device_api.h"
+#include "model.h"
+#include "neighbor.h"
+
+using namespace mlip;
+
+namespace {
+
+template <typename T>
+struct DBuf {
+    T* p = nullptr;
+    size_t cap = 0;
+    bool ensure(size_t n) {
+        if (n <= cap && p) return true;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+        if (n == 0) n = 1;
+        if (hipMalloc(&p, n * sizeof(T)) != hipSuccess) {
+            p = nullptr;
+            return false;
+        }
+        cap = n;
+        return true;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
+struct ProfEvent {
+    const char* name;
+    hipEvent_t a, b;
+};
+
+}  // namespace
+
+struct mlip_ctx {
+    Model model;
+    std::vector<int> elem_of_type;  // LAMMPS type-1 -> element (or -1)
+    int precision = kFP64;
+    std::string err;
+    hipStream_t stream = nullptr;
+    rocblas_handle blas = nullptr;
+    int device = 0;
+    // model tables
+    DBuf<double> d_coef, d_weight, d_wmat, d_linp, d_c3w;
+    DBuf<VisitQ12> d_q12;
+    DBuf<int32_t> d_q12off, d_c3off, d_c3bc;
+    DBuf<VisitQ34> d_vadj, d_vfeat;
+    DBuf<GroupQ34> d_gadj, d_gfeat;
+    DevModel dm{};
+    // structure buffers
+    DBuf<double> x, pshift, A, G, D, Hq, H, Ep, Eq12, Eq34, gpair, vown, f, e_slot, red, out;
+    DBuf<int32_t> elem, poff, pnbr, pcen, atom_of_slot, slot_of_atom, block_type, goff, gent;
+    DevSystem ds{};
+    bool resident = false;
+    int resident_natoms = 0;
+    // profiling
+    bool prof = false;
+    std::vector<ProfEvent> events;
+    size_t ev_used = 0;
+    std::vector<std::pair<std::string, double>> last_prof;
+};
+
+namespace {
+
+bool hip_ok(mlip_ctx* c, hipError_t e, const char* what) {
+    if (e == hipSuccess) return true;
+    c->err = std::string(what) + ": " + hipGetErrorString(e);
+    return false;
+}
+
+template <typename T>
+bool upload(mlip_ctx* c, DBuf<T>& b, const T* h, size_t n, const char* what) {
+    if (!b.ensure(n)) {
+        c->err = std::string("device allocation failed for ") + what;
+        return false;
+    }
+    if (n == 0) return true;
+    return hip_ok(c, hipMemcpyAsync(b.p, h, n * sizeof(T), hipMemcpyHostToDevice, c->stream), what);
+}
+
+void set_err(char* buf, size_t len, const std::string& msg) {
+    if (buf && len) std::snprintf(buf, len, "%s", msg.c_str());
+}
+
+bool upload_model(mlip_ctx* c) {
+    const Model& m = c->model;
+    DevModel& dm = c->dm;
+    std::memset(&dm, 0, sizeof dm);
+    dm.n_type = m.n_type;
+    dm.n_fn = m.n_fn;
+    dm.n_gtinv = m.n_gtinv;
+    dm.n_lmh = m.n_lmh;
+    dm.n_lmh3 = m.n_lmh3;
+    dm.maxl = m.maxl;
+    dm.lmax3 = m.lmax3;
+    dm.n_pitc = m.n_pitc;
+    dm.n_p = m.n_p;
+    dm.has_q34 = m.has_order34 ? 1 : 0;
+    dm.cutoff = m.cutoff;
+    for (int n = 0; n < m.n_fn; ++n) {
+        dm.beta[n] = m.beta[n];
+        dm.mu[n] = m.mu[n];
+    }
+    std::vector<VisitQ12> q12;
+    std::vector<int32_t> q12off;
+    std::vector<VisitQ34> vadj, vfeat;
+    std::vector<GroupQ34> gadj, gfeat;
+    for (int t = 0; t < m.n_type; ++t) {
+        const TypeTables& tt = m.types[t];
+        const int base = static_cast<int>(q12.size());
+        for (int v : tt.q12_off) q12off.push_back(base + v);
+        q12.insert(q12.end(), tt.visits_q12.begin(), tt.visits_q12.end());
+        auto append = [](const std::vector<GroupQ34>& gs, const std::vector<VisitQ34>& vs,
+                         std::vector<GroupQ34>& gout, std::vector<VisitQ34>& vout, int32_t& beg, int32_t& end) {
+            const int vb = static_cast<int>(vout.size());
+            beg = static_cast<int>(gout.size());
+            for (GroupQ34 g : gs) {
+                g.vbeg += vb;
+                g.vend += vb;
+                gout.push_back(g);
+            }
+            end = static_cast<int>(gout.size());
+            vout.insert(vout.end(), vs.begin(), vs.end());
+        };
+        append(tt.groups_adj, tt.visits_adj, gadj, vadj, dm.gadj_beg[t], dm.gadj_end[t]);
+        append(tt.groups_feat, tt.visits_feat, gfeat, vfeat, dm.gfeat_beg[t], dm.gfeat_end[t]);
+    }
+    // cubic terms transposed by target feature
+    std::vector<std::vector<std::pair<std::array<int, 2>, double>>> c3(m.n_p);
+    for (size_t k = 0; k < m.c3_w.size(); ++k) {
+        const int a = m.c3_idx[3 * k], b = m.c3_idx[3 * k + 1], cc = m.c3_idx[3 * k + 2];
+        const double w = m.c3_w[k];
+        c3[a].push_back({{b, cc}, w});
+        c3[b].push_back({{a, cc}, w});
+        c3[cc].push_back({{a, b}, w});
+    }
+    std::vector<int32_t> c3off(m.n_p + 1, 0), c3bc;
+    std::vector<double> c3w;
+    for (int a = 0; a < m.n_p; ++a) {
+        c3off[a] = static_cast<int>(c3w.size());
+        for (const auto& e : c3[a]) {
+            c3bc.push_back(e.first[0]);
+            c3bc.push_back(e.first[1]);
+            c3w.push_back(e.second);
+        }
+    }
+    if (m.n_p > 0) c3off[m.n_p] = static_cast<int>(c3w.size());
+    bool ok = upload(c, c->d_coef, m.coef.data(), m.coef.size(), "coefficients") &&
+              upload(c, c->d_weight, m.weight.data(), m.weight.size(), "weights") &&
+              upload(c, c->d_wmat, m.wmat.data(), m.wmat.size(), "quadratic form") &&
+              upload(c, c->d_linp, m.lin_p.data(), m.lin_p.size(), "linear poly weights") &&
+              upload(c, c->d_c3w, c3w.data(), c3w.size(), "cubic weights") &&
+              upload(c, c->d_c3bc, c3bc.data(), c3bc.size(), "cubic index") &&
+              upload(c, c->d_c3off, c3off.data(), c3off.size(), "cubic offsets") &&
+              upload(c, c->d_q12, q12.data(), q12.size(), "q12 visits") &&
+              upload(c, c->d_q12off, q12off.data(), q12off.size(), "q12 offsets") &&
+              upload(c, c->d_vadj, vadj.data(), vadj.size(), "adjoint visits") &&
+              upload(c, c->d_vfeat, vfeat.data(), vfeat.size(), "feature visits") &&
+              upload(c, c->d_gadj, gadj.data(), gadj.size(), "adjoint groups") &&
+              upload(c, c->d_gfeat, gfeat.data(), gfeat.size(), "feature groups");
+    if (!ok) return false;
+    dm.coef = c->d_coef.p;
+    dm.weight = c->d_weight.p;
+    dm.q12 = c->d_q12.p;
+    dm.q12_off = c->d_q12off.p;
+    dm.vadj = c->d_vadj.p;
+    dm.gadj = c->d_gadj.p;
+    dm.vfeat = c->d_vfeat.p;
+    dm.gfeat = c->d_gfeat.p;
+    dm.lin_p = c->d_linp.p;
+    dm.c3_off = c->d_c3off.p;
+    dm.c3_bc = c->d_c3bc.p;
+    dm.c3_w = c->d_c3w.p;
+    return hip_ok(c, hipStreamSynchronize(c->stream), "model upload");
+}
+
+// Slot assignment: centres sorted by element (stable), each element padded
+// to whole blocks of kBlock lanes.
+void assign_slots(const mlip_ctx* c, int nown, const std::vector<int>& elem_of_owner,
+                  std::vector<int32_t>& atom_of_slot, std::vector<int32_t>& slot_of_atom,
+                  std::vector<int32_t>& block_type) {
+    atom_of_slot.clear();
+    block_type.clear();
+    slot_of_atom.assign(nown, -1);
+    for (int t = 0; t < c->model.n_type; ++t) {
+        std::vector<int> mem;
+        for (int i = 0; i < nown; ++i)
+            if (elem_of_owner[i] == t) mem.push_back(i);
+        const int nb = static_cast<int>((mem.size() + kBlock - 1) / kBlock);
+        for (int b = 0; b < nb; ++b) {
+            block_type.push_back(t);
+            for (int l = 0; l < kBlock; ++l) {
+                const size_t k = static_cast<size_t>(b) * kBlock + l;
+                const int a = k < mem.size() ? mem[k] : -1;
+                if (a >= 0) slot_of_atom[a] = static_cast<int>(atom_of_slot.size());
+                atom_of_slot.push_back(a);
+            }
+        }
+    }
+}
+
+// Allocate per-structure buffers and fill the DevSystem view.
+bool setup_system(mlip_ctx* c, int nall, int nown, int64_t npairs, const std::vector<double>& x,
+                  const std::vector<int32_t>& elem, const std::vector<int32_t>& elem_own,
+                  const std::vector<int32_t>& poff,
+                  const std::vector<int32_t>& pnbr, const std::vector<int32_t>* pcen,
+                  const std::vector<double>* pshift, const std::vector<int32_t>& goff,
+                  const std::vector<int32_t>& gent) {
+    const Model& m = c->model;
+    std::vector<int32_t> atom_of_slot, slot_of_atom, block_type;
+    assign_slots(c, nown, elem_own, atom_of_slot, slot_of_atom, block_type);
+    const int nblocks = static_cast<int>(block_type.size());
+    const size_t nslot = static_cast<size_t>(nblocks) * kBlock;
+    const size_t asz = nslot * m.n_fn * m.n_type * m.n_lmh * 2;
+    bool ok = upload(c, c->x, x.data(), x.size(), "positions") &&
+              upload(c, c->elem, elem.data(), elem.size(), "elements") &&
+              upload(c, c->poff, poff.data(), poff.size(), "pair offsets") &&
+              upload(c, c->pnbr, pnbr.data(), pnbr.size(), "pair neighbors") &&
+              upload(c, c->atom_of_slot, atom_of_slot.data(), atom_of_slot.size(), "slots") &&
+              upload(c, c->slot_of_atom, slot_of_atom.data(), slot_of_atom.size(), "slots") &&
+              upload(c, c->block_type, block_type.data(), block_type.size(), "block types") &&
+              upload(c, c->goff, goff.data(), goff.size(), "gather offsets") &&
+              upload(c, c->gent, gent.data(), gent.size(), "gather entries");
+    if (!ok) return false;
+    if (pcen && !upload(c, c->pcen, pcen->data(), pcen->size(), "pair centres")) return false;
+    if (pshift && !upload(c, c->pshift, pshift->data(), pshift->size(), "pair shifts")) return false;
+    const size_t np = static_cast<size_t>(m.n_p);
+    if (!c->A.ensure(asz) || !c->G.ensure(asz) || !c->D.ensure(nslot * np) || !c->Hq.ensure(nslot * np) ||
+        !c->H.ensure(nslot * np) || !c->Ep.ensure(nslot) || !c->Eq12.ensure(nslot * m.n_fn) ||
+        !c->Eq34.ensure(nslot * m.n_fn * kWavesPerWg) || !c->gpair.ensure(static_cast<size_t>(npairs) * 3) ||
+        !c->vown.ensure(static_cast<size_t>(nown) * 6) || !c->f.ensure(static_cast<size_t>(nall) * 3) ||
+        !c->e_slot.ensure(nslot) || !c->red.ensure(7 * 128) || !c->out.ensure(8)) {
+        c->err = "device allocation failed for structure buffers";
+        return false;
+    }
+    if (!hip_ok(c, hipMemsetAsync(c->A.p, 0, asz * sizeof(double), c->stream), "clear a_nlm")) return false;
+    if (!hip_ok(c, hipMemsetAsync(c->vown.p, 0, static_cast<size_t>(nown) * 6 * sizeof(double), c->stream),
+                "clear virial"))
+        return false;
+    DevSystem& s = c->ds;
+    s.nall = nall;
+    s.nown = nown;
+    s.nblocks = nblocks;
+    s.npairs = npairs;
+    s.x = c->x.p;
+    s.elem = c->elem.p;
+    s.poff = c->poff.p;
+    s.pnbr = c->pnbr.p;
+    s.pcen = pcen ? c->pcen.p : nullptr;
+    s.pshift = pshift ? c->pshift.p : nullptr;
+    s.atom_of_slot = c->atom_of_slot.p;
+    s.slot_of_atom = c->slot_of_atom.p;
+    s.block_type = c->block_type.p;
+    s.A = c->A.p;
+    s.G = c->G.p;
+    s.D = c->D.p;
+    s.Hq = c->Hq.p;
+    s.H = c->H.p;
+    s.Ep = c->Ep.p;
+    s.Eq12 = c->Eq12.p;
+    s.Eq34 = c->Eq34.p;
+    s.gpair = c->gpair.p;
+    s.vown = c->vown.p;
+    s.goff = c->goff.p;
+    s.gent = c->gent.p;
+    s.f = c->f.p;
+    s.e_slot = c->e_slot.p;
+    s.red = c->red.p;
+    s.out = c->out.p;
+    return true;
+}
+
+// Force gather lists: for every position k, the pairs whose centre (-) or
+// neighbor (+) is k, in increasing pair order.
+void build_gather(int nall, const std::vector<int32_t>& poff, const std::vector<int32_t>& pnbr,
+                  const std::vector<int32_t>* pcen, std::vector<int32_t>& goff, std::vector<int32_t>& gent) {
+    const int nown = static_cast<int>(poff.size()) - 1;
+    const size_t np = pnbr.size();
+    std::vector<int32_t> cnt(nall + 1, 0);
+    std::vector<int32_t> cen(np);
+    for (int o = 0; o < nown; ++o)
+        for (int p = poff[o]; p < poff[o + 1]; ++p) cen[p] = pcen ? (*pcen)[p] : o;
+    for (size_t p = 0; p < np; ++p) {
+        ++cnt[cen[p] + 1];
+        ++cnt[pnbr[p] + 1];
+    }
+    goff.assign(nall + 1, 0);
+    for (int k = 0; k < nall; ++k) goff[k + 1] = goff[k] + cnt[k + 1];
+    gent.assign(goff[nall], 0);
+    std::vector<int32_t> fill(goff.begin(), goff.end() - 1);
+    for (size_t p = 0; p < np; ++p) {
+        gent[fill[cen[p]]++] = -static_cast<int32_t>(p + 1);
+        gent[fill[pnbr[p]]++] = static_cast<int32_t>(p + 1);
+    }
+}
+
+void prof_mark(mlip_ctx* c, const char* name, bool start) {
+    if (!c->prof) return;
+    if (start) {
+        if (c->ev_used == c->events.size()) {
+            ProfEvent e{name, nullptr, nullptr};
+            (void)hipEventCreate(&e.a);
+            (void)hipEventCreate(&e.b);
+            c->events.push_back(e);
+        }
+        c->events[c->ev_used].name = name;
+        (void)hipEventRecord(c->events[c->ev_used].a, c->stream);
+    } else {
+        (void)hipEventRecord(c->events[c->ev_used].b, c->stream);
+        ++c->ev_used;
+    }
+}
+
+struct Scope {
+    mlip_ctx* c;
+    Scope(mlip_ctx* cc, const char* n) : c(cc) { prof_mark(c, n, true); }
+    ~Scope() { prof_mark(c, nullptr, false); }
+};
+
+bool run_pipeline(mlip_ctx* c) {
+    const DevModel& m = c->dm;
+    const DevSystem& s = c->ds;
+    const Model& mm = c->model;
+    hipStream_t st = c->stream;
+    c->ev_used = 0;
+    { Scope sc(c, "anlm"); launch_anlm(m, s, c->precision, st); }
+    if (m.n_p > 0) {
+        { Scope sc(c, "q12_feat"); launch_q12(m, s, true, st); }
+        if (mm.has_feat34) { Scope sc(c, "q34_feat"); launch_q34(m, s, true, st); }
+        {
+            Scope sc(c, "poly_gemm");
+            const double one = 1.0, zero = 0.0;
+            const int nslot = s.nblocks * kBlock;
+            if (nslot > 0 &&
+                rocblas_dgemm(c->blas, rocblas_operation_none, rocblas_operation_none, m.n_p, nslot, m.n_p, &one,
+                              c->d_wmat.p, m.n_p, s.D, m.n_p, &zero, s.Hq, m.n_p) != rocblas_status_success) {
+                c->err = "rocblas_dgemm failed";
+                return false;
+            }
+        }
+        { Scope sc(c, "poly"); launch_poly(m, s, st); }
+    }
+    { Scope sc(c, "q12_adj"); launch_q12(m, s, false, st); }
+    if (mm.has_order34) { Scope sc(c, "q34_adj"); launch_q34(m, s, false, st); }
+    { Scope sc(c, "force"); launch_force(m, s, c->precision, st); }
+    { Scope sc(c, "gather"); launch_gather(m, s, st); }
+    { Scope sc(c, "energy_virial"); launch_energy_virial(m, s, st); }
+    return hip_ok(c, hipGetLastError(), "kernel launch");
+}
+
+void collect_profile(mlip_ctx* c) {
+    c->last_prof.clear();
+    if (!c->prof) return;
+    for (size_t k = 0; k < c->ev_used; ++k) {
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, c->events[k].a, c->events[k].b);
+        c->last_prof.push_back({c->events[k].name, ms});
+    }
+}
+
+int lammps_compute(mlip_ctx* c, int nlocal, int nghost, const double* x, const int* type, const int64_t* tag,
+                   int inum, const int* ilist, const int* numneigh, const int* const* firstneigh,
+                   const int* neigh_off, const int* neigh_idx, int neigh_full, int eflag, int vflag, double* f,
+                   double* eng, double* virial, double* eatom, double* vatom) {
+    (void)eflag;
+    (void)vflag;
+    const int nall = nlocal + nghost;
+    if (nlocal < 0 || nghost < 0 || inum < 0 || inum > nlocal || !x || !type) {
+        c->err = "bad atom counts or null arrays";
+        return MLIP_ERR_ARG;
+    }
+    std::vector<int32_t> elem(nall);
+    for (int k = 0; k < nall; ++k) {
+        const int t = type[k];
+        if (t < 1 || t > static_cast<int>(c->elem_of_type.size()) || c->elem_of_type[t - 1] < 0) {
+            c->err = "atom type without an element mapping";
+            return MLIP_ERR_ARG;
+        }
+        elem[k] = c->elem_of_type[t - 1];
+    }
+    // owners: ilist order; half lists need the owner of each ghost (by tag)
+    std::vector<int> owner_of_local(nlocal, -1);
+    for (int ii = 0; ii < inum; ++ii) {
+        if (ilist[ii] < 0 || ilist[ii] >= nlocal) {
+            c->err = "ilist entry out of range";
+            return MLIP_ERR_ARG;
+        }
+        owner_of_local[ilist[ii]] = ii;
+    }
+    std::unordered_map<int64_t, int> local_of_tag;
+    if (!neigh_full) {
+        if (!tag) {
+            c->err = "half neighbor lists need atom tags (newton on, single rank)";
+            return MLIP_ERR_ARG;
+        }
+        local_of_tag.reserve(nlocal * 2);
+        for (int i = 0; i < nlocal; ++i) local_of_tag[tag[i]] = i;
+    }
+    auto list_of = [&](int ii, int& n) -> const int* {
+        const int i = ilist[ii];
+        n = numneigh[i];
+        return firstneigh ? firstneigh[i] : neigh_idx + neigh_off[ii];
+    };
+    std::vector<std::vector<std::pair<int32_t, int32_t>>> per(inum);  // (centre pos, neighbor pos)
+    for (int ii = 0; ii < inum; ++ii) {
+        int n = 0;
+        const int* js = list_of(ii, n);
+        const int i = ilist[ii];
+        for (int jj = 0; jj < n; ++jj) {
+            const int j = js[jj];
+            if (j < 0 || j >= nall) {
+                c->err = "neighbor index out of range";
+                return MLIP_ERR_ARG;
+            }
+            per[ii].push_back({i, j});
+            if (!neigh_full) {
+                auto it = local_of_tag.find(tag[j]);
+                if (it == local_of_tag.end() || owner_of_local[it->second] < 0) {
+                    c->err = "neighbor tag without a local owner (half lists are single-rank)";
+                    return MLIP_ERR_ARG;
+                }
+                per[owner_of_local[it->second]].push_back({j, i});
+            }
+        }
+    }
+    std::vector<int32_t> poff(inum + 1, 0), pnbr, pcen;
+    for (int ii = 0; ii < inum; ++ii) {
+        poff[ii + 1] = poff[ii] + static_cast<int32_t>(per[ii].size());
+        for (const auto& e : per[ii]) {
+            pcen.push_back(e.first);
+            pnbr.push_back(e.second);
+        }
+    }
+    const int64_t npairs = static_cast<int64_t>(pnbr.size());
+    // owners are numbered by ilist position; centre and neighbor positions
+    // keep their LAMMPS indices through pcen / pnbr
+    std::vector<double> xv(x, x + 3 * static_cast<size_t>(nall));
+    std::vector<int32_t> elem_own(inum);
+    for (int ii = 0; ii < inum; ++ii) elem_own[ii] = elem[ilist[ii]];
+    std::vector<int32_t> gof, gen;
+    build_gather(nall, poff, pnbr, &pcen, gof, gen);
+    c->resident = false;
+    if (!setup_system(c, nall, inum, npairs, xv, elem, elem_own, poff, pnbr, &pcen, nullptr, gof, gen))
+        return MLIP_ERR_DEVICE;
+    if (!run_pipeline(c)) return MLIP_ERR_DEVICE;
+    std::vector<double> fh(3 * static_cast<size_t>(nall)), outv(7);
+    if (!hip_ok(c, hipMemcpyAsync(fh.data(), c->f.p, fh.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream),
+                "copy forces") ||
+        !hip_ok(c, hipMemcpyAsync(outv.data(), c->out.p, 7 * sizeof(double), hipMemcpyDeviceToHost, c->stream),
+                "copy energy") ||
+        !hip_ok(c, hipStreamSynchronize(c->stream), "compute"))
+        return MLIP_ERR_DEVICE;
+    collect_profile(c);
+    for (size_t k = 0; k < fh.size(); ++k) f[k] += fh[k];
+    if (eng) *eng += outv[0];
+    if (virial)
+        for (int k = 0; k < 6; ++k) virial[k] += outv[1 + k];
+    if (eatom || vatom) {
+        const size_t nslot = static_cast<size_t>(c->ds.nblocks) * kBlock;
+        std::vector<double> es(nslot), vo(static_cast<size_t>(inum) * 6);
+        std::vector<int32_t> sl(inum);
+        (void)hipMemcpy(es.data(), c->e_slot.p, nslot * sizeof(double), hipMemcpyDeviceToHost);
+        (void)hipMemcpy(vo.data(), c->vown.p, vo.size() * sizeof(double), hipMemcpyDeviceToHost);
+        (void)hipMemcpy(sl.data(), c->slot_of_atom.p, inum * sizeof(int32_t), hipMemcpyDeviceToHost);
+        for (int ii = 0; ii < inum; ++ii) {
+            if (eatom) eatom[ilist[ii]] += es[sl[ii]];
+            if (vatom)
+                for (int k = 0; k < 6; ++k) vatom[6 * ilist[ii] + k] += vo[6 * ii + k];
+        }
+    }
+    return MLIP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mlip_settings(int narg) { return narg == 0 ? MLIP_OK : MLIP_ERR_ARG; }
+
+int mlip_create(const char* potfile, int ntypes, const char* const* elem_of_type, mlip_ctx** out, char* errbuf,
+                size_t errlen) {
+    if (!out || !potfile || ntypes < 1) {
+        set_err(errbuf, errlen, "Incorrect args for pair coefficients");
+        return MLIP_ERR_ARG;
+    }
+    *out = nullptr;
+    GtinvTable table;
+    std::string e;
+    if (!load_builtin_gtinv(table, e)) {
+        set_err(errbuf, errlen, e);
+        return MLIP_ERR_MODEL;
+    }
+    auto* c = new mlip_ctx();
+    if (!load_model(potfile, table, c->model, e)) {
+        set_err(errbuf, errlen, e);
+        delete c;
+        return e.rfind("cannot open", 0) == 0 ? MLIP_ERR_FILE : MLIP_ERR_MODEL;
+    }
+    c->elem_of_type.assign(ntypes, -1);
+    for (int t = 0; t < ntypes; ++t) {
+        const char* name = elem_of_type ? elem_of_type[t] : nullptr;
+        if (!name || std::strcmp(name, "NULL") == 0) continue;
+        const auto& el = c->model.elements;
+        auto it = std::find(el.begin(), el.end(), std::string(name));
+        if (it == el.end()) {
+            set_err(errbuf, errlen, std::string("element ") + name + " not in potential file");
+            delete c;
+            return MLIP_ERR_ARG;
+        }
+        c->elem_of_type[t] = static_cast<int>(it - el.begin());
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+        set_err(errbuf, errlen, "no HIP device available: the gtinv path runs on the GPU only");
+        delete c;
+        return MLIP_ERR_DEVICE;
+    }
+    (void)hipGetDevice(&c->device);
+    static bool constants_done = false;
+    if (!constants_done) {
+        init_device_constants();
+        constants_done = true;
+    }
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        rocblas_create_handle(&c->blas) != rocblas_status_success ||
+        rocblas_set_stream(c->blas, c->stream) != rocblas_status_success) {
+        set_err(errbuf, errlen, "failed to create HIP stream / rocBLAS handle");
+        mlip_destroy(c);
+        return MLIP_ERR_DEVICE;
+    }
+    if (!upload_model(c)) {
+        set_err(errbuf, errlen, c->err);
+        mlip_destroy(c);
+        return MLIP_ERR_DEVICE;
+    }
+    *out = c;
+    return MLIP_OK;
+}
+
+void mlip_destroy(mlip_ctx* c) {
+    if (!c) return;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (auto& e : c->events) {
+        (void)hipEventDestroy(e.a);
+        (void)hipEventDestroy(e.b);
+    }
+    DBuf<double>* dbs[] = {&c->d_coef, &c->d_weight, &c->d_wmat, &c->d_linp, &c->d_c3w, &c->x, &c->pshift,
+                           &c->A, &c->G, &c->D, &c->Hq, &c->H, &c->Ep, &c->Eq12, &c->Eq34, &c->gpair,
+                           &c->vown, &c->f, &c->e_slot, &c->red, &c->out};
+    for (auto* b : dbs) b->release();
+    DBuf<int32_t>* ibs[] = {&c->d_q12off, &c->d_c3off, &c->d_c3bc, &c->elem, &c->poff, &c->pnbr, &c->pcen,
+                            &c->atom_of_slot, &c->slot_of_atom, &c->block_type, &c->goff, &c->gent};
+    for (auto* b : ibs) b->release();
+    c->d_q12.release();
+    c->d_vadj.release();
+    c->d_vfeat.release();
+    c->d_gadj.release();
+    c->d_gfeat.release();
+    if (c->blas) rocblas_destroy_handle(c->blas);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+double mlip_cutoff(const mlip_ctx* c) { return c ? c->model.cutoff : 0.0; }
+
+int mlip_masses(const mlip_ctx* c, double* mass) {
+    if (!c || !mass) return MLIP_ERR_ARG;
+    for (size_t t = 0; t < c->elem_of_type.size(); ++t)
+        mass[t] = c->elem_of_type[t] >= 0 ? c->model.masses[c->elem_of_type[t]] : 0.0;
+    return MLIP_OK;
+}
+
+int mlip_set_precision(mlip_ctx* c, int mode) {
+    if (!c || (mode != MLIP_FP64 && mode != MLIP_MIXED)) return MLIP_ERR_ARG;
+    c->precision = mode;
+    return MLIP_OK;
+}
+
+const char* mlip_last_error(const mlip_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int mlip_compute(mlip_ctx* c, int nlocal, int nghost, const double* x, const int* type, const int64_t* tag, int inum,
+                 const int* ilist, const int* numneigh, const int* const* firstneigh, int neigh_full, int eflag,
+                 int vflag, double* f, double* eng, double* virial, double* eatom, double* vatom) {
+    if (!c || !firstneigh || !f) return MLIP_ERR_ARG;
+    return lammps_compute(c, nlocal, nghost, x, type, tag, inum, ilist, numneigh, firstneigh, nullptr, nullptr,
+                          neigh_full, eflag, vflag, f, eng, virial, eatom, vatom);
+}
+
+int mlip_compute_flat(mlip_ctx* c, int nlocal, int nghost, const double* x, const int* type, const int64_t* tag,
+                      int inum, const int* ilist, const int* numneigh, const int* neigh_off, const int* neigh_idx,
+                      int neigh_full, int eflag, int vflag, double* f, double* eng, double* virial, double* eatom,
+                      double* vatom) {
+    if (!c || !neigh_off || !neigh_idx || !f) return MLIP_ERR_ARG;
+    return lammps_compute(c, nlocal, nghost, x, type, tag, inum, ilist, numneigh, nullptr, neigh_off, neigh_idx,
+                          neigh_full, eflag, vflag, f, eng, virial, eatom, vatom);
+}
+
+int mlip_resident_set(mlip_ctx* c, int natoms, const double* x, const int* type, int64_t npairs,
+                      const int64_t* nbr_off, const int* nbr_idx, const double* nbr_shift) {
+    if (!c || natoms < 0 || !x || !type || !nbr_off || (npairs > 0 && !nbr_idx)) return MLIP_ERR_ARG;
+    if (npairs > INT32_MAX) {
+        c->err = "more than 2^31 pairs";
+        return MLIP_ERR_ARG;
+    }
+    std::vector<int32_t> elem(natoms), poff(natoms + 1), pnbr(nbr_idx, nbr_idx + npairs);
+    for (int i = 0; i < natoms; ++i) {
+        const int t = type[i];
+        if (t < 1 || t > static_cast<int>(c->elem_of_type.size()) || c->elem_of_type[t - 1] < 0) {
+            c->err = "atom type without an element mapping";
+            return MLIP_ERR_ARG;
+        }
+        elem[i] = c->elem_of_type[t - 1];
+    }
+    for (int i = 0; i <= natoms; ++i) poff[i] = static_cast<int32_t>(nbr_off[i]);
+    for (int64_t p = 0; p < npairs; ++p)
+        if (pnbr[p] < 0 || pnbr[p] >= natoms) {
+            c->err = "neighbor index out of range";
+            return MLIP_ERR_ARG;
+        }
+    std::vector<int32_t> gof, gen;
+    build_gather(natoms, poff, pnbr, nullptr, gof, gen);
+    std::vector<double> xv(x, x + 3 * static_cast<size_t>(natoms));
+    std::vector<double> sh;
+    if (nbr_shift) sh.assign(nbr_shift, nbr_shift + 3 * npairs);
+    if (!setup_system(c, natoms, natoms, npairs, xv, elem, elem, poff, pnbr, nullptr, nbr_shift ? &sh : nullptr, gof,
+                      gen))
+        return MLIP_ERR_DEVICE;
+    if (!hip_ok(c, hipStreamSynchronize(c->stream), "resident upload")) return MLIP_ERR_DEVICE;
+    c->resident = true;
+    c->resident_natoms = natoms;
+    return MLIP_OK;
+}
+
+int mlip_resident_update_positions(mlip_ctx* c, const double* x) {
+    if (!c || !x) return MLIP_ERR_ARG;
+    if (!c->resident) {
+        c->err = "no resident structure";
+        return MLIP_ERR_STATE;
+    }
+    return hip_ok(c, hipMemcpyAsync(c->x.p, x, 3 * sizeof(double) * c->resident_natoms, hipMemcpyHostToDevice,
+                                    c->stream),
+                  "update positions")
+               ? MLIP_OK
+               : MLIP_ERR_DEVICE;
+}
+
+int mlip_resident_compute(mlip_ctx* c) {
+    if (!c) return MLIP_ERR_ARG;
+    if (!c->resident) {
+        c->err = "no resident structure";
+        return MLIP_ERR_STATE;
+    }
+    return run_pipeline(c) ? MLIP_OK : MLIP_ERR_DEVICE;
+}
+
+int mlip_resident_sync(mlip_ctx* c) {
+    if (!c) return MLIP_ERR_ARG;
+    if (!hip_ok(c, hipStreamSynchronize(c->stream), "resident compute")) return MLIP_ERR_DEVICE;
+    collect_profile(c);
+    return MLIP_OK;
+}
+
+int mlip_resident_get(mlip_ctx* c, double* f, double* energy, double* virial, double* eatom) {
+    if (!c) return MLIP_ERR_ARG;
+    if (!c->resident) {
+        c->err = "no resident structure";
+        return MLIP_ERR_STATE;
+    }
+    const int n = c->resident_natoms;
+    double outv[7];
+    if (f && !hip_ok(c, hipMemcpyAsync(f, c->f.p, 3 * sizeof(double) * n, hipMemcpyDeviceToHost, c->stream), "forces"))
+        return MLIP_ERR_DEVICE;
+    if (!hip_ok(c, hipMemcpyAsync(outv, c->out.p, sizeof outv, hipMemcpyDeviceToHost, c->stream), "energy") ||
+        !hip_ok(c, hipStreamSynchronize(c->stream), "readback"))
+        return MLIP_ERR_DEVICE;
+    if (energy) *energy = outv[0];
+    if (virial)
+        for (int k = 0; k < 6; ++k) virial[k] = outv[1 + k];
+    if (eatom) {
+        const size_t nslot = static_cast<size_t>(c->ds.nblocks) * kBlock;
+        std::vector<double> es(nslot);
+        std::vector<int32_t> sl(n);
+        (void)hipMemcpy(es.data(), c->e_slot.p, nslot * sizeof(double), hipMemcpyDeviceToHost);
+        (void)hipMemcpy(sl.data(), c->slot_of_atom.p, n * sizeof(int32_t), hipMemcpyDeviceToHost);
+        for (int i = 0; i < n; ++i) eatom[i] = es[sl[i]];
+    }
+    return MLIP_OK;
+}
+
+int mlip_profile_enable(mlip_ctx* c, int on) {
+    if (!c) return MLIP_ERR_ARG;
+    c->prof = on != 0;
+    return MLIP_OK;
+}
+
+int mlip_profile_read(mlip_ctx* c, const char** names, double* ms, int cap) {
+    if (!c) return 0;
+    int n = 0;
+    for (const auto& p : c->last_prof) {
+        if (n >= cap) break;
+        names[n] = p.first.c_str();
+        ms[n] = p.second;
+        ++n;
+    }
+    return n;
+}
+
+int64_t mlip_debug_fetch(mlip_ctx* c, int what, double* out, int64_t cap) {
+    if (!c || !out) return -1;
+    const Model& m = c->model;
+    const size_t nslot = static_cast<size_t>(c->ds.nblocks) * kBlock;
+    const double* src = nullptr;
+    size_t n = 0;
+    if (what == 0) {
+        src = c->A.p;
+        n = nslot * m.n_fn * m.n_type * m.n_lmh * 2;
+    } else if (what == 1) {
+        src = c->D.p;
+        n = nslot * m.n_p;
+    } else if (what == 2) {
+        src = c->e_slot.p;
+        n = nslot;
+    } else if (what == 3) {
+        src = c->G.p;
+        n = nslot * m.n_fn * m.n_type * m.n_lmh * 2;
+    }
+    if (!src || static_cast<int64_t>(n) > cap) return -1;
+    if (hipStreamSynchronize(c->stream) != hipSuccess) return -1;
+    if (hipMemcpy(out, src, n * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) return -1;
+    return static_cast<int64_t>(n);
+}
+
+int mlip_model_inspect(const char* potfile, int which, int* out, int64_t cap, char* errbuf, size_t errlen) {
+    GtinvTable table;
+    Model m;
+    std::string e;
+    if (!potfile || !out || !load_builtin_gtinv(table, e) || !load_model(potfile, table, m, e, which != 4)) {
+        set_err(errbuf, errlen, e.empty() ? "bad arguments" : e);
+        return -1;
+    }
+    std::vector<int> v;
+    if (which == 4) {
+        v = {m.n_coeff};
+    } else if (which == 0) {
+        v = {m.n_type, m.n_fn, m.n_gtinv, m.n_des, m.n_coeff, m.maxl, m.lmax3, m.n_p,
+             static_cast<int>(m.irreps.size()), m.gtinv_order, m.model_type, m.maxp};
+    } else if (which == 1) {
+        for (const Itc& it : m.itcs) {
+            v.push_back(it.irrep);
+            v.push_back(it.order);
+            for (int k = 0; k < 6; ++k) v.push_back(it.tc[k]);
+            v.push_back(static_cast<int>(it.type_mask));
+        }
+    } else if (which == 2) {
+        for (const auto& c2 : m.comb2) v.insert(v.end(), c2.begin(), c2.end());
+    } else if (which == 3) {
+        for (const auto& c3 : m.comb3) v.insert(v.end(), c3.begin(), c3.end());
+    } else {
+        set_err(errbuf, errlen, "unknown table");
+        return -1;
+    }
+    if (static_cast<int64_t>(v.size()) > cap) {
+        set_err(errbuf, errlen, "buffer too small");
+        return -2;
+    }
+    std::copy(v.begin(), v.end(), out);
+    return static_cast<int>(v.size());
+}
+
+struct mlip_nlist {
+    NeighborList nl;
+};
+
+int mlip_nlist_build(const double* cell9, int natoms, const double* x, double rc, mlip_nlist** out, char* errbuf,
+                     size_t errlen) {
+    if (!cell9 || !x || !out || natoms < 0) {
+        set_err(errbuf, errlen, "bad arguments");
+        return MLIP_ERR_ARG;
+    }
+    auto* h = new mlip_nlist();
+    std::string e;
+    if (!build_full_list(cell9, natoms, x, rc, h->nl, e)) {
+        set_err(errbuf, errlen, e);
+        delete h;
+        return MLIP_ERR_ARG;
+    }
+    *out = h;
+    return MLIP_OK;
+}
+
+int64_t mlip_nlist_npairs(const mlip_nlist* h) { return h ? static_cast<int64_t>(h->nl.idx.size()) : -1; }
+
+int mlip_nlist_copy(const mlip_nlist* h, int64_t* off, int* idx, double* shift) {
+    if (!h) return MLIP_ERR_ARG;
+    if (off) std::copy(h->nl.off.begin(), h->nl.off.end(), off);
+    if (idx) std::copy(h->nl.idx.begin(), h->nl.idx.end(), idx);
+    if (shift) std::copy(h->nl.shift.begin(), h->nl.shift.end(), shift);
+    return MLIP_OK;
+}
+
+void mlip_nlist_free(mlip_nlist* h) { delete h; }
+
+int mlip_model_info(const mlip_ctx* c, int* info) {
+    if (!c || !info) return MLIP_ERR_ARG;
+    const Model& m = c->model;
+    int v[12] = {m.n_type, m.n_fn, m.n_gtinv, m.n_des, m.n_coeff, m.maxl, m.lmax3, m.n_p,
+                 static_cast<int>(m.irreps.size()), m.gtinv_order, m.model_type, m.maxp};
+    std::memcpy(info, v, sizeof v);
+    return MLIP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,542 @@
+// Host-side model layer: parser, index enumeration and device tables.
+// See model.h for the reference functions each part corresponds to.
+#include "model.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <set>
+#include <sstream>
+
+extern "C" {
+extern const char mlip_gtinv_table_begin[];
+extern const char mlip_gtinv_table_end[];
+}
+
+namespace mlip {
+
+// ------------------------------------------------------------ gtinv table
+bool parse_gtinv_text(const char* text, size_t len, GtinvTable& out, std::string& err) {
+    std::istringstream in(std::string(text, len));
+    std::string tag;
+    int n = 0;
+    if (!(in >> tag >> n) || tag != "gtinv_table" || n <= 0) {
+        err = "gtinv table: bad header";
+        return false;
+    }
+    out.irreps.assign(n, Irrep());
+    for (int i = 0; i < n; ++i) {
+        int idx = -1, q = 0, nt = 0;
+        if (!(in >> tag >> idx >> q) || tag != "irrep" || idx != i || q < 1 || q > 6) {
+            err = "gtinv table: bad irrep header";
+            return false;
+        }
+        Irrep& r = out.irreps[i];
+        r.l.resize(q);
+        for (int k = 0; k < q; ++k) in >> r.l[k];
+        in >> nt;
+        r.m.assign(nt, std::vector<int>(q));
+        r.c.resize(nt);
+        for (int t = 0; t < nt; ++t) {
+            for (int k = 0; k < q; ++k) in >> r.m[t][k];
+            std::string cs;
+            in >> cs;
+            r.c[t] = std::strtod(cs.c_str(), nullptr);
+        }
+        if (!in) {
+            err = "gtinv table: truncated";
+            return false;
+        }
+    }
+    return true;
+}
+
+bool load_builtin_gtinv(GtinvTable& out, std::string& err) {
+    const size_t len = static_cast<size_t>(mlip_gtinv_table_end - mlip_gtinv_table_begin);
+    return parse_gtinv_text(mlip_gtinv_table_begin, len, out, err);
+}
+
+// ------------------------------------------------------------ file parsing
+namespace {
+
+bool next_line(std::istream& in, std::string& line) { return static_cast<bool>(std::getline(in, line)); }
+
+template <typename T>
+bool first_token(std::istream& in, T& v) {
+    std::string line;
+    if (!next_line(in, line)) return false;
+    std::istringstream ss(line);
+    return static_cast<bool>(ss >> v);
+}
+
+template <typename T>
+bool n_tokens(std::istream& in, std::vector<T>& v, int n) {
+    std::string line;
+    if (!next_line(in, line)) return false;
+    std::istringstream ss(line);
+    v.resize(n);
+    for (int i = 0; i < n; ++i)
+        if (!(ss >> v[i])) return false;
+    return true;
+}
+
+// tc-tuple validity: some centre type sees every type combination
+bool tuple_valid(const Model& m, const std::vector<int>& tcs) {
+    for (int t = 0; t < m.n_type; ++t) {
+        bool ok = true;
+        for (int tc : tcs)
+            if (m.tcp_other[tc][t] < 0) {
+                ok = false;
+                break;
+            }
+        if (ok) return true;
+    }
+    return false;
+}
+
+unsigned mask_of(const Model& m, const std::vector<int>& tcs) {
+    unsigned mask = 0;
+    for (int t = 0; t < m.n_type; ++t) {
+        bool ok = true;
+        for (int tc : tcs)
+            if (m.tcp_other[tc][t] < 0) ok = false;
+        if (ok) mask |= 1u << t;
+    }
+    return mask;
+}
+
+// Features of each irrep: unique multisets {(l_k, tc_k)} over valid tc
+// tuples, in ascending lexicographic order of the sorted pair sequence
+// (the order std::set<std::multiset<pair>> yields in the reference,
+// mlip_model_params.cpp:100-124).
+void enumerate_itcs(Model& m) {
+    m.itcs.clear();
+    for (size_t i = 0; i < m.irreps.size(); ++i) {
+        const std::vector<int>& ls = m.irreps[i].l;
+        const int q = static_cast<int>(ls.size());
+        std::set<std::vector<std::pair<int, int>>> uniq;
+        std::vector<int> tup(q, 0);
+        while (true) {
+            if (tuple_valid(m, tup)) {
+                std::vector<std::pair<int, int>> key(q);
+                for (int k = 0; k < q; ++k) key[k] = {ls[k], tup[k]};
+                std::sort(key.begin(), key.end());
+                uniq.insert(key);
+            }
+            int k = q - 1;
+            while (k >= 0 && ++tup[k] == m.n_tc) tup[k--] = 0;
+            if (k < 0) break;
+        }
+        for (const auto& key : uniq) {
+            Itc it{};
+            it.irrep = static_cast<int>(i);
+            it.order = q;
+            std::vector<int> tcs(q);
+            for (int k = 0; k < q; ++k) {
+                tcs[k] = key[k].second;
+                it.tc[k] = key[k].second;
+            }
+            for (int k = q; k < 6; ++k) it.tc[k] = -1;
+            it.type_mask = mask_of(m, tcs);
+            m.itcs.push_back(it);
+        }
+    }
+    m.n_gtinv = static_cast<int>(m.itcs.size());
+}
+
+bool share_type(unsigned a, unsigned b) { return (a & b) != 0; }
+
+void enumerate_polynomial(Model& m) {
+    m.n_des = m.n_fn * m.n_gtinv;
+    m.comb2.clear();
+    m.comb3.clear();
+    m.poly_features.clear();
+    if (m.model_type == 1) {
+        // products of a feature with itself (mlip_polynomial.cpp:36-40)
+        for (int f = 0; f < m.n_des; ++f) {
+            if (m.maxp > 1) m.comb2.push_back({f, f});
+            if (m.maxp > 2) m.comb3.push_back({f, f, f});
+        }
+        m.n_coeff = m.n_des * m.maxp;
+        if (m.maxp > 1)
+            for (int f = 0; f < m.n_des; ++f) m.poly_features.push_back(f);
+        return;
+    }
+    std::vector<int> pidx;
+    for (int f = 0; f < m.n_des; ++f) {
+        const Itc& it = m.itcs[f % m.n_gtinv];
+        const bool use = (m.model_type == 2) || (m.model_type == 3 && it.order == 1) ||
+                         (m.model_type == 4 && it.order < 3);
+        if (use) pidx.push_back(f);
+    }
+    auto mask = [&](int f) { return m.itcs[f % m.n_gtinv].type_mask; };
+    if (m.maxp > 1)
+        for (size_t i1 = 0; i1 < pidx.size(); ++i1)
+            for (size_t i2 = 0; i2 <= i1; ++i2)
+                if (share_type(mask(pidx[i1]), mask(pidx[i2]))) m.comb2.push_back({pidx[i2], pidx[i1]});
+    if (m.maxp > 2)
+        for (size_t i1 = 0; i1 < pidx.size(); ++i1)
+            for (size_t i2 = 0; i2 <= i1; ++i2)
+                for (size_t i3 = 0; i3 <= i2; ++i3)
+                    if (mask(pidx[i1]) & mask(pidx[i2]) & mask(pidx[i3]))
+                        m.comb3.push_back({pidx[i3], pidx[i2], pidx[i1]});
+    m.n_coeff = m.n_des + static_cast<int>(m.comb2.size() + m.comb3.size());
+    if (m.maxp > 1) m.poly_features = pidx;
+}
+
+double lookup(const std::map<std::vector<int>, double>& c, const std::vector<int>& mt) {
+    auto it = c.find(mt);
+    return it == c.end() ? 0.0 : it->second;
+}
+
+// Builds the flattened visit tables for centre type t.
+void build_type_tables(Model& m, int t, const std::vector<std::map<std::vector<int>, double>>& cmap) {
+    TypeTables tt;
+    std::vector<std::vector<VisitQ12>> by_l(m.maxl + 1);
+    struct Pending {
+        int S, L;
+        VisitQ34 v;
+        long cost;
+    };
+    std::vector<Pending> adj, feat;
+    auto push_coef = [&](double v) { m.coef.push_back(v); };
+    for (int i = 0; i < m.n_gtinv; ++i) {
+        const Itc& it = m.itcs[i];
+        if (!((it.type_mask >> t) & 1u)) continue;
+        const Irrep& ir = m.irreps[it.irrep];
+        const std::map<std::vector<int>, double>& cm = cmap[it.irrep];
+        const int q = it.order;
+        std::vector<int> sp(q);
+        for (int k = 0; k < q; ++k) sp[k] = m.tcp_other[it.tc[k]][t];
+        const int pslot = m.prank[i];
+        if (q <= 2) {
+            VisitQ12 v{};
+            v.order = q;
+            v.itc = i;
+            v.s0 = sp[0];
+            v.s1 = q == 2 ? sp[1] : sp[0];
+            v.pslot = pslot;
+            v.coef = static_cast<int>(m.coef.size());
+            const int L = ir.l[0];
+            if (q == 1) {
+                push_coef(lookup(cm, {0}));
+            } else {
+                for (int m0 = -L; m0 <= L; ++m0) push_coef(lookup(cm, {m0, -m0}));
+            }
+            by_l[L].push_back(v);
+            continue;
+        }
+        // orders 3 and 4: one visit per factor position
+        for (int p = 0; p < q; ++p) {
+            std::vector<int> oth;
+            for (int k = 0; k < q; ++k)
+                if (k != p) oth.push_back(k);
+            const int L = ir.l[p];
+            VisitQ34 v{};
+            v.order = q;
+            v.itc = i;
+            v.la = ir.l[oth[0]];
+            v.sa = sp[oth[0]];
+            v.lb = ir.l[oth[1]];
+            v.sb = sp[oth[1]];
+            v.lc = q == 4 ? ir.l[oth[2]] : 0;
+            v.sc = q == 4 ? sp[oth[2]] : 0;
+            v.pslot = pslot;
+            const bool is_poly = pslot >= 0;
+            const bool with_d = (p == 0) && !is_poly;
+            v.flags = (with_d ? 1 : 0) | 2;
+            std::vector<int> mt(q);
+            long cost = 0;
+            auto emit = [&](bool dense_d, bool g_only_sum) {
+                // dense block over (ia[, ib], k); k <-> m_p = -k
+                const int nA = 2 * v.la + 1, nB = q == 4 ? 2 * v.lb + 1 : 1;
+                for (int ia = 0; ia < nA; ++ia)
+                    for (int ib = 0; ib < nB; ++ib)
+                        for (int k = 0; k <= L; ++k) {
+                            const int mp = -k, ma = ia - v.la;
+                            int mb, mc = 0;
+                            bool ok;
+                            if (q == 3) {
+                                mb = -mp - ma;
+                                ok = std::abs(mb) <= v.lb;
+                            } else {
+                                mb = ib - v.lb;
+                                mc = -mp - ma - mb;
+                                ok = std::abs(mc) <= v.lc;
+                            }
+                            double val = 0.0;
+                            if (ok) {
+                                std::fill(mt.begin(), mt.end(), 0);
+                                mt[p] = mp;
+                                mt[oth[0]] = ma;
+                                mt[oth[1]] = mb;
+                                if (q == 4) mt[oth[2]] = mc;
+                                const double c = lookup(cm, mt);
+                                if (g_only_sum) {
+                                    val = c;
+                                } else {
+                                    std::vector<int> neg(mt);
+                                    for (int& x : neg) x = -x;
+                                    val = (k > 0 && dense_d) ? c + lookup(cm, neg) : c;
+                                }
+                                ++cost;
+                            }
+                            push_coef(val);
+                        }
+            };
+            v.coef = static_cast<int>(m.coef.size());
+            emit(false, true);
+            v.dcoef = -1;
+            if (with_d) {
+                v.dcoef = static_cast<int>(m.coef.size());
+                emit(true, false);
+            }
+            adj.push_back({sp[p], L, v, cost});
+            if (p == 0 && is_poly) {
+                VisitQ34 f = v;
+                f.flags = 1;
+                f.dcoef = static_cast<int>(m.coef.size());
+                f.coef = f.dcoef;
+                emit(true, false);
+                feat.push_back({sp[p], L, f, cost});
+            }
+        }
+    }
+    tt.q12_off.assign(m.maxl + 2, 0);
+    for (int L = 0; L <= m.maxl; ++L) {
+        tt.q12_off[L] = static_cast<int>(tt.visits_q12.size());
+        tt.visits_q12.insert(tt.visits_q12.end(), by_l[L].begin(), by_l[L].end());
+    }
+    tt.q12_off[m.maxl + 1] = static_cast<int>(tt.visits_q12.size());
+    // group by (S, L), balance groups over the waves of a workgroup
+    auto make_groups = [&](std::vector<Pending>& pend, std::vector<GroupQ34>& groups,
+                           std::vector<VisitQ34>& visits) {
+        std::stable_sort(pend.begin(), pend.end(), [](const Pending& a, const Pending& b) {
+            return a.S != b.S ? a.S < b.S : a.L < b.L;
+        });
+        std::vector<long> gcost;
+        for (size_t k = 0; k < pend.size();) {
+            size_t e = k;
+            long c = 0;
+            while (e < pend.size() && pend[e].S == pend[k].S && pend[e].L == pend[k].L) c += pend[e++].cost;
+            GroupQ34 g{};
+            g.S = pend[k].S;
+            g.L = pend[k].L;
+            g.vbeg = static_cast<int>(visits.size());
+            for (size_t j = k; j < e; ++j) visits.push_back(pend[j].v);
+            g.vend = static_cast<int>(visits.size());
+            groups.push_back(g);
+            gcost.push_back(c);
+            k = e;
+        }
+        std::vector<size_t> order(groups.size());
+        for (size_t k = 0; k < order.size(); ++k) order[k] = k;
+        std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return gcost[a] > gcost[b]; });
+        std::vector<long> load(kWavesPerWg, 0);
+        for (size_t k : order) {
+            int w = static_cast<int>(std::min_element(load.begin(), load.end()) - load.begin());
+            groups[k].wave = w;
+            load[w] += gcost[k] + 16;
+        }
+    };
+    make_groups(adj, tt.groups_adj, tt.visits_adj);
+    make_groups(feat, tt.groups_feat, tt.visits_feat);
+    m.types[t] = std::move(tt);
+}
+
+}  // namespace
+
+bool load_model(const std::string& path, const GtinvTable& table, Model& m, std::string& err, bool strict_count) {
+    std::ifstream in(path);
+    if (!in) {
+        err = "cannot open potential file " + path;
+        return false;
+    }
+    std::string line;
+    if (!next_line(in, line)) {
+        err = "empty potential file";
+        return false;
+    }
+    {
+        std::istringstream ss(line);
+        std::string tok;
+        std::vector<std::string> toks;
+        while (ss >> tok) toks.push_back(tok);
+        // element names followed by exactly two trailing tokens (mlip_model.h:53-54)
+        if (toks.size() < 3) {
+            err = "element line must list elements followed by two comment tokens";
+            return false;
+        }
+        m.elements.assign(toks.begin(), toks.end() - 2);
+    }
+    m.n_type = static_cast<int>(m.elements.size());
+    std::string pair_type, des_type;
+    if (!first_token(in, m.cutoff) || !first_token(in, pair_type) || !first_token(in, des_type) ||
+        !first_token(in, m.model_type) || !first_token(in, m.maxp) || !first_token(in, m.maxl)) {
+        err = "truncated potential header";
+        return false;
+    }
+    if (pair_type != "gaussian") {
+        err = "unknown pair_type " + pair_type + " (only gaussian)";
+        return false;
+    }
+    if (des_type != "gtinv") {
+        err = "des_type must be gtinv";
+        return false;
+    }
+    if (m.model_type < 1 || m.model_type > 4 || m.maxp < 1 || m.maxp > 3) {
+        err = "unsupported model_type/max_p";
+        return false;
+    }
+    if (!(m.cutoff > 0.0) || m.maxl < 0 || m.maxl > kMaxL) {
+        err = "cutoff must be positive and max_l in [0,12]";
+        return false;
+    }
+    if (!first_token(in, m.gtinv_order) || m.gtinv_order < 1 || m.gtinv_order > 6) {
+        err = "bad gtinv_order";
+        return false;
+    }
+    if (!n_tokens(in, m.gtinv_maxl, m.gtinv_order - 1) || !n_tokens(in, m.gtinv_sym, m.gtinv_order - 1)) {
+        err = "bad gtinv_maxl/gtinv_sym line";
+        return false;
+    }
+    int nreg = 0;
+    if (!first_token(in, nreg) || nreg <= 0) {
+        err = "bad number of regression coefficients";
+        return false;
+    }
+    if (!n_tokens(in, m.coeff_file, nreg) || !n_tokens(in, m.scale_file, nreg)) {
+        err = "truncated coefficient or scale line";
+        return false;
+    }
+    int npar = 0;
+    if (!first_token(in, npar) || npar <= 0 || npar > kMaxFn) {
+        err = "number of radial functions must be in [1,16]";
+        return false;
+    }
+    m.beta.resize(npar);
+    m.mu.resize(npar);
+    for (int i = 0; i < npar; ++i) {
+        std::vector<double> p;
+        if (!n_tokens(in, p, 2)) {
+            err = "truncated radial parameters";
+            return false;
+        }
+        m.beta[i] = p[0];
+        m.mu[i] = p[1];
+    }
+    if (!n_tokens(in, m.masses, m.n_type)) {
+        err = "truncated mass line";
+        return false;
+    }
+    m.n_fn = npar;
+    if (m.n_type < 1 || m.n_type > 2) {
+        err = "only one or two elements are supported (reference ModelParams)";
+        return false;
+    }
+    // type combinations (mlip_model_params.cpp:30-33)
+    if (m.n_type == 1) {
+        m.tcp_other = {{0, -1}};
+    } else {
+        m.tcp_other = {{0, -1}, {1, 0}, {-1, 1}};
+    }
+    m.n_tc = static_cast<int>(m.tcp_other.size());
+    // screening (mlip_read_gtinv.cpp:35-76)
+    m.irreps.clear();
+    for (const Irrep& ir : table.irreps) {
+        const int q = static_cast<int>(ir.l.size());
+        if (q > 1) {
+            if (q > m.gtinv_order || ir.l.back() > m.gtinv_maxl[q - 2]) continue;
+            if (m.gtinv_sym[q - 2] && std::count(ir.l.begin(), ir.l.end(), ir.l[0]) != q) continue;
+        }
+        if (ir.l.back() > m.maxl) {
+            err = "gtinv_maxl exceeds max_l";
+            return false;
+        }
+        m.irreps.push_back(ir);
+    }
+    enumerate_itcs(m);
+    enumerate_polynomial(m);
+    if (!strict_count) return true;  // enumeration only (coefficient-count probe)
+    if (m.n_coeff != nreg) {
+        char buf[200];
+        std::snprintf(buf, sizeof buf, "potential lists %d coefficients, the model defines %d", nreg, m.n_coeff);
+        err = buf;
+        return false;
+    }
+    m.weight.resize(nreg);
+    for (int i = 0; i < nreg; ++i) {
+        if (m.scale_file[i] == 0.0) {
+            err = "zero scale coefficient";
+            return false;
+        }
+        m.weight[i] = m.coeff_file[i] / m.scale_file[i];
+    }
+    for (const Irrep& ir : m.irreps)
+        if (static_cast<int>(ir.l.size()) > kMaxOrderGpu) {
+            err = "invariants of order > 4 are not supported by the HIP path";
+            return false;
+        }
+    // polynomial features: per itc (all n or none)
+    m.prank.assign(m.n_gtinv, -1);
+    {
+        std::vector<char> isp(m.n_gtinv, 0);
+        for (int f : m.poly_features) isp[f % m.n_gtinv] = 1;
+        int r = 0;
+        for (int i = 0; i < m.n_gtinv; ++i)
+            if (isp[i]) m.prank[i] = r++;
+        m.n_pitc = r;
+        m.n_p = m.n_fn * m.n_pitc;
+        // every feature of a polynomial itc must be in the list for all n
+        if (static_cast<int>(m.poly_features.size()) != m.n_p) {
+            err = "internal: polynomial features are not closed over n";
+            return false;
+        }
+    }
+    auto pos = [&](int f) { return (f / m.n_gtinv) * m.n_pitc + m.prank[f % m.n_gtinv]; };
+    m.lin_p.assign(m.n_p, 0.0);
+    for (int f : m.poly_features) m.lin_p[pos(f)] = m.weight[f];
+    m.wmat.assign(static_cast<size_t>(m.n_p) * m.n_p, 0.0);
+    for (size_t k = 0; k < m.comb2.size(); ++k) {
+        const double w = m.weight[m.n_des + k];
+        const int a = pos(m.comb2[k][0]), b = pos(m.comb2[k][1]);
+        if (a == b) {
+            m.wmat[static_cast<size_t>(a) * m.n_p + a] += 2.0 * w;
+        } else {
+            m.wmat[static_cast<size_t>(a) * m.n_p + b] += w;
+            m.wmat[static_cast<size_t>(b) * m.n_p + a] += w;
+        }
+    }
+    m.c3_idx.clear();
+    m.c3_w.clear();
+    for (size_t k = 0; k < m.comb3.size(); ++k) {
+        for (int q = 0; q < 3; ++q) m.c3_idx.push_back(pos(m.comb3[k][q]));
+        m.c3_w.push_back(m.weight[m.n_des + m.comb2.size() + k]);
+    }
+    // device layout
+    m.n_lmh = n_lmh_of(m.maxl);
+    m.lmax3 = 0;
+    m.has_order34 = false;
+    for (const Irrep& ir : m.irreps)
+        if (ir.l.size() >= 3) {
+            m.has_order34 = true;
+            m.lmax3 = std::max(m.lmax3, ir.l.back());
+        }
+    m.n_lmh3 = n_lmh_of(m.lmax3);
+    std::vector<std::map<std::vector<int>, double>> cmap(m.irreps.size());
+    for (size_t i = 0; i < m.irreps.size(); ++i)
+        for (size_t t = 0; t < m.irreps[i].m.size(); ++t) cmap[i][m.irreps[i].m[t]] = m.irreps[i].c[t];
+    m.coef.clear();
+    m.types.assign(m.n_type, TypeTables());
+    for (int t = 0; t < m.n_type; ++t) build_type_tables(m, t, cmap);
+    m.has_feat34 = false;
+    for (const auto& tt : m.types)
+        if (!tt.visits_feat.empty()) m.has_feat34 = true;
+    return true;
+}
+
+}  // namespace mlip

@@ -1,0 +1,133 @@
+// Host-side model layer of the gtinv evaluation path: potential-file parser,
+// generalized Clebsch-Gordan table, feature / polynomial index enumeration
+// and the flattened per-species tables the HIP kernels consume.
+//
+// Reference counterparts:
+//   potential file + initialize     src/USER-MLIP/mlip_model.h:35-120
+//   gtinv screening                 src/USER-MLIP/mlip_read_gtinv.cpp:35-76
+//   (irreps x type-comb) features   src/USER-MLIP/mlip_model_params.cpp:86-126
+//   polynomial index enumeration    src/USER-MLIP/mlip_model_params.cpp:27-82,128-170
+//   model_type 1 products           src/USER-MLIP/mlip_polynomial.cpp:36-40
+#pragma once
+
+#include <array>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace mlip {
+
+constexpr int kMaxFn = 16;      // radial functions supported by the kernels
+constexpr int kMaxL = 12;       // max angular momentum of a_nlm
+constexpr int kMaxOrderGpu = 4; // highest invariant order of the HIP path
+constexpr int kWavesPerWg = 8;  // waves sharing one a_nlm tile in the adjoint kernel
+
+struct Irrep {
+    std::vector<int> l;                // nondecreasing l_1..l_q
+    std::vector<std::vector<int>> m;   // m-tuples with sum 0
+    std::vector<double> c;             // coefficients (reference decimal strings)
+};
+
+struct GtinvTable {
+    std::vector<Irrep> irreps;
+};
+
+// The gtinv table is compiled into the library (see gtinv_blob.S).
+bool load_builtin_gtinv(GtinvTable& out, std::string& err);
+bool parse_gtinv_text(const char* text, size_t len, GtinvTable& out, std::string& err);
+
+// One structural feature family: an irrep combined with a multiset of type
+// combinations (LinearTermGtinv, mlip_model_params.h:88-92).
+struct Itc {
+    int irrep;            // index into Model::irreps (screened)
+    int order;            // q
+    std::array<int, 6> tc;
+    unsigned type_mask;   // bit t set <=> centre type t sees this feature
+};
+
+// ---- device table records (plain data, copied to HBM) ----
+// Order-1/2 visit (cheap orders, evaluated from HBM-resident a_nlm).
+struct VisitQ12 {
+    int32_t order;     // 1 or 2
+    int32_t itc;
+    int32_t s0, s1;    // species of the two factors (s1 unused for order 1)
+    int32_t pslot;     // rank among polynomial itcs or -1
+    int32_t coef;      // offset into coefficient array: order 2 -> 2L+1 values (m0=-L..L)
+};
+
+// Order-3/4 visit: contributions of one (itc, factor position) to the
+// adjoint G_{S,L,m<=0} and, for the designated position, the feature d.
+struct VisitQ34 {
+    int32_t order;     // 3 or 4
+    int32_t itc;
+    int32_t la, sa;    // other factors in irrep order
+    int32_t lb, sb;
+    int32_t lc, sc;    // order 4 only
+    int32_t pslot;     // rank among polynomial itcs or -1
+    int32_t flags;     // bit0: compute d here; bit1: accumulate G
+    int32_t coef;      // dense coefficient block offset
+    int32_t dcoef;     // dense pair-summed block offset (flags bit0)
+};
+
+// Output group: all visits of one (species S, angular momentum L) handled by
+// one wave of the adjoint kernel with a statically unrolled G_L accumulator.
+struct GroupQ34 {
+    int32_t S, L;
+    int32_t vbeg, vend;
+    int32_t wave;      // wave of the workgroup that owns the group
+    int32_t pad;
+};
+
+struct TypeTables {
+    // order 1/2: per L, range into visits_q12
+    std::vector<int32_t> q12_off;      // size maxl+2
+    std::vector<VisitQ12> visits_q12;
+    // order 3/4: adjoint pass and polynomial-feature pass
+    std::vector<GroupQ34> groups_adj, groups_feat;
+    std::vector<VisitQ34> visits_adj, visits_feat;
+};
+
+struct Model {
+    // file contents
+    std::vector<std::string> elements;
+    std::vector<double> masses;
+    double cutoff = 0.0;
+    int model_type = 0, maxp = 0, maxl = 0, gtinv_order = 0;
+    std::vector<int> gtinv_maxl, gtinv_sym;
+    std::vector<double> beta, mu;
+    std::vector<double> coeff_file, scale_file;
+    // enumeration
+    int n_type = 0, n_fn = 0, n_tc = 0, n_gtinv = 0, n_des = 0, n_coeff = 0;
+    std::vector<Irrep> irreps;
+    std::vector<Itc> itcs;
+    std::vector<std::array<int, 2>> tcp_other;   // [tc][type] -> species or -1
+    std::vector<double> weight;                  // coeff/scale, size n_coeff
+    std::vector<int> poly_features;              // sorted feature ids in products
+    std::vector<std::array<int, 2>> comb2;
+    std::vector<std::array<int, 3>> comb3;
+    // GPU-side layout
+    int lmax3 = 0;             // largest l among order>=3 irreps (LDS tile)
+    int n_lmh = 0;             // (maxl+1)(maxl+2)/2, m<=0 entries of a_nlm
+    int n_lmh3 = 0;            // same for lmax3
+    int n_pitc = 0;            // itcs whose features enter polynomial products
+    std::vector<int> prank;    // itc -> rank among polynomial itcs or -1
+    int n_p = 0;               // n_fn * n_pitc
+    std::vector<double> wmat;  // n_p x n_p symmetric quadratic form
+    std::vector<int32_t> c3_idx;  // comb3 in polynomial-feature space, 3 per term
+    std::vector<double> c3_w;
+    std::vector<double> lin_p;    // linear weights of the polynomial features (n_p)
+    std::vector<double> coef;     // dense coefficient blocks for all visits
+    std::vector<TypeTables> types;
+    bool has_order34 = false;
+    bool has_feat34 = false;
+};
+
+// Parse a potential file (format of mlip_model.h:45-101) and build every
+// table.  Returns false with a message instead of exiting.
+bool load_model(const std::string& path, const GtinvTable& table, Model& m, std::string& err,
+                bool strict_count = true);
+
+inline int lmh_index(int l, int m) { return l * (l + 1) / 2 + l + m; }  // m <= 0
+inline int n_lmh_of(int lmax) { return (lmax + 1) * (lmax + 2) / 2; }
+
+}  // namespace mlip

@@ -1,0 +1,214 @@
+"""Python mirror of the reference pair style ``mlip_gtinv``.
+
+``PairMLIPGtinv`` follows src/USER-MLIP/pair_mlip_gtinv.{h,cpp}: ``settings``
+(:545-549) takes no argument, ``coeff`` (:555-593) reads ``* * potfile
+elem...``, ``init_one`` (:599-603) returns the cutoff and ``compute``
+(:63-125) evaluates energy, forces and virial for a LAMMPS-shaped neighbor
+list.  All arithmetic happens in the HIP library.
+"""
+import ctypes
+
+import numpy as np
+
+from ._lib import MLIP_OK, MlipError, library
+
+INFO_KEYS = ["n_type", "n_fn", "n_gtinv", "n_des", "n_coeff", "maxl", "lmax3", "n_poly_features", "n_irreps",
+             "gtinv_order", "model_type", "maxp"]
+
+
+def _p(a, ct):
+    return a.ctypes.data_as(ctypes.POINTER(ct))
+
+
+def _null(ct=ctypes.c_double):
+    return ctypes.POINTER(ct)()
+
+
+class PairMLIPGtinv:
+    single_enable = 0
+    restartinfo = 0
+    one_coeff = 1
+    manybody_flag = 1
+
+    def __init__(self):
+        self.ctx = None
+        self.lib = library()
+
+    def __del__(self):
+        self.close()
+
+    def close(self):
+        if getattr(self, "ctx", None):
+            self.lib.mlip_destroy(self.ctx)
+            self.ctx = None
+
+    def _check(self, rc):
+        if rc != MLIP_OK:
+            raise MlipError(rc, self.lib.mlip_last_error(self.ctx).decode())
+
+    def settings(self, *args):
+        if self.lib.mlip_settings(len(args)) != MLIP_OK:
+            raise MlipError(1, "Illegal pair_style command")
+
+    def coeff(self, *args, ntypes=None):
+        """``coeff('*', '*', potfile, 'Ti', 'Al', ...)``"""
+        if len(args) < 4 or args[0] != "*" or args[1] != "*":
+            raise MlipError(1, "Incorrect args for pair coefficients")
+        potfile, elems = args[2], list(args[3:])
+        if ntypes is not None and len(elems) != ntypes:
+            raise MlipError(1, "Incorrect args for pair coefficients")
+        arr = (ctypes.c_char_p * len(elems))(*[e.encode() for e in elems])
+        err = ctypes.create_string_buffer(1024)
+        ctx = ctypes.c_void_p()
+        rc = self.lib.mlip_create(potfile.encode(), len(elems), arr, ctypes.byref(ctx), err, 1024)
+        if rc != MLIP_OK:
+            raise MlipError(rc, err.value.decode())
+        self.close()
+        self.ctx = ctx
+        self.ntypes = len(elems)
+
+    def init_one(self, i=1, j=1):
+        return self.lib.mlip_cutoff(self.ctx)
+
+    def masses(self):
+        out = np.zeros(self.ntypes)
+        self._check(self.lib.mlip_masses(self.ctx, _p(out, ctypes.c_double)))
+        return out
+
+    def model_info(self):
+        v = np.zeros(12, dtype=np.int32)
+        self.lib.mlip_model_info(self.ctx, _p(v, ctypes.c_int))
+        return dict(zip(INFO_KEYS, v.tolist()))
+
+    def set_precision(self, mode):
+        self._check(self.lib.mlip_set_precision(self.ctx, {"fp64": 0, "mixed": 1}[mode]))
+
+    def compute(self, nl, types, eflag=1, vflag=1, eatom=False, vatom=False):
+        """Evaluate on a ``workloads.NeighborList``; ``types`` are the
+        1-based LAMMPS types of all nall positions.  Returns a dict with the
+        energy, forces on all positions (ghost entries unreduced), forces
+        folded onto owned atoms, and the virial."""
+        f = np.zeros((nl.nall, 3))
+        eng = np.zeros(1)
+        vir = np.zeros(6)
+        ea = np.zeros(nl.nall) if eatom else None
+        va = np.zeros((nl.nall, 6)) if vatom else None
+        typ = np.ascontiguousarray(types, dtype=np.int32)
+        tag = np.ascontiguousarray(nl.tag, dtype=np.int64)
+        rc = self.lib.mlip_compute_flat(
+            self.ctx, nl.nlocal, nl.nall - nl.nlocal, _p(nl.x, ctypes.c_double), _p(typ, ctypes.c_int),
+            _p(tag, ctypes.c_int64), nl.inum, _p(nl.ilist, ctypes.c_int), _p(nl.numneigh, ctypes.c_int),
+            _p(nl.off, ctypes.c_int), _p(nl.idx, ctypes.c_int), 1 if nl.full else 0, eflag, vflag,
+            _p(f, ctypes.c_double), _p(eng, ctypes.c_double), _p(vir, ctypes.c_double),
+            _p(ea, ctypes.c_double) if eatom else _null(), _p(va, ctypes.c_double) if vatom else _null())
+        self._check(rc)
+        out = {"energy": float(eng[0]), "forces_all": f, "virial": vir}
+        if nl.owner is not None:
+            fo = np.zeros((nl.nlocal, 3))
+            np.add.at(fo, nl.owner, f)
+            out["forces"] = fo
+        if eatom:
+            out["eatom"] = ea
+        if vatom:
+            out["vatom"] = va
+        return out
+
+    def debug_fetch(self, what, n):
+        buf = np.zeros(n)
+        got = self.lib.mlip_debug_fetch(self.ctx, what, _p(buf, ctypes.c_double), n)
+        if got < 0:
+            raise MlipError(5, "debug fetch failed")
+        return buf[:got]
+
+
+class ResidentSystem:
+    """Device-resident structure: positions, types and a full neighbor list
+    of owned atoms with image shifts stay in HBM; compute() only enqueues."""
+
+    def __init__(self, pair, x, types, nbr_off, nbr_idx, nbr_shift=None):
+        self.pair = pair
+        self.lib = pair.lib
+        self.n = int(x.shape[0])
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        t = np.ascontiguousarray(types, dtype=np.int32)
+        off = np.ascontiguousarray(nbr_off, dtype=np.int64)
+        idx = np.ascontiguousarray(nbr_idx, dtype=np.int32)
+        sh = None if nbr_shift is None else np.ascontiguousarray(nbr_shift, dtype=np.float64)
+        rc = self.lib.mlip_resident_set(pair.ctx, self.n, _p(x, ctypes.c_double), _p(t, ctypes.c_int),
+                                        int(idx.size), _p(off, ctypes.c_int64), _p(idx, ctypes.c_int),
+                                        _p(sh, ctypes.c_double) if sh is not None else _null())
+        pair._check(rc)
+        self.npairs = int(idx.size)
+
+    def update_positions(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        self.pair._check(self.lib.mlip_resident_update_positions(self.pair.ctx, _p(x, ctypes.c_double)))
+
+    def compute(self):
+        self.pair._check(self.lib.mlip_resident_compute(self.pair.ctx))
+
+    def sync(self):
+        self.pair._check(self.lib.mlip_resident_sync(self.pair.ctx))
+
+    def get(self, eatom=False):
+        f = np.zeros((self.n, 3))
+        e = np.zeros(1)
+        v = np.zeros(6)
+        ea = np.zeros(self.n) if eatom else None
+        self.pair._check(self.lib.mlip_resident_get(self.pair.ctx, _p(f, ctypes.c_double), _p(e, ctypes.c_double),
+                                                    _p(v, ctypes.c_double),
+                                                    _p(ea, ctypes.c_double) if eatom else _null()))
+        out = {"energy": float(e[0]), "forces": f, "virial": v}
+        if eatom:
+            out["eatom"] = ea
+        return out
+
+    def profile(self, on=True):
+        self.lib.mlip_profile_enable(self.pair.ctx, 1 if on else 0)
+
+    def read_profile(self):
+        names = (ctypes.c_char_p * 64)()
+        ms = np.zeros(64)
+        n = self.lib.mlip_profile_read(self.pair.ctx, names, _p(ms, ctypes.c_double), 64)
+        return [(names[k].decode(), float(ms[k])) for k in range(n)]
+
+
+def inspect_model(potfile, which, cap=1 << 26):
+    """Device-free model tables (see mlip_model_inspect in include/mlip_hip.h)."""
+    lib = library()
+    out = np.zeros(cap, dtype=np.int32)
+    err = ctypes.create_string_buffer(1024)
+    n = lib.mlip_model_inspect(potfile.encode(), which, _p(out, ctypes.c_int), cap, err, 1024)
+    if n < 0:
+        raise MlipError(3, err.value.decode())
+    out = out[:n]
+    if which == 0:
+        return dict(zip(INFO_KEYS, out.tolist()))
+    if which == 4:
+        return int(out[0])
+    if which == 1:
+        return out.reshape(-1, 9)
+    return out.reshape(-1, which)
+
+
+def periodic_full_list(cell, x, rc):
+    """Binned periodic full list (native builder): returns (off int64[n+1],
+    idx int32[npairs], shift float64[npairs,3]) with del = x_i - x_j + shift."""
+    lib = library()
+    cell = np.ascontiguousarray(cell, dtype=np.float64).reshape(9)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    h = ctypes.c_void_p()
+    err = ctypes.create_string_buffer(512)
+    rc_ = lib.mlip_nlist_build(_p(cell, ctypes.c_double), int(x.shape[0]), _p(x, ctypes.c_double), float(rc),
+                               ctypes.byref(h), err, 512)
+    if rc_ != MLIP_OK:
+        raise MlipError(rc_, err.value.decode())
+    try:
+        npairs = lib.mlip_nlist_npairs(h)
+        off = np.zeros(x.shape[0] + 1, dtype=np.int64)
+        idx = np.zeros(max(npairs, 1), dtype=np.int32)
+        sh = np.zeros((max(npairs, 1), 3))
+        lib.mlip_nlist_copy(h, _p(off, ctypes.c_int64), _p(idx, ctypes.c_int), _p(sh, ctypes.c_double))
+    finally:
+        lib.mlip_nlist_free(h)
+    return off, idx[:npairs], sh[:npairs]

@@ -323,6 +323,49 @@ def build_neighbor_list(struct, rcut, full=False, skin=0.0):
                         slot.astype(np.int32), full, owner=owner, shift=shift)
 
 
+# Benchmark workload models (the potential-file half of each configuration).
+CONFIG_MODELS = {
+    # 256-atom single-element fcc: cutoff 6, 10 Gaussians on [0,5], gtinv
+    # order 3 with maxl [8,4], linear model, coefficients N(0,1e-3), seed 2
+    "fcc256_order3_linear": dict(elements=["X"], masses=[26.98], cutoff=6.0, model_type=2, maxp=1, maxl=8,
+                                 gtinv_order=3, gtinv_maxl=[8, 4], gtinv_sym=[0, 0],
+                                 centers=(0.0, 5.0, 10), beta=1.0, sigma=1e-3, seed=2),
+    # 256,000-atom binary fcc: cutoff 8, 12 Gaussians on [0,7], gtinv order 4
+    # with maxl [12,8,2], quadratic polynomial in the order<=2 invariants
+    # (model_type 4, max_p 2), coefficients N(0,1e-4), seed 5
+    "binary_order4_quadratic": dict(elements=["Ti", "Al"], masses=[47.87, 26.98], cutoff=8.0, model_type=4,
+                                    maxp=2, maxl=12, gtinv_order=4, gtinv_maxl=[12, 8, 2], gtinv_sym=[0, 0, 0],
+                                    centers=(0.0, 7.0, 12), beta=1.0, sigma=1e-4, seed=5),
+}
+
+
+def write_config_potential(path, name, n_coeff):
+    """Write the named benchmark potential with seeded N(0, sigma^2)
+    coefficients; n_coeff comes from the model enumeration."""
+    c = CONFIG_MODELS[name]
+    lo, hi, n = c["centers"]
+    params = [(c["beta"], mu) for mu in np.linspace(lo, hi, n)]
+    rng = np.random.default_rng(c["seed"])
+    coeffs = rng.normal(0.0, c["sigma"], size=n_coeff)
+    write_potential(path, c["elements"], c["cutoff"], c["model_type"], c["maxp"], c["maxl"], c["gtinv_order"],
+                    c["gtinv_maxl"], c["gtinv_sym"], coeffs, params, c["masses"])
+    return path
+
+
+def config_potential(path, name, count_fn):
+    """Write the named potential; count_fn(path_of_probe_file) -> n_coeff."""
+    c = CONFIG_MODELS[name]
+    lo, hi, n = c["centers"]
+    params = [(c["beta"], mu) for mu in np.linspace(lo, hi, n)]
+    # a one-coefficient probe file lets the enumeration report the count
+    probe = path + ".probe"
+    write_potential(probe, c["elements"], c["cutoff"], c["model_type"], c["maxp"], c["maxl"], c["gtinv_order"],
+                    c["gtinv_maxl"], c["gtinv_sym"], [0.0], params, c["masses"])
+    n_coeff = count_fn(probe)
+    os.remove(probe)
+    return write_config_potential(path, name, n_coeff)
+
+
 def elem_of_type(pot_elements, type_elements):
     """Map LAMMPS types (1-based) to potential element indices, as pair_coeff
     does (pair_mlip_gtinv.cpp:573-582)."""
@@ -336,3 +379,16 @@ def elem_of_type(pot_elements, type_elements):
 
 def reference_data_path(*parts):
     return os.path.join(os.path.dirname(os.path.abspath(__file__)), *parts)
+
+
+def resident_lists(nl):
+    """Convert a full LAMMPS-shaped list into the device-resident form: for
+    every owned atom, neighbor indices into the owned atoms plus the image
+    shift s with del = x_i - x_j + s (s = x_owner(j) - x_slot(j))."""
+    if not nl.full:
+        raise ValueError("resident lists need a full neighbor list")
+    off = nl.off.astype(np.int64)
+    slots = nl.idx.astype(np.int64)
+    idx = nl.owner[slots].astype(np.int32)
+    shift = -nl.shift[slots]
+    return off, idx, np.ascontiguousarray(shift)
