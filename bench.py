@@ -1,0 +1,174 @@
+#!/usr/bin/env python3
+"""Benchmark of the gtinv energy/force path on one MI355X.
+
+Workload (BASELINE metric, 256k-atom configuration): binary random
+substitutional fcc, 40x40x40 conventional cells = 256,000 atoms, a = 4.0 A,
+Gaussian displacements sigma = 0.08 A (seed 4), species 50/50 (seed 4);
+potential r_c = 8 A, 12 Gaussians (beta 1, centres 0..7 A), gtinv order 4
+with maxl [12, 8, 2], model_type 4 / max_p 2 (quadratic in the order<=2
+invariants, 239,184 coefficients, N(0, 1e-4), seed 5).  One step = one full
+energy + force + virial evaluation of the whole structure with positions
+and the neighbor list resident in HBM.  value = atom-evaluations per second.
+
+Also reported on the same line:
+* roofline: the dominant kernel's algorithmic FP64 rate (FLOP model of
+  mlip_flops in the engine) over its average duration, measured with HIP
+  events on the engine's stream during the timed steps, against the
+  MI355X FP64 peak;
+* cpu_baseline: the CPU oracle (FP64 restatement of the reference CPU pair
+  style) timed on a bounded sub-cell of the same generator on this host.
+"""
+import argparse
+import json
+import os
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "lammps-mlip-package_amd"))
+
+from mlip_hip import PairMLIPGtinv, ResidentSystem, inspect_model, periodic_full_list  # noqa: E402
+from mlip_hip import workloads as W  # noqa: E402
+
+FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 (vector and matrix)
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+def make_workload(ncell, tmpdir):
+    pot = os.path.join(tmpdir, "binary_order4_quadratic.mlip")
+    W.config_potential(pot, "binary_order4_quadratic", lambda probe: inspect_model(probe, 4))
+    s = W.fcc(ncell, 4.0, 0.08, 4, type_fraction=0.5, type_seed=4)
+    return pot, s
+
+
+def cpu_baseline(pot, budget_s):
+    """Oracle on a 4x4x4-cell (256-atom) sub-cell of the same generator."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib
+    o = oracle_lib.Oracle(pot)
+    s = W.fcc(4, 4.0, 0.08, 4, type_fraction=0.5, type_seed=4)
+    nl = W.build_neighbor_list(s, o.cutoff, full=False)
+    n_eval, t_tot = 0, 0.0
+    while t_tot < budget_s or n_eval == 0:
+        t0 = time.perf_counter()
+        o.compute(nl, s.types - 1)
+        t_tot += time.perf_counter() - t0
+        n_eval += 1
+        if n_eval >= 50:
+            break
+    threads = oracle_lib.lib().oracle_num_threads()
+    return {"value": s.natoms * n_eval / t_tot, "unit": "atom-evals/s", "cores": int(threads),
+            "kind": "port",
+            "sample": "%d evaluation(s) of a 256-atom (4x4x4-cell) sub-cell of the same generator and model, "
+                      "%.1f s of CPU work; per-atom cost is size-independent" % (n_eval, t_tot)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--ncell", type=int, default=40, help="fcc cells per edge (40 -> 256,000 atoms)")
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU-oracle work")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="also launch a plain copy of this many MiB (HBM counter calibration)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build evaluates on one GPU; --gpus %d is not supported "
+              "(the multi-GPU decomposition is described in DESIGN.md)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    tmp = tempfile.mkdtemp(prefix="mlip_bench_")
+    pot, s = make_workload(args.ncell, tmp)
+    pair = PairMLIPGtinv()
+    pair.settings()
+    pair.coeff("*", "*", pot, "Ti", "Al")
+    pair.set_precision(args.precision)
+    log("model %s" % pair.model_info())
+    t0 = time.time()
+    off, idx, sh = periodic_full_list(s.cell, s.positions, pair.init_one())
+    log("neighbor list: %d atoms, %d pairs (%.1f per atom) in %.1fs"
+        % (s.natoms, idx.size, idx.size / s.natoms, time.time() - t0))
+    rs = ResidentSystem(pair, s.positions, s.types, off, idx, sh)
+    del off, idx, sh
+    for _ in range(args.warmup):
+        rs.compute()
+    rs.sync()
+    log("warmup done")
+    rs.profile(True)
+    rs.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        rs.compute()
+    rs.sync()
+    dt = time.perf_counter() - t0
+    prof = rs.read_profile()
+    rs.profile(False)
+    res = rs.get()
+    if not np.isfinite(res["energy"]) or not np.all(np.isfinite(res["forces"])):
+        raise SystemExit("non-finite result")
+    flops = rs.flops()
+    if args.calibrate_copy_mib > 0:
+        rs.calibration_copy(args.calibrate_copy_mib << 20, reps=3)
+
+    natoms = s.natoms
+    value = natoms * args.steps / dt
+    kernels = {k: {"avg_ms": v[0] / max(v[1], 1), "launches": v[1]} for k, v in prof.items()}
+    dom = max(prof.items(), key=lambda kv: kv[1][0])[0]
+    fl_key = {"q34_adj": "q34_adj", "q34_feat": "q34_feat", "anlm": "anlm", "force": "force",
+              "poly_gemm": "poly", "q12_adj": "q12", "q12_feat": "q12"}.get(dom)
+    dom_ms = kernels[dom]["avg_ms"]
+    achieved = flops[fl_key] / (dom_ms * 1e-3) / 1e12 if fl_key else None
+    roofline = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": (achieved / FP64_PEAK_TFLOPS) if achieved else None, "traffic": None,
+                "kernel": dom, "kernel_avg_ms": dom_ms,
+                "algorithmic_flops_per_launch": flops[fl_key] if fl_key else None,
+                "note": "FP64 VALU work; the MI355X FP64 peak is the same for vector and matrix units"}
+    total_rate = flops["total"] * args.steps / dt / 1e12
+    log("steps %d in %.3fs -> %.1f ms/step, %.4g atom-evals/s; whole-step FP64 rate %.2f TFLOP/s"
+        % (args.steps, dt, 1e3 * dt / args.steps, value, total_rate))
+    for k, v in sorted(kernels.items(), key=lambda kv: -kv[1]["avg_ms"]):
+        log("  %-14s %9.3f ms/launch" % (k, v["avg_ms"]))
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("timing the CPU oracle (bounded sample)")
+        cpu = cpu_baseline(pot, args.cpu_budget)
+    line = {
+        "metric": "atom-evals/s (E+F) FP64 vs mixed at 256k atoms; p50/p99 latency served",
+        "value": value,
+        "unit": "atom-evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * dt / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f32/f64 mixed",
+        "data": "synthetic (seeded fcc structure and random potential coefficients)",
+        "config": {"workload": "binary fcc %dx%dx%d cells = %d atoms, a=4.0 A, sigma=0.08 A; gtinv order 4 "
+                               "[12,8,2], model_type 4 max_p 2 (239,184 coeffs), r_c 8 A, 12 Gaussians; "
+                               "one E+F+virial evaluation per step, inputs resident in HBM"
+                               % (args.ncell, args.ncell, args.ncell, natoms),
+                   "natoms": natoms, "pairs_per_atom": rs.npairs / natoms, "precision": args.precision},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "kernels_ms": {k: round(v["avg_ms"], 4) for k, v in kernels.items()},
+        "flops_per_step": flops,
+        "step_tflops": total_rate,
+        "energy_per_atom": res["energy"] / natoms,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -90,10 +90,20 @@ int mlip_resident_compute(mlip_ctx* ctx);
 int mlip_resident_sync(mlip_ctx* ctx);
 int mlip_resident_get(mlip_ctx* ctx, double* f, double* energy, double* virial6, double* eatom);
 
-/* Kernel timing of the last resident evaluation, in milliseconds, measured
- * with HIP events on the context's stream: names[i] / ms[i], returns count. */
+/* Per-kernel device time, measured with HIP events on the context's own
+ * stream around every launch while enabled.  mlip_profile_enable resets the
+ * totals; mlip_profile_read returns, per kernel name, the summed
+ * milliseconds and the number of launches. */
 int mlip_profile_enable(mlip_ctx* ctx, int on);
-int mlip_profile_read(mlip_ctx* ctx, const char** names, double* ms, int cap);
+int mlip_profile_read(mlip_ctx* ctx, const char** names, double* ms, int* count, int cap);
+
+/* Algorithmic FP64 operation counts of one evaluation of the current
+ * structure, per kernel: a_nlm, order 1/2, order 3/4 features, order 3/4
+ * adjoint, polynomial, forces, total (see engine.hip for the rule). */
+int mlip_flops(const mlip_ctx* ctx, double* out7);
+
+/* Launches a plain device copy of nbytes `reps` times (byte-counter calibration). */
+int mlip_calibration_copy(mlip_ctx* ctx, int64_t nbytes, int reps);
 
 /* Diagnostics for parity tests: copy device intermediates of the last
  * evaluation to host.  what = 0: a_nlm [slot order], 1: polynomial features,

@@ -80,5 +80,7 @@ void launch_poly(const DevModel& m, const DevSystem& s, hipStream_t st);
 void launch_force(const DevModel& m, const DevSystem& s, int precision, hipStream_t st);
 void launch_gather(const DevModel& m, const DevSystem& s, hipStream_t st);
 void launch_energy_virial(const DevModel& m, const DevSystem& s, hipStream_t st);
+// plain streaming copy used to calibrate hardware byte counters
+void launch_copy(void* dst, const void* src, int64_t nbytes, int reps, hipStream_t st);
 
 }  // namespace mlip

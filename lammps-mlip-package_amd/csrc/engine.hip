@@ -73,11 +73,17 @@ struct mlip_ctx {
     DevSystem ds{};
     bool resident = false;
     int resident_natoms = 0;
-    // profiling
+    // profiling: events recorded around every launch while enabled, folded
+    // into per-name totals at each synchronisation
     bool prof = false;
     std::vector<ProfEvent> events;
     size_t ev_used = 0;
-    std::vector<std::pair<std::string, double>> last_prof;
+    std::vector<std::string> prof_names;
+    std::vector<double> prof_ms;
+    std::vector<int> prof_count;
+    // algorithmic work of the current structure (see mlip_flops)
+    std::vector<int64_t> owners_per_type;
+    int64_t npairs = 0;
 };
 
 namespace {
@@ -230,6 +236,10 @@ bool setup_system(mlip_ctx* c, int nall, int nown, int64_t npairs, const std::ve
     std::vector<int32_t> atom_of_slot, slot_of_atom, block_type;
     assign_slots(c, nown, elem_own, atom_of_slot, slot_of_atom, block_type);
     const int nblocks = static_cast<int>(block_type.size());
+    c->owners_per_type.assign(m.n_type, 0);
+    for (int e : elem_own)
+        if (e >= 0 && e < m.n_type) ++c->owners_per_type[e];
+    c->npairs = npairs;
     const size_t nslot = static_cast<size_t>(nblocks) * kBlock;
     const size_t asz = nslot * m.n_fn * m.n_type * m.n_lmh * 2;
     bool ok = upload(c, c->x, x.data(), x.size(), "positions") &&
@@ -342,7 +352,6 @@ bool run_pipeline(mlip_ctx* c) {
     const DevSystem& s = c->ds;
     const Model& mm = c->model;
     hipStream_t st = c->stream;
-    c->ev_used = 0;
     { Scope sc(c, "anlm"); launch_anlm(m, s, c->precision, st); }
     if (m.n_p > 0) {
         { Scope sc(c, "q12_feat"); launch_q12(m, s, true, st); }
@@ -369,13 +378,21 @@ bool run_pipeline(mlip_ctx* c) {
 }
 
 void collect_profile(mlip_ctx* c) {
-    c->last_prof.clear();
-    if (!c->prof) return;
     for (size_t k = 0; k < c->ev_used; ++k) {
         float ms = 0.0f;
         (void)hipEventElapsedTime(&ms, c->events[k].a, c->events[k].b);
-        c->last_prof.push_back({c->events[k].name, ms});
+        const std::string name = c->events[k].name;
+        size_t j = 0;
+        while (j < c->prof_names.size() && c->prof_names[j] != name) ++j;
+        if (j == c->prof_names.size()) {
+            c->prof_names.push_back(name);
+            c->prof_ms.push_back(0.0);
+            c->prof_count.push_back(0);
+        }
+        c->prof_ms[j] += ms;
+        c->prof_count[j] += 1;
     }
+    c->ev_used = 0;
 }
 
 int lammps_compute(mlip_ctx* c, int nlocal, int nghost, const double* x, const int* type, const int64_t* tag,
@@ -709,20 +726,101 @@ int mlip_resident_get(mlip_ctx* c, double* f, double* energy, double* virial, do
 
 int mlip_profile_enable(mlip_ctx* c, int on) {
     if (!c) return MLIP_ERR_ARG;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    collect_profile(c);
     c->prof = on != 0;
+    c->prof_names.clear();
+    c->prof_ms.clear();
+    c->prof_count.clear();
     return MLIP_OK;
 }
 
-int mlip_profile_read(mlip_ctx* c, const char** names, double* ms, int cap) {
+int mlip_profile_read(mlip_ctx* c, const char** names, double* ms, int* count, int cap) {
     if (!c) return 0;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    collect_profile(c);
     int n = 0;
-    for (const auto& p : c->last_prof) {
-        if (n >= cap) break;
-        names[n] = p.first.c_str();
-        ms[n] = p.second;
-        ++n;
+    for (size_t k = 0; k < c->prof_names.size() && n < cap; ++k, ++n) {
+        names[n] = c->prof_names[k].c_str();
+        ms[n] = c->prof_ms[k];
+        if (count) count[n] = c->prof_count[k];
     }
     return n;
+}
+
+// Algorithmic FP64 operation counts of one evaluation of the current
+// structure, per kernel (adds and multiplies count 1, fused multiply-adds 2):
+//   [0] a_nlm       per pair 4 n_fn n_lmh (accumulate) + 8 n_lmh (Y_lm) + 8 n_fn
+//   [1] order 1/2   per centre and n: 4 per order-1 family, 20 (L+1) per order-2 family
+//   [2] order 3/4 polynomial features  per valid term 11 (order 3) / 17 (order 4)
+//   [3] order 3/4 adjoint              per valid term 11 / 17, +5 where d is formed
+//   [4] polynomial  2 n_p^2 (quadratic form) + 6 n_p + 4 per cubic entry, per centre
+//   [5] forces      per pair 8 n_fn n_lmh (contraction) + 40 n_lmh + 14 n_fn
+//   [6] total
+int mlip_flops(const mlip_ctx* c, double* out7) {
+    if (!c || !out7) return MLIP_ERR_ARG;
+    const Model& m = c->model;
+    for (int k = 0; k < 7; ++k) out7[k] = 0.0;
+    const double pairs = static_cast<double>(c->npairs);
+    out7[0] = pairs * (4.0 * m.n_fn * m.n_lmh + 8.0 * m.n_lmh + 8.0 * m.n_fn);
+    out7[5] = pairs * (8.0 * m.n_fn * m.n_lmh + 40.0 * m.n_lmh + 14.0 * m.n_fn);
+    for (int t = 0; t < m.n_type && t < static_cast<int>(c->owners_per_type.size()); ++t) {
+        const double na = static_cast<double>(c->owners_per_type[t]) * m.n_fn;
+        const TypeTables& tt = m.types[t];
+        double q12 = 0.0;
+        for (const VisitQ12& v : tt.visits_q12) {
+            int L = 0;
+            for (int l = 0; l <= m.maxl; ++l)
+                if (&v - tt.visits_q12.data() >= tt.q12_off[l]) L = l;
+            q12 += v.order == 1 ? 4.0 : 20.0 * (L + 1);
+        }
+        out7[1] += na * q12;
+        auto terms = [&](const std::vector<GroupQ34>& gs, const std::vector<VisitQ34>& vs, bool adj) {
+            double f = 0.0;
+            for (const GroupQ34& g : gs)
+                for (int vi = g.vbeg; vi < g.vend; ++vi) {
+                    const VisitQ34& v = vs[vi];
+                    long cnt = 0;
+                    for (int ma = -v.la; ma <= v.la; ++ma) {
+                        if (v.order == 3) {
+                            for (int k = 0; k <= g.L; ++k)
+                                if (std::abs(k - ma) <= v.lb) ++cnt;
+                        } else {
+                            for (int mb = -v.lb; mb <= v.lb; ++mb)
+                                for (int k = 0; k <= g.L; ++k)
+                                    if (std::abs(k - ma - mb) <= v.lc) ++cnt;
+                        }
+                    }
+                    const double per = (v.order == 3 ? 11.0 : 17.0) + ((adj && (v.flags & 1)) ? 5.0 : 0.0);
+                    f += per * cnt;
+                }
+            return f;
+        };
+        out7[2] += na * terms(tt.groups_feat, tt.visits_feat, false);
+        out7[3] += na * terms(tt.groups_adj, tt.visits_adj, true);
+        const double np = m.n_p;
+        out7[4] += static_cast<double>(c->owners_per_type[t]) * (np > 0 ? 2.0 * np * np + 6.0 * np + 12.0 * m.c3_w.size() / np : 0.0);
+    }
+    for (int k = 0; k < 6; ++k) out7[6] += out7[k];
+    return MLIP_OK;
+}
+
+// Plain device copy of nbytes (calibrates HBM byte counters against a
+// kernel whose traffic is known exactly).
+int mlip_calibration_copy(mlip_ctx* c, int64_t nbytes, int reps) {
+    if (!c || nbytes <= 0) return MLIP_ERR_ARG;
+    void *a = nullptr, *b = nullptr;
+    if (hipMalloc(&a, nbytes) != hipSuccess || hipMalloc(&b, nbytes) != hipSuccess) {
+        if (a) (void)hipFree(a);
+        c->err = "calibration buffers";
+        return MLIP_ERR_DEVICE;
+    }
+    (void)hipMemsetAsync(a, 1, nbytes, c->stream);
+    launch_copy(b, a, nbytes, reps, c->stream);
+    const bool ok = hipStreamSynchronize(c->stream) == hipSuccess;
+    (void)hipFree(a);
+    (void)hipFree(b);
+    return ok ? MLIP_OK : MLIP_ERR_DEVICE;
 }
 
 int64_t mlip_debug_fetch(mlip_ctx* c, int what, double* out, int64_t cap) {

@@ -759,6 +759,11 @@ __global__ __launch_bounds__(kRedThreads) void k_reduce2(const double* part, int
     }
 }
 
+__global__ __launch_bounds__(256) void k_copy(double4* __restrict__ dst, const double4* __restrict__ src, int64_t n) {
+    for (int64_t i = blockIdx.x * 256LL + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+        dst[i] = src[i];
+}
+
 void check(hipError_t e, const char* what) {
     if (e != hipSuccess) {
         std::fprintf(stderr, "mlip_hip: %s failed: %s\n", what, hipGetErrorString(e));
@@ -870,6 +875,14 @@ void launch_gather(const DevModel&, const DevSystem& s, hipStream_t st) {
     if (s.nall == 0) return;
     hipLaunchKernelGGL(k_gather, dim3((s.nall + 255) / 256), dim3(256), 0, st, s);
     check(hipGetLastError(), "k_gather launch");
+}
+
+void launch_copy(void* dst, const void* src, int64_t nbytes, int reps, hipStream_t st) {
+    const int64_t n = nbytes / static_cast<int64_t>(sizeof(double4));
+    for (int r = 0; r < reps; ++r)
+        hipLaunchKernelGGL(k_copy, dim3(2048), dim3(256), 0, st, static_cast<double4*>(dst),
+                           static_cast<const double4*>(src), n);
+    check(hipGetLastError(), "k_copy launch");
 }
 
 void launch_energy_virial(const DevModel& m, const DevSystem& s, hipStream_t st) {

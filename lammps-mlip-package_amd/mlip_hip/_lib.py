@@ -57,7 +57,9 @@ def library():
     L.mlip_resident_sync.argtypes = [vp]
     L.mlip_resident_get.argtypes = [vp, dp, dp, dp, dp]
     L.mlip_profile_enable.argtypes = [vp, c_int]
-    L.mlip_profile_read.argtypes = [vp, ctypes.POINTER(c_char_p), dp, c_int]
+    L.mlip_profile_read.argtypes = [vp, ctypes.POINTER(c_char_p), dp, ip, c_int]
+    L.mlip_flops.argtypes = [vp, dp]
+    L.mlip_calibration_copy.argtypes = [vp, ctypes.c_int64, c_int]
     L.mlip_debug_fetch.argtypes = [vp, c_int, dp, ctypes.c_int64]
     L.mlip_debug_fetch.restype = ctypes.c_int64
     L.mlip_model_info.argtypes = [vp, ip]

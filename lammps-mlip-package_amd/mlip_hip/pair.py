@@ -167,10 +167,21 @@ class ResidentSystem:
         self.lib.mlip_profile_enable(self.pair.ctx, 1 if on else 0)
 
     def read_profile(self):
+        """{kernel: (total_ms, launches)} since profile(True)."""
         names = (ctypes.c_char_p * 64)()
         ms = np.zeros(64)
-        n = self.lib.mlip_profile_read(self.pair.ctx, names, _p(ms, ctypes.c_double), 64)
-        return [(names[k].decode(), float(ms[k])) for k in range(n)]
+        cnt = np.zeros(64, dtype=np.int32)
+        n = self.lib.mlip_profile_read(self.pair.ctx, names, _p(ms, ctypes.c_double), _p(cnt, ctypes.c_int), 64)
+        return {names[k].decode(): (float(ms[k]), int(cnt[k])) for k in range(n)}
+
+    def flops(self):
+        out = np.zeros(7)
+        self.pair._check(self.lib.mlip_flops(self.pair.ctx, _p(out, ctypes.c_double)))
+        keys = ["anlm", "q12", "q34_feat", "q34_adj", "poly", "force", "total"]
+        return dict(zip(keys, out.tolist()))
+
+    def calibration_copy(self, nbytes, reps=1):
+        self.pair._check(self.lib.mlip_calibration_copy(self.pair.ctx, int(nbytes), int(reps)))
 
 
 def inspect_model(potfile, which, cap=1 << 26):
