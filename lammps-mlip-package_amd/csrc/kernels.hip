@@ -84,9 +84,9 @@ __device__ __forceinline__ PairGeom pair_geom(const DevModel& M, const DevSystem
 }
 
 // ---------------------------------------------------------------- a_nlm
-// One workgroup per centre.  Phase 1: threads 0..31 each take a neighbor,
-// evaluate g_n and all Y_lm (m <= 0) into LDS.  Phase 2: thread = lm,
-// accumulates sum_j g_n(j) Y_lm(j) for both species in registers.
+// One workgroup per centre.  Phase 1: four threads per neighbor of a chunk
+// of 32 evaluate g_n and the Y_lm (m <= 0) columns into LDS.  Phase 2:
+// thread = lm, accumulates sum_j g_n(j) Y_lm(j) for both species in registers.
 template <int LMAX>
 __global__ __launch_bounds__(128) void k_anlm(DevModel M, DevSystem S) {
     constexpr int NLMH = (LMAX + 1) * (LMAX + 2) / 2;
@@ -104,50 +104,54 @@ __global__ __launch_bounds__(128) void k_anlm(DevModel M, DevSystem S) {
     for (int n = 0; n < kMaxFn; ++n) a0r[n] = a0i[n] = a1r[n] = a1i[n] = 0.0;
     const int p0 = S.poff[atom], p1 = S.poff[atom + 1];
     for (int c0 = p0; c0 < p1; c0 += kAnlmChunk) {
-        if (tid < kAnlmChunk) {
-            const int64_t p = c0 + tid;
+        {
+            // four threads per neighbor: part k evaluates the radial functions
+            // n = k, k+4, ... and the Y_lm columns m = k, k+4, ...
+            const int j = tid >> 2, part = tid & 3;
+            const int64_t p = c0 + j;
             int spec = -1;
             if (p < p1) {
                 PairGeom g = pair_geom(M, S, atom, p);
                 if (g.valid) {
                     spec = S.elem[S.pnbr[p]];
                     const double fc = 0.5 * (cos(kPiRef * g.r / M.cutoff) + 1.0);
-#pragma unroll
-                    for (int n = 0; n < kMaxFn; ++n)
-                        if (n < nfn) {
-                            const double d = g.r - M.mu[n];
-                            gs[tid][n] = exp(-M.beta[n] * (d * d)) * fc;
-                        }
+#pragma unroll 1
+                    for (int n = part; n < nfn; n += 4) {
+                        const double d = g.r - M.mu[n];
+                        gs[j][n] = exp(-M.beta[n] * (d * d)) * fc;
+                    }
                     const double ir = 1.0 / g.r;
                     const double x = g.dx * ir, y = g.dy * ir, z = g.dz * ir;
-                    // columns m = 0..LMAX, stored Y_{l,-m} = (-1)^m Qb_l^m conj(u)^m
+                    // conj(u)^part and conj(u)^4
                     double vr = 1.0, vi = 0.0;
-#pragma unroll
-                    for (int m = 0; m <= LMAX; ++m) {
+                    for (int k = 0; k < part; ++k) {
+                        const double nr = vr * x + vi * y, ni = vi * x - vr * y;
+                        vr = nr;
+                        vi = ni;
+                    }
+                    const double u2r = x * x - y * y, u2i = -2.0 * x * y;
+                    const double u4r = u2r * u2r - u2i * u2i, u4i = 2.0 * u2r * u2i;
+                    // stored Y_{l,-m} = (-1)^m Qb_l^m conj(u)^m
+#pragma unroll 1
+                    for (int m = part; m <= LMAX; m += 4) {
                         const double sg = (m & 1) ? -1.0 : 1.0;
                         double q2 = 0.0, q1 = c_diag[m];
-                        ys[tid][lmh(m, -m)] = make_double2(sg * q1 * vr, sg * q1 * vi);
-                        if (m + 1 <= LMAX) {
-                            const double q = c_sub[m] * z * q1;
-                            ys[tid][lmh(m + 1, -m)] = make_double2(sg * q * vr, sg * q * vi);
+                        ys[j][lmh(m, -m)] = make_double2(sg * q1 * vr, sg * q1 * vi);
+#pragma unroll 1
+                        for (int l = m + 1; l <= LMAX; ++l) {
+                            const double q = (l == m + 1) ? c_sub[m] * z * q1
+                                                          : c_alm[l][m] * (z * q1 - c_ainv[l][m] * q2);
+                            ys[j][lmh(l, -m)] = make_double2(sg * q * vr, sg * q * vi);
                             q2 = q1;
                             q1 = q;
                         }
-#pragma unroll
-                        for (int l = m + 2; l <= LMAX; ++l) {
-                            const double q = c_alm[l][m] * (z * q1 - c_ainv[l][m] * q2);
-                            ys[tid][lmh(l, -m)] = make_double2(sg * q * vr, sg * q * vi);
-                            q2 = q1;
-                            q1 = q;
-                        }
-                        const double nr = vr * x + vi * y;
-                        const double ni = vi * x - vr * y;
+                        const double nr = vr * u4r - vi * u4i, ni = vr * u4i + vi * u4r;
                         vr = nr;
                         vi = ni;
                     }
                 }
             }
-            sp[tid] = spec;
+            if (part == 0) sp[j] = spec;
         }
         __syncthreads();
         const int nc = min(kAnlmChunk, p1 - c0);
@@ -573,33 +577,37 @@ __global__ __launch_bounds__(128) void k_force(DevModel M, DevSystem S) {
             double er = 0.0;      // sum w Re(Gp Y)
             double tx = 0.0, ty = 0.0, tz = 0.0;  // sum w Re(Gpp grad Y) * r
             const double zz = 1.0 - z * z;
-            double col[LMAX + 2], nxt[LMAX + 2];
-            // column 0
-            col[0] = c_diag[0];
-            if (LMAX >= 1) col[1] = c_sub[0] * z * col[0];
-#pragma unroll
-            for (int l = 2; l <= LMAX; ++l) col[l] = c_alm[l][0] * (z * col[l - 1] - c_ainv[l][0] * col[l - 2]);
-            // powers of conj(u): vm1 = conj(u)^(m-1), v0c = conj(u)^m, vp1 = conj(u)^(m+1)
+            // Columns m and m+1 of the recursion advance in lockstep along l,
+            // each carried in two scalars, so no per-lane array is indexed.
             double cr = 1.0, ci = 0.0;  // conj(u)^m
             double mr = 0.0, mi = 0.0;  // conj(u)^(m-1), used for m >= 1
-#pragma unroll
+#pragma unroll 1
             for (int m = 0; m <= LMAX; ++m) {
-                // next column m+1 (Qb_l^{m+1}, l >= m+1)
-#pragma unroll
-                for (int l = 0; l <= LMAX + 1; ++l) nxt[l] = 0.0;
-                if (m + 1 <= LMAX) {
-                    nxt[m + 1] = c_diag[m + 1];
-                    if (m + 2 <= LMAX) nxt[m + 2] = c_sub[m + 1] * z * nxt[m + 1];
-#pragma unroll
-                    for (int l = m + 3; l <= LMAX; ++l)
-                        nxt[l] = c_alm[l][m + 1] * (z * nxt[l - 1] - c_ainv[l][m + 1] * nxt[l - 2]);
-                }
                 const double sg = (m & 1) ? -1.0 : 1.0;
                 const double w = (m == 0) ? 1.0 : 2.0;
-                // conj(u)^(m+1) and conj(u)^(m-1) (the latter only used for m >= 1)
-                const double pr = cr * x + ci * y, pi = ci * x - cr * y;
-#pragma unroll
+                const double pr = cr * x + ci * y, pi = ci * x - cr * y;  // conj(u)^(m+1)
+                double a1 = 0.0, a2 = 0.0;  // Qb_{l-1}^m, Qb_{l-2}^m
+                double b1 = 0.0, b2 = 0.0;  // Qb_{l-1}^{m+1}, Qb_{l-2}^{m+1}
+#pragma unroll 1
                 for (int l = m; l <= LMAX; ++l) {
+                    double q;
+                    if (l == m)
+                        q = c_diag[m];
+                    else if (l == m + 1)
+                        q = c_sub[m] * z * a1;
+                    else
+                        q = c_alm[l][m] * (z * a1 - c_ainv[l][m] * a2);
+                    a2 = a1;
+                    a1 = q;
+                    double qp = 0.0;  // Qb_l^{m+1}
+                    if (l == m + 1)
+                        qp = c_diag[m + 1];
+                    else if (l == m + 2)
+                        qp = c_sub[m + 1] * z * b1;
+                    else if (l > m + 2)
+                        qp = c_alm[l][m + 1] * (z * b1 - c_ainv[l][m + 1] * b2);
+                    b2 = b1;
+                    b1 = qp;
                     const int lm = lmh(l, -m);
                     // contract G over n
                     double gpr = 0.0, gpi = 0.0, gqr = 0.0, gqi = 0.0;
@@ -613,14 +621,13 @@ __global__ __launch_bounds__(128) void k_force(DevModel M, DevSystem S) {
                             gqr = fma(gn[n], grr, gqr);
                             gqi = fma(gn[n], gii, gqi);
                         }
-                    const double q = col[l];
                     // Y_{l,-m} = sg q conj(u)^m
                     const double yr = sg * q * cr, yi = sg * q * ci;
                     er = fma(w, gpr * yr - gpi * yi, er);
                     // grad Y_{l,-m} * r = sg conj(grad Y_l^m * r)
                     double gxr, gxi, gyr, gyi, gzr, gzi;
                     if (m == 0) {
-                        const double c = c_grad[l][0] * nxt[l];  // Qb_l^1 (0 for l = 0)
+                        const double c = c_grad[l][0] * qp;  // Qb_l^1 (0 for l = 0)
                         gxr = c * z * x;
                         gxi = 0.0;
                         gyr = c * z * y;
@@ -628,18 +635,15 @@ __global__ __launch_bounds__(128) void k_force(DevModel M, DevSystem S) {
                         gzr = -c * zz;
                         gzi = 0.0;
                     } else {
-                        const double qp = (l >= m + 1) ? nxt[l] : 0.0;
                         const double c = c_grad[l][m];
                         const double A = m * z * q + zz * c * qp;
                         const double B = m * q - z * c * qp;
                         const double E = m * (1.0 + z * z) * q + z * zz * c * qp;
-                        // conj of: gx = (-B u^{m+1} + E u^{m-1})/2, gy = (i/2)(B u^{m+1} + E u^{m-1}), gz = -A u^m
-                        // with conj(u^k) = conj(u)^k: P = conj(u)^{m+1} = (pr,pi), Q = conj(u)^{m-1}
-                        const double qr_ = mr, qi_ = mi;
-                        gxr = 0.5 * (-B * pr + E * qr_);
-                        gxi = 0.5 * (-B * pi + E * qi_);
-                        // conj((i/2) Z) = (-i/2) conj(Z), Z = B u^{m+1} + E u^{m-1}
-                        const double zr_ = B * pr + E * qr_, zi_ = B * pi + E * qi_;
+                        // conj of: gx = (-B u^{m+1} + E u^{m-1})/2, gy = (i/2)(B u^{m+1} + E u^{m-1}),
+                        // gz = -A u^m, with conj(u)^{m+1} = (pr, pi), conj(u)^{m-1} = (mr, mi)
+                        gxr = 0.5 * (-B * pr + E * mr);
+                        gxi = 0.5 * (-B * pi + E * mi);
+                        const double zr_ = B * pr + E * mr, zi_ = B * pi + E * mi;
                         gyr = 0.5 * zi_;
                         gyi = -0.5 * zr_;
                         gzr = -A * cr;
@@ -654,8 +658,6 @@ __global__ __launch_bounds__(128) void k_force(DevModel M, DevSystem S) {
                 mi = ci;
                 cr = pr;
                 ci = pi;
-#pragma unroll
-                for (int l = 0; l <= LMAX + 1; ++l) col[l] = nxt[l];
             }
             // dE/d(del) = er * rhat + t / r
             fx = er * x + tx * ir;
