@@ -75,7 +75,8 @@ void init_device_constants();
 
 void launch_anlm(const DevModel& m, const DevSystem& s, int precision, hipStream_t st);
 void launch_q12(const DevModel& m, const DevSystem& s, bool feat, hipStream_t st);
-void launch_q34(const DevModel& m, const DevSystem& s, bool feat, hipStream_t st);
+void launch_q34_adj(const DevModel& m, const DevSystem& s, hipStream_t st);
+void launch_q34_feat(const DevModel& m, const DevSystem& s, hipStream_t st);
 void launch_poly(const DevModel& m, const DevSystem& s, hipStream_t st);
 void launch_force(const DevModel& m, const DevSystem& s, int precision, hipStream_t st);
 void launch_gather(const DevModel& m, const DevSystem& s, hipStream_t st);

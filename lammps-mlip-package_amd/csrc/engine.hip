@@ -355,7 +355,7 @@ bool run_pipeline(mlip_ctx* c) {
     { Scope sc(c, "anlm"); launch_anlm(m, s, c->precision, st); }
     if (m.n_p > 0) {
         { Scope sc(c, "q12_feat"); launch_q12(m, s, true, st); }
-        if (mm.has_feat34) { Scope sc(c, "q34_feat"); launch_q34(m, s, true, st); }
+        if (mm.has_feat34) { Scope sc(c, "q34_feat"); launch_q34_feat(m, s, st); }
         {
             Scope sc(c, "poly_gemm");
             const double one = 1.0, zero = 0.0;
@@ -370,7 +370,7 @@ bool run_pipeline(mlip_ctx* c) {
         { Scope sc(c, "poly"); launch_poly(m, s, st); }
     }
     { Scope sc(c, "q12_adj"); launch_q12(m, s, false, st); }
-    if (mm.has_order34) { Scope sc(c, "q34_adj"); launch_q34(m, s, false, st); }
+    if (mm.has_order34) { Scope sc(c, "q34_adj"); launch_q34_adj(m, s, st); }
     { Scope sc(c, "force"); launch_force(m, s, c->precision, st); }
     { Scope sc(c, "gather"); launch_gather(m, s, st); }
     { Scope sc(c, "energy_virial"); launch_energy_virial(m, s, st); }

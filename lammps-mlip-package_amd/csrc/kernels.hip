@@ -22,10 +22,13 @@
 #include <cstdio>
 
 #include "device_api.h"
+#include "device_common.h"
 
 namespace mlip {
 
 namespace {
+
+using namespace dev;
 
 constexpr double kPiRef = 3.141592653589793;  // mlip_basis_function.cpp:26
 constexpr int kL1 = kMaxL + 1;
@@ -40,26 +43,6 @@ __constant__ double c_sub[kL1];
 __constant__ double c_alm[kL1][kL1];
 __constant__ double c_ainv[kL1][kL1];
 __constant__ double c_grad[kL1][kL1];  // sqrt((l+m+1)(l-m))
-
-__device__ __forceinline__ int lmh(int l, int m) { return l * (l + 1) / 2 + l + m; }
-
-// XCD-aware mapping of a 1-D grid onto slots: consecutive workgroups of one
-// XCD (workgroups are dealt round-robin over the 8 XCDs) walk consecutive
-// lanes of the same slot block, so the block-interleaved a/G rows they touch
-// share that XCD's L2 lines.
-__device__ __forceinline__ int xcd_slot(int wg, int nblocks) {
-    const int xcd = wg & 7;
-    const int k = wg >> 3;
-    const int blk = xcd + 8 * (k / kBlock);
-    if (blk >= nblocks) return -1;
-    return blk * kBlock + (k % kBlock);
-}
-
-__device__ __forceinline__ size_t a_index(const DevModel& M, int blk, int n, int s, int lm, int c, int lane) {
-    return (((((size_t)blk * M.n_fn + n) * M.n_type + s) * M.n_lmh + lm) * 2 + c) * kBlock + lane;
-}
-
-__device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
 struct PairGeom {
     double dx, dy, dz, r;
@@ -307,190 +290,6 @@ __global__ __launch_bounds__(64) void k_q12(DevModel M, DevSystem S) {
     }
     Q12Levels<0, kMaxL, FEAT>::run(M, S, blk, n, lane, t, e);
     if (!FEAT) S.Eq12[(size_t)(blk * kBlock + lane) * M.n_fn + n] = e;
-}
-
-// ---------------------------------------------------------------- order 3/4
-// Workgroup = 8 waves on one (block of 64 same-type centres, n).  The a_nlm
-// tile (l <= lmax3, both species) sits in LDS as [s][lm][re,im][lane]; each
-// wave owns whole (S, L) output groups and keeps G_{S,L,m<=0} in registers.
-struct Tile {
-    const double* As;
-    int nlmh3;
-    int lane;
-    __device__ __forceinline__ void get(int s, int l, int m, double& re, double& im) const {
-        const int am = m < 0 ? -m : m;
-        const int idx = s * nlmh3 + l * (l + 1) / 2 + l - am;
-        re = As[(2 * idx) * kBlock + lane];
-        im = As[(2 * idx + 1) * kBlock + lane];
-        if (m > 0) {
-            const double sg = (m & 1) ? -1.0 : 1.0;
-            re *= sg;
-            im *= -sg;
-        }
-    }
-};
-
-template <int L, bool DOG, bool DOD>
-__device__ __forceinline__ void visit3(const Tile& T, const DevModel& M, const VisitQ34& v, double H,
-                                       double (&gr)[L + 1], double (&gi)[L + 1], const double (&lr)[L + 1],
-                                       const double (&li)[L + 1], double& d) {
-    const int la = uniform(v.la), lb = uniform(v.lb), sa = uniform(v.sa), sb = uniform(v.sb);
-    const double* cg = M.coef + uniform(v.coef);
-    const double* cd = M.coef + uniform(v.dcoef);
-    for (int ia = 0; ia <= 2 * la; ++ia) {
-        const int ma = ia - la;
-        double xr, xi;
-        T.get(sa, la, ma, xr, xi);
-        const double* cgk = cg + ia * (L + 1);
-        const double* cdk = cd + ia * (L + 1);
-#pragma unroll
-        for (int k = 0; k <= L; ++k) {
-            const int mb = k - ma;
-            if (mb < -lb || mb > lb) continue;
-            double yr, yi;
-            T.get(sb, lb, mb, yr, yi);
-            const double pr = xr * yr - xi * yi;
-            const double pi = xr * yi + xi * yr;
-            if (DOG) {
-                const double hc = H * cgk[k];
-                gr[k] = fma(hc, pr, gr[k]);
-                gi[k] = fma(hc, pi, gi[k]);
-            }
-            if (DOD) d = fma(cdk[k], lr[k] * pr - li[k] * pi, d);
-        }
-    }
-}
-
-template <int L, bool DOG, bool DOD>
-__device__ __forceinline__ void visit4(const Tile& T, const DevModel& M, const VisitQ34& v, double H,
-                                       double (&gr)[L + 1], double (&gi)[L + 1], const double (&lr)[L + 1],
-                                       const double (&li)[L + 1], double& d) {
-    const int la = uniform(v.la), lb = uniform(v.lb), lc = uniform(v.lc);
-    const int sa = uniform(v.sa), sb = uniform(v.sb), sc = uniform(v.sc);
-    const double* cg = M.coef + uniform(v.coef);
-    const double* cd = M.coef + uniform(v.dcoef);
-    const int nb = 2 * lb + 1;
-    for (int ia = 0; ia <= 2 * la; ++ia) {
-        const int ma = ia - la;
-        double xr, xi;
-        T.get(sa, la, ma, xr, xi);
-        for (int ib = 0; ib < nb; ++ib) {
-            const int mb = ib - lb;
-            double yr, yi;
-            T.get(sb, lb, mb, yr, yi);
-            const double qr = xr * yr - xi * yi;
-            const double qi = xr * yi + xi * yr;
-            const double* cgk = cg + (ia * nb + ib) * (L + 1);
-            const double* cdk = cd + (ia * nb + ib) * (L + 1);
-#pragma unroll
-            for (int k = 0; k <= L; ++k) {
-                const int mc = k - ma - mb;
-                if (mc < -lc || mc > lc) continue;
-                double zr, zi;
-                T.get(sc, lc, mc, zr, zi);
-                const double pr = qr * zr - qi * zi;
-                const double pi = qr * zi + qi * zr;
-                if (DOG) {
-                    const double hc = H * cgk[k];
-                    gr[k] = fma(hc, pr, gr[k]);
-                    gi[k] = fma(hc, pi, gi[k]);
-                }
-                if (DOD) d = fma(cdk[k], lr[k] * pr - li[k] * pi, d);
-            }
-        }
-    }
-}
-
-template <int L, bool FEAT>
-__device__ void q34_group(const Tile& T, const DevModel& M, const DevSystem& S, const GroupQ34& g, int blk,
-                          int n, double& eacc) {
-    double gr[L + 1], gi[L + 1], lr[L + 1], li[L + 1];
-    const int Sx = uniform(g.S);
-#pragma unroll
-    for (int k = 0; k <= L; ++k) {
-        gr[k] = gi[k] = 0.0;
-        T.get(Sx, L, -k, lr[k], li[k]);
-    }
-    const int slot = blk * kBlock + T.lane;
-    const VisitQ34* vis = FEAT ? M.vfeat : M.vadj;
-    const int vb = uniform(g.vbeg), ve = uniform(g.vend);
-    for (int vi = vb; vi < ve; ++vi) {
-        const VisitQ34 v = vis[vi];
-        const int order = uniform(v.order), flags = uniform(v.flags), pslot = uniform(v.pslot);
-        const int itc = uniform(v.itc);
-        double d = 0.0;
-        if (FEAT) {
-            if (order == 3)
-                visit3<L, false, true>(T, M, v, 0.0, gr, gi, lr, li, d);
-            else
-                visit4<L, false, true>(T, M, v, 0.0, gr, gi, lr, li, d);
-            S.D[(size_t)slot * M.n_p + n * M.n_pitc + pslot] = d;
-            continue;
-        }
-        const double H =
-            pslot >= 0 ? S.H[(size_t)slot * M.n_p + n * M.n_pitc + pslot] : M.weight[n * M.n_gtinv + itc];
-        if (flags & 1) {
-            if (order == 3)
-                visit3<L, true, true>(T, M, v, H, gr, gi, lr, li, d);
-            else
-                visit4<L, true, true>(T, M, v, H, gr, gi, lr, li, d);
-            eacc = fma(M.weight[n * M.n_gtinv + itc], d, eacc);
-        } else {
-            if (order == 3)
-                visit3<L, true, false>(T, M, v, H, gr, gi, lr, li, d);
-            else
-                visit4<L, true, false>(T, M, v, H, gr, gi, lr, li, d);
-        }
-    }
-    if (!FEAT) {
-#pragma unroll
-        for (int k = 0; k <= L; ++k) {
-            const size_t ir = a_index(M, blk, n, Sx, lmh(L, -k), 0, T.lane);
-            const size_t ii = a_index(M, blk, n, Sx, lmh(L, -k), 1, T.lane);
-            S.G[ir] += gr[k];
-            S.G[ii] += gi[k];
-        }
-    }
-}
-
-constexpr int kQ34Threads = 64 * kWavesPerWg;
-
-template <bool FEAT>
-__global__ __launch_bounds__(kQ34Threads) void k_q34(DevModel M, DevSystem S) {
-    extern __shared__ double As[];
-    const int blk = blockIdx.x, n = blockIdx.y;
-    const int t = S.block_type[blk];
-    const int per_s = M.n_lmh3 * 2 * kBlock;
-    for (int s = 0; s < M.n_type; ++s) {
-        const double* src = S.A + a_index(M, blk, n, s, 0, 0, 0);
-        for (int k = threadIdx.x; k < per_s; k += kQ34Threads) As[s * per_s + k] = src[k];
-    }
-    __syncthreads();
-    const int wave = uniform(threadIdx.x >> 6);
-    Tile T{As, M.n_lmh3, static_cast<int>(threadIdx.x & 63)};
-    const GroupQ34* groups = FEAT ? M.gfeat : M.gadj;
-    const int gb = FEAT ? M.gfeat_beg[t] : M.gadj_beg[t];
-    const int ge = FEAT ? M.gfeat_end[t] : M.gadj_end[t];
-    double eacc = 0.0;
-    for (int gi = gb; gi < ge; ++gi) {
-        const GroupQ34 g = groups[gi];
-        if (uniform(g.wave) != wave) continue;
-        switch (uniform(g.L)) {
-            case 0: q34_group<0, FEAT>(T, M, S, g, blk, n, eacc); break;
-            case 1: q34_group<1, FEAT>(T, M, S, g, blk, n, eacc); break;
-            case 2: q34_group<2, FEAT>(T, M, S, g, blk, n, eacc); break;
-            case 3: q34_group<3, FEAT>(T, M, S, g, blk, n, eacc); break;
-            case 4: q34_group<4, FEAT>(T, M, S, g, blk, n, eacc); break;
-            case 5: q34_group<5, FEAT>(T, M, S, g, blk, n, eacc); break;
-            case 6: q34_group<6, FEAT>(T, M, S, g, blk, n, eacc); break;
-            case 7: q34_group<7, FEAT>(T, M, S, g, blk, n, eacc); break;
-            case 8: q34_group<8, FEAT>(T, M, S, g, blk, n, eacc); break;
-            case 9: q34_group<9, FEAT>(T, M, S, g, blk, n, eacc); break;
-            case 10: q34_group<10, FEAT>(T, M, S, g, blk, n, eacc); break;
-            default: break;
-        }
-    }
-    if (!FEAT) S.Eq34[((size_t)(blk * kBlock + T.lane) * M.n_fn + n) * kWavesPerWg + wave] = eacc;
 }
 
 // ---------------------------------------------------------------- polynomial
@@ -828,23 +627,6 @@ void launch_q12(const DevModel& m, const DevSystem& s, bool feat, hipStream_t st
     else
         hipLaunchKernelGGL(k_q12<false>, dim3(s.nblocks, m.n_fn), dim3(64), 0, st, m, s);
     check(hipGetLastError(), "k_q12 launch");
-}
-
-void launch_q34(const DevModel& m, const DevSystem& s, bool feat, hipStream_t st) {
-    if (s.nblocks == 0) return;
-    const size_t lds = sizeof(double) * m.n_type * m.n_lmh3 * 2 * kBlock;
-    if (feat) {
-        check(hipFuncSetAttribute(reinterpret_cast<const void*>(k_q34<true>),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)),
-              "k_q34 lds attr");
-        hipLaunchKernelGGL(k_q34<true>, dim3(s.nblocks, m.n_fn), dim3(kQ34Threads), lds, st, m, s);
-    } else {
-        check(hipFuncSetAttribute(reinterpret_cast<const void*>(k_q34<false>),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)),
-              "k_q34 lds attr");
-        hipLaunchKernelGGL(k_q34<false>, dim3(s.nblocks, m.n_fn), dim3(kQ34Threads), lds, st, m, s);
-    }
-    check(hipGetLastError(), "k_q34 launch");
 }
 
 void launch_poly(const DevModel& m, const DevSystem& s, hipStream_t st) {

@@ -11,6 +11,7 @@
 #include <map>
 #include <set>
 #include <sstream>
+#include <tuple>
 
 extern "C" {
 extern const char mlip_gtinv_table_begin[];
@@ -235,6 +236,9 @@ void build_type_tables(Model& m, int t, const std::vector<std::map<std::vector<i
             std::vector<int> oth;
             for (int k = 0; k < q; ++k)
                 if (k != p) oth.push_back(k);
+            // other factors ordered by angular momentum: the product is
+            // symmetric in them, and the unrolled kernel expects l_a <= l_b <= l_c
+            std::stable_sort(oth.begin(), oth.end(), [&](int x, int y) { return ir.l[x] < ir.l[y]; });
             const int L = ir.l[p];
             VisitQ34 v{};
             v.order = q;
@@ -248,11 +252,16 @@ void build_type_tables(Model& m, int t, const std::vector<std::map<std::vector<i
             v.pslot = pslot;
             const bool is_poly = pslot >= 0;
             const bool with_d = (p == 0) && !is_poly;
-            v.flags = (with_d ? 1 : 0) | 2;
+            int lmax_v = std::max(L, std::max(v.la, v.lb));
+            if (q == 4) lmax_v = std::max(lmax_v, v.lc);
+            const bool fast = (q == 3 && lmax_v <= kUnrollCap3) || (q == 4 && lmax_v <= kUnrollCap4);
+            v.flags = (with_d ? 1 : 0) | 2 | (fast ? 4 : 0);
             std::vector<int> mt(q);
             long cost = 0;
             auto emit = [&](bool dense_d, bool g_only_sum) {
-                // dense block over (ia[, ib], k); k <-> m_p = -k
+                // block over (ia[, ib], k); k <-> m_p = -k.  Unrolled visits
+                // keep only the valid terms, in loop order; the general loop
+                // reads a dense block with zero padding.
                 const int nA = 2 * v.la + 1, nB = q == 4 ? 2 * v.lb + 1 : 1;
                 for (int ia = 0; ia < nA; ++ia)
                     for (int ib = 0; ib < nB; ++ib)
@@ -285,7 +294,7 @@ void build_type_tables(Model& m, int t, const std::vector<std::map<std::vector<i
                                 }
                                 ++cost;
                             }
-                            push_coef(val);
+                            if (ok || !fast) push_coef(val);
                         }
             };
             v.coef = static_cast<int>(m.coef.size());
@@ -298,7 +307,7 @@ void build_type_tables(Model& m, int t, const std::vector<std::map<std::vector<i
             adj.push_back({sp[p], L, v, cost});
             if (p == 0 && is_poly) {
                 VisitQ34 f = v;
-                f.flags = 1;
+                f.flags = 1 | (fast ? 4 : 0);
                 f.dcoef = static_cast<int>(m.coef.size());
                 f.coef = f.dcoef;
                 emit(true, false);
@@ -315,8 +324,14 @@ void build_type_tables(Model& m, int t, const std::vector<std::map<std::vector<i
     // group by (S, L), balance groups over the waves of a workgroup
     auto make_groups = [&](std::vector<Pending>& pend, std::vector<GroupQ34>& groups,
                            std::vector<VisitQ34>& visits) {
+        // group by (S, L); inside a group, visits with the same code path
+        // (signature and flags) are adjacent for instruction-cache locality
         std::stable_sort(pend.begin(), pend.end(), [](const Pending& a, const Pending& b) {
-            return a.S != b.S ? a.S < b.S : a.L < b.L;
+            if (a.S != b.S) return a.S < b.S;
+            if (a.L != b.L) return a.L < b.L;
+            const auto ka = std::make_tuple(a.v.flags & 4, a.v.order, a.v.la, a.v.lb, a.v.lc, a.v.flags & 1);
+            const auto kb = std::make_tuple(b.v.flags & 4, b.v.order, b.v.la, b.v.lb, b.v.lc, b.v.flags & 1);
+            return ka < kb;
         });
         std::vector<long> gcost;
         for (size_t k = 0; k < pend.size();) {

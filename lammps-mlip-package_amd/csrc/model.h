@@ -21,6 +21,8 @@ constexpr int kMaxFn = 16;      // radial functions supported by the kernels
 constexpr int kMaxL = 12;       // max angular momentum of a_nlm
 constexpr int kMaxOrderGpu = 4; // highest invariant order of the HIP path
 constexpr int kWavesPerWg = 8;  // waves sharing one a_nlm tile in the adjoint kernel
+constexpr int kUnrollCap3 = 8;  // order-3 visits with all l <= 8 run fully unrolled (q34_signatures.inc)
+constexpr int kUnrollCap4 = 2;  // order-4 visits with all l <= 2 run fully unrolled
 
 struct Irrep {
     std::vector<int> l;                // nondecreasing l_1..l_q
@@ -64,7 +66,7 @@ struct VisitQ34 {
     int32_t lb, sb;
     int32_t lc, sc;    // order 4 only
     int32_t pslot;     // rank among polynomial itcs or -1
-    int32_t flags;     // bit0: compute d here; bit1: accumulate G
+    int32_t flags;     // bit0: compute d here; bit1: accumulate G; bit2: unrolled signature
     int32_t coef;      // dense coefficient block offset
     int32_t dcoef;     // dense pair-summed block offset (flags bit0)
 };
