@@ -8,6 +8,27 @@
 namespace mlip {
 namespace dev {
 
+// Read-only tables (coefficients, visit descriptors) are addressed through
+// the constant address space: wave-uniform reads of it compile to scalar
+// loads through the scalar cache instead of per-lane vector loads.
+#if defined(__HIP_DEVICE_COMPILE__)
+template <typename T>
+using cptr = const __attribute__((address_space(4))) T*;
+#else
+template <typename T>
+using cptr = const T*;  // host pass: kernels are not compiled for the host
+#endif
+
+// Pointer asserted wave-uniform (both 32-bit halves through readfirstlane),
+// so that loads through it stay scalar even across non-inlined calls.
+template <typename T>
+__device__ __forceinline__ cptr<T> as_const(const T* p) {
+    const unsigned long long v = reinterpret_cast<unsigned long long>(p);
+    const unsigned lo = __builtin_amdgcn_readfirstlane(static_cast<unsigned>(v));
+    const unsigned hi = __builtin_amdgcn_readfirstlane(static_cast<unsigned>(v >> 32));
+    return (cptr<T>)(reinterpret_cast<const T*>((static_cast<unsigned long long>(hi) << 32) | lo));
+}
+
 __device__ __forceinline__ int lmh(int l, int m) { return l * (l + 1) / 2 + l + m; }
 
 // XCD-aware mapping of a 1-D grid onto slots: consecutive workgroups of one

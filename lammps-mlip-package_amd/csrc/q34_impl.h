@@ -53,14 +53,14 @@ __device__ __forceinline__ void visit3(const Tile& T, const DevModel& M, const V
     double lr[L + 1], li[L + 1];
     if (DOD) load_out_row<L>(T, S, lr, li);
     const int la = uniform(v.la), lb = uniform(v.lb), sa = uniform(v.sa), sb = uniform(v.sb);
-    const double* cg = M.coef + uniform(v.coef);
-    const double* cd = M.coef + uniform(v.dcoef);
+    cptr<double> cg = as_const(M.coef) + uniform(v.coef);
+    cptr<double> cd = as_const(M.coef) + uniform(v.dcoef);
     for (int ia = 0; ia <= 2 * la; ++ia) {
         const int ma = ia - la;
         double xr, xi;
         T.get(sa, la, ma, xr, xi);
-        const double* cgk = cg + ia * (L + 1);
-        const double* cdk = cd + ia * (L + 1);
+        cptr<double> cgk = cg + ia * (L + 1);
+        cptr<double> cdk = cd + ia * (L + 1);
 #pragma unroll
         for (int k = 0; k <= L; ++k) {
             const int mb = k - ma;
@@ -86,8 +86,8 @@ __device__ __forceinline__ void visit4(const Tile& T, const DevModel& M, const V
     if (DOD) load_out_row<L>(T, S, lr, li);
     const int la = uniform(v.la), lb = uniform(v.lb), lc = uniform(v.lc);
     const int sa = uniform(v.sa), sb = uniform(v.sb), sc = uniform(v.sc);
-    const double* cg = M.coef + uniform(v.coef);
-    const double* cd = M.coef + uniform(v.dcoef);
+    cptr<double> cg = as_const(M.coef) + uniform(v.coef);
+    cptr<double> cd = as_const(M.coef) + uniform(v.dcoef);
     const int nb = 2 * lb + 1;
     for (int ia = 0; ia <= 2 * la; ++ia) {
         const int ma = ia - la;
@@ -99,8 +99,8 @@ __device__ __forceinline__ void visit4(const Tile& T, const DevModel& M, const V
             T.get(sb, lb, mb, yr, yi);
             const double qr = xr * yr - xi * yi;
             const double qi = xr * yi + xi * yr;
-            const double* cgk = cg + (ia * nb + ib) * (L + 1);
-            const double* cdk = cd + (ia * nb + ib) * (L + 1);
+            cptr<double> cgk = cg + (ia * nb + ib) * (L + 1);
+            cptr<double> cdk = cd + (ia * nb + ib) * (L + 1);
 #pragma unroll
             for (int k = 0; k <= L; ++k) {
                 const int mc = k - ma - mb;
@@ -142,8 +142,8 @@ __device__ __forceinline__ void load_row(const Tile& T, int s, double (&r)[2 * L
 }
 
 template <int L, int LA, int LB, bool DOG, bool DOD>
-__device__ __forceinline__ void visit3u(const Tile& T, int S, int sa, int sb, const double* __restrict__ cg,
-                                        const double* __restrict__ cd, double H, double (&gr)[L + 1],
+__device__ __forceinline__ void visit3u(const Tile& T, int S, int sa, int sb, cptr<double> cg,
+                                        cptr<double> cd, double H, double (&gr)[L + 1],
                                         double (&gi)[L + 1], double& d) {
     double br[2 * LB + 1], bi[2 * LB + 1];
     load_row<LB>(T, sb, br, bi);
@@ -179,8 +179,8 @@ __device__ __forceinline__ void visit3u(const Tile& T, int S, int sa, int sb, co
 }
 
 template <int L, int LA, int LB, int LC, bool DOG, bool DOD>
-__device__ __forceinline__ void visit4u(const Tile& T, int S, int sa, int sb, int sc, const double* __restrict__ cg,
-                                        const double* __restrict__ cd, double H, double (&gr)[L + 1],
+__device__ __forceinline__ void visit4u(const Tile& T, int S, int sa, int sb, int sc, cptr<double> cg,
+                                        cptr<double> cd, double H, double (&gr)[L + 1],
                                         double (&gi)[L + 1], double& d) {
     double br[2 * LB + 1], bi[2 * LB + 1], cr[2 * LC + 1], ci[2 * LC + 1];
     load_row<LB>(T, sb, br, bi);
@@ -237,7 +237,7 @@ struct Unrolled;
     struct Unrolled<LL, DOG, DOD> {                                                                           \
         static constexpr int LV = LL;                                                                         \
         __device__ static bool run(const Tile& T, const VisitQ34& v, int S, int order, int la, int lb, int lc, \
-                                   const double* cg, const double* cd, double H, double (&gr)[LL + 1],        \
+                                   cptr<double> cg, cptr<double> cd, double H, double (&gr)[LL + 1],          \
                                    double (&gi)[LL + 1], double& d) {                                         \
             if (order == 3) {                                                                                 \
                 switch (la * 16 + lb) { CASES3 default: return false; }                                       \
@@ -257,21 +257,21 @@ MLIP_UNROLLED(7, MLIP_SIG3_L7(MLIP_CASE3), MLIP_NO_CASES)
 MLIP_UNROLLED(8, MLIP_SIG3_L8(MLIP_CASE3), MLIP_NO_CASES)
 template <int L, bool DOG, bool DOD>
 struct Unrolled {
-    __device__ static bool run(const Tile&, const VisitQ34&, int, int, int, int, int, const double*, const double*,
+    __device__ static bool run(const Tile&, const VisitQ34&, int, int, int, int, int, cptr<double>, cptr<double>,
                                double, double (&)[L + 1], double (&)[L + 1], double&) {
         return false;
     }
 };
 
 template <int L, bool FEAT>
-__device__ void q34_group(const Tile& T, const DevModel& M, const DevSystem& S, const GroupQ34& g, int blk,
+__device__ __forceinline__ void q34_group(const Tile& T, const DevModel& M, const DevSystem& S, const GroupQ34& g, int blk,
                           int n, double& eacc) {
     double gr[L + 1], gi[L + 1];
     const int Sx = uniform(g.S);
 #pragma unroll
     for (int k = 0; k <= L; ++k) gr[k] = gi[k] = 0.0;
     const int slot = blk * kBlock + T.lane;
-    const VisitQ34* vis = FEAT ? M.vfeat : M.vadj;
+    cptr<VisitQ34> vis = as_const(FEAT ? M.vfeat : M.vadj);
     const int vb = uniform(g.vbeg), ve = uniform(g.vend);
     for (int vi = vb; vi < ve; ++vi) {
         const VisitQ34 v = vis[vi];
@@ -280,8 +280,8 @@ __device__ void q34_group(const Tile& T, const DevModel& M, const DevSystem& S, 
         double d = 0.0;
         const bool fast = (flags & 4) != 0;
         const int la = uniform(v.la), lb = uniform(v.lb), lc = uniform(v.lc);
-        const double* cg = M.coef + uniform(v.coef);
-        const double* cd = M.coef + uniform(v.dcoef);
+        cptr<double> cg = as_const(M.coef) + uniform(v.coef);
+        cptr<double> cd = as_const(M.coef) + uniform(v.dcoef);
         if (FEAT) {
             if (!(fast && Unrolled<L, false, true>::run(T, v, Sx, order, la, lb, lc, cg, cd, 0.0, gr, gi, d))) {
                 if (order == 3)
@@ -293,7 +293,7 @@ __device__ void q34_group(const Tile& T, const DevModel& M, const DevSystem& S, 
             continue;
         }
         const double H =
-            pslot >= 0 ? S.H[(size_t)slot * M.n_p + n * M.n_pitc + pslot] : M.weight[n * M.n_gtinv + itc];
+            pslot >= 0 ? S.H[(size_t)slot * M.n_p + n * M.n_pitc + pslot] : as_const(M.weight)[n * M.n_gtinv + itc];
         if (flags & 1) {
             if (!(fast && Unrolled<L, true, true>::run(T, v, Sx, order, la, lb, lc, cg, cd, H, gr, gi, d))) {
                 if (order == 3)
@@ -301,7 +301,7 @@ __device__ void q34_group(const Tile& T, const DevModel& M, const DevSystem& S, 
                 else
                     visit4<L, true, true>(T, M, v, Sx, H, gr, gi, d);
             }
-            eacc = fma(M.weight[n * M.n_gtinv + itc], d, eacc);
+            eacc = fma(as_const(M.weight)[n * M.n_gtinv + itc], d, eacc);
         } else {
             if (!(fast && Unrolled<L, true, false>::run(T, v, Sx, order, la, lb, lc, cg, cd, H, gr, gi, d))) {
                 if (order == 3)
@@ -336,7 +336,7 @@ __global__ __launch_bounds__(kQ34Threads) void k_q34(DevModel M, DevSystem S) {
     __syncthreads();
     const int wave = uniform(threadIdx.x >> 6);
     Tile T{As, M.n_lmh3, static_cast<int>(threadIdx.x & 63)};
-    const GroupQ34* groups = FEAT ? M.gfeat : M.gadj;
+    cptr<GroupQ34> groups = as_const(FEAT ? M.gfeat : M.gadj);
     const int gb = FEAT ? M.gfeat_beg[t] : M.gadj_beg[t];
     const int ge = FEAT ? M.gfeat_end[t] : M.gadj_end[t];
     double eacc = 0.0;
