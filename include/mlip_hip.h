@@ -90,6 +90,40 @@ int mlip_resident_compute(mlip_ctx* ctx);
 int mlip_resident_sync(mlip_ctx* ctx);
 int mlip_resident_get(mlip_ctx* ctx, double* f, double* energy, double* virial6, double* eatom);
 
+/* ---- Batched evaluation (BASELINE batched configuration; no reference
+ * counterpart).  nstruct independent periodic structures: natoms[k] atoms
+ * with cell cells[9k..9k+8] (rows a, b, c), positions x and 1-based types
+ * concatenated in structure order.  Each structure gets its own periodic
+ * neighbor list; one evaluation covers the whole batch and reduces energy
+ * and virial per structure in a fixed order.  mlip_resident_compute /
+ * mlip_resident_sync evaluate it; mlip_batch_get returns energies[nstruct],
+ * virials[6 nstruct] and forces[3 natoms_total]. */
+int mlip_batch_set(mlip_ctx* ctx, int nstruct, const int* natoms, const double* cells, const double* x,
+                   const int* type);
+int mlip_batch_get(mlip_ctx* ctx, double* energies, double* virials, double* f);
+
+/* Synthetic batched/served structures: sizes log-uniform in [nmin, nmax],
+ * random triclinic cells at `density` (atoms/A^3), uniform positions with an
+ * rmin rejection (periodic images included), two species with a random
+ * fraction.  Deterministic in seed.  Call with x = NULL to get natoms[] only,
+ * then with buffers of sum(natoms) atoms. */
+int mlip_random_structures(int nstruct, int nmin, int nmax, double density, double rmin, uint64_t seed,
+                           int* natoms, double* cells, double* x, int* type, int64_t cap_atoms);
+
+/* Capture one evaluation of the resident structure or batch as a HIP graph
+ * and replay it on later mlip_resident_compute calls (on = 1). */
+int mlip_graph_mode(mlip_ctx* ctx, int on);
+
+/* Times each launch shape of the force kernel (64/128/192/256 threads per
+ * workgroup) `reps` times on the current resident structure or batch and
+ * keeps the fastest; *chosen receives it. */
+int mlip_autotune(mlip_ctx* ctx, int reps, int* chosen);
+
+/* Sets the force kernel's launch shape directly (64, 128, 192 or 256
+ * threads per workgroup; 128 by default).  Results do not depend on it
+ * beyond the summation order of the per-atom virial. */
+int mlip_set_force_threads(mlip_ctx* ctx, int threads);
+
 /* Per-kernel device time, measured with HIP events on the context's own
  * stream around every launch while enabled.  mlip_profile_enable resets the
  * totals; mlip_profile_read returns, per kernel name, the summed

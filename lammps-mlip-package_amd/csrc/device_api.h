@@ -78,7 +78,10 @@ void launch_q12(const DevModel& m, const DevSystem& s, bool feat, hipStream_t st
 void launch_q34_adj(const DevModel& m, const DevSystem& s, hipStream_t st);
 void launch_q34_feat(const DevModel& m, const DevSystem& s, hipStream_t st);
 void launch_poly(const DevModel& m, const DevSystem& s, hipStream_t st);
-void launch_force(const DevModel& m, const DevSystem& s, int precision, hipStream_t st);
+// threads per workgroup of the force kernel: 64, 128 (default), 192 or 256
+void launch_force(const DevModel& m, const DevSystem& s, int threads, hipStream_t st);
+// per-structure energy + virial (7 doubles each) of a batch; soff = atom offsets
+void launch_struct_reduce(const DevSystem& s, const int32_t* soff, int nstruct, double* out, hipStream_t st);
 void launch_gather(const DevModel& m, const DevSystem& s, hipStream_t st);
 void launch_energy_virial(const DevModel& m, const DevSystem& s, hipStream_t st);
 // plain streaming copy used to calibrate hardware byte counters

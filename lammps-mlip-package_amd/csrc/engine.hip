@@ -16,6 +16,7 @@
 #include "device_api.h"
 #include "model.h"
 #include "neighbor.h"
+#include "workload.h"
 
 using namespace mlip;
 
@@ -84,6 +85,16 @@ struct mlip_ctx {
     // algorithmic work of the current structure (see mlip_flops)
     std::vector<int64_t> owners_per_type;
     int64_t npairs = 0;
+    // launch shape chosen by mlip_autotune
+    int force_threads = 128;
+    // batch of independent structures (0 = single structure)
+    int nstruct = 0;
+    DBuf<int32_t> soff;
+    DBuf<double> sout;
+    // one evaluation captured as a HIP graph and replayed
+    bool use_graph = false;
+    bool graph_warm = false;  // one eager evaluation ran since the last change
+    hipGraphExec_t gexec = nullptr;
 };
 
 namespace {
@@ -200,6 +211,8 @@ bool upload_model(mlip_ctx* c) {
     return hip_ok(c, hipStreamSynchronize(c->stream), "model upload");
 }
 
+void drop_graph(mlip_ctx* c);
+
 // Slot assignment: centres sorted by element (stable), each element padded
 // to whole blocks of kBlock lanes.
 void assign_slots(const mlip_ctx* c, int nown, const std::vector<int>& elem_of_owner,
@@ -233,6 +246,7 @@ bool setup_system(mlip_ctx* c, int nall, int nown, int64_t npairs, const std::ve
                   const std::vector<double>* pshift, const std::vector<int32_t>& goff,
                   const std::vector<int32_t>& gent) {
     const Model& m = c->model;
+    drop_graph(c);
     std::vector<int32_t> atom_of_slot, slot_of_atom, block_type;
     assign_slots(c, nown, elem_own, atom_of_slot, slot_of_atom, block_type);
     const int nblocks = static_cast<int>(block_type.size());
@@ -371,10 +385,50 @@ bool run_pipeline(mlip_ctx* c) {
     }
     { Scope sc(c, "q12_adj"); launch_q12(m, s, false, st); }
     if (mm.has_order34) { Scope sc(c, "q34_adj"); launch_q34_adj(m, s, st); }
-    { Scope sc(c, "force"); launch_force(m, s, c->precision, st); }
+    { Scope sc(c, "force"); launch_force(m, s, c->force_threads, st); }
     { Scope sc(c, "gather"); launch_gather(m, s, st); }
     { Scope sc(c, "energy_virial"); launch_energy_virial(m, s, st); }
+    if (c->nstruct > 0) {
+        Scope sc(c, "struct_reduce");
+        launch_struct_reduce(s, c->soff.p, c->nstruct, c->sout.p, st);
+    }
     return hip_ok(c, hipGetLastError(), "kernel launch");
+}
+
+void drop_graph(mlip_ctx* c) {
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+    c->gexec = nullptr;
+    c->graph_warm = false;
+}
+
+// One evaluation: replays the captured graph when graph mode is on.  After a
+// structure or setting change the first call runs eagerly (rocBLAS loads its
+// kernels and workspace outside any capture) and the second captures.
+// Profiling events are not recorded inside a capture.
+bool run_eval(mlip_ctx* c) {
+    if (!c->use_graph || c->prof) return run_pipeline(c);
+    if (!c->gexec && !c->graph_warm) {
+        c->graph_warm = true;
+        return run_pipeline(c);
+    }
+    if (!c->gexec) {
+        hipGraph_t g = nullptr;
+        if (!hip_ok(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal), "begin capture"))
+            return false;
+        const bool ok = run_pipeline(c);
+        const hipError_t e = hipStreamEndCapture(c->stream, &g);
+        if (!ok || !hip_ok(c, e, "end capture")) {
+            if (g) (void)hipGraphDestroy(g);
+            return false;
+        }
+        const hipError_t ie = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (!hip_ok(c, ie, "graph instantiate")) {
+            c->gexec = nullptr;
+            return false;
+        }
+    }
+    return hip_ok(c, hipGraphLaunch(c->gexec, c->stream), "graph launch");
 }
 
 void collect_profile(mlip_ctx* c) {
@@ -477,6 +531,7 @@ int lammps_compute(mlip_ctx* c, int nlocal, int nghost, const double* x, const i
     std::vector<int32_t> gof, gen;
     build_gather(nall, poff, pnbr, &pcen, gof, gen);
     c->resident = false;
+    c->nstruct = 0;
     if (!setup_system(c, nall, inum, npairs, xv, elem, elem_own, poff, pnbr, &pcen, nullptr, gof, gen))
         return MLIP_ERR_DEVICE;
     if (!run_pipeline(c)) return MLIP_ERR_DEVICE;
@@ -593,6 +648,9 @@ void mlip_destroy(mlip_ctx* c) {
     c->d_vfeat.release();
     c->d_gadj.release();
     c->d_gfeat.release();
+    drop_graph(c);
+    c->soff.release();
+    c->sout.release();
     if (c->blas) rocblas_destroy_handle(c->blas);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -609,6 +667,7 @@ int mlip_masses(const mlip_ctx* c, double* mass) {
 
 int mlip_set_precision(mlip_ctx* c, int mode) {
     if (!c || (mode != MLIP_FP64 && mode != MLIP_MIXED)) return MLIP_ERR_ARG;
+    if (c->precision != mode) drop_graph(c);
     c->precision = mode;
     return MLIP_OK;
 }
@@ -659,6 +718,7 @@ int mlip_resident_set(mlip_ctx* c, int natoms, const double* x, const int* type,
     std::vector<double> xv(x, x + 3 * static_cast<size_t>(natoms));
     std::vector<double> sh;
     if (nbr_shift) sh.assign(nbr_shift, nbr_shift + 3 * npairs);
+    c->nstruct = 0;
     if (!setup_system(c, natoms, natoms, npairs, xv, elem, elem, poff, pnbr, nullptr, nbr_shift ? &sh : nullptr, gof,
                       gen))
         return MLIP_ERR_DEVICE;
@@ -687,7 +747,7 @@ int mlip_resident_compute(mlip_ctx* c) {
         c->err = "no resident structure";
         return MLIP_ERR_STATE;
     }
-    return run_pipeline(c) ? MLIP_OK : MLIP_ERR_DEVICE;
+    return run_eval(c) ? MLIP_OK : MLIP_ERR_DEVICE;
 }
 
 int mlip_resident_sync(mlip_ctx* c) {
@@ -721,6 +781,149 @@ int mlip_resident_get(mlip_ctx* c, double* f, double* energy, double* virial, do
         (void)hipMemcpy(sl.data(), c->slot_of_atom.p, n * sizeof(int32_t), hipMemcpyDeviceToHost);
         for (int i = 0; i < n; ++i) eatom[i] = es[sl[i]];
     }
+    return MLIP_OK;
+}
+
+int mlip_batch_set(mlip_ctx* c, int nstruct, const int* natoms, const double* cells, const double* x,
+                   const int* type) {
+    if (!c || nstruct <= 0 || !natoms || !cells || !x || !type) return MLIP_ERR_ARG;
+    std::vector<int32_t> soff(nstruct + 1, 0);
+    for (int k = 0; k < nstruct; ++k) {
+        if (natoms[k] < 0) return MLIP_ERR_ARG;
+        soff[k + 1] = soff[k] + natoms[k];
+    }
+    const int n = soff[nstruct];
+    std::vector<int32_t> elem(n), poff(n + 1, 0), pnbr;
+    std::vector<double> sh;
+    for (int i = 0; i < n; ++i) {
+        const int t = type[i];
+        if (t < 1 || t > static_cast<int>(c->elem_of_type.size()) || c->elem_of_type[t - 1] < 0) {
+            c->err = "atom type without an element mapping";
+            return MLIP_ERR_ARG;
+        }
+        elem[i] = c->elem_of_type[t - 1];
+    }
+    for (int k = 0; k < nstruct; ++k) {
+        NeighborList nl;
+        std::string e;
+        if (!build_full_list(cells + 9 * k, natoms[k], x + 3 * static_cast<size_t>(soff[k]), c->model.cutoff, nl, e)) {
+            c->err = "structure " + std::to_string(k) + ": " + e;
+            return MLIP_ERR_ARG;
+        }
+        for (int i = 0; i < natoms[k]; ++i) poff[soff[k] + i + 1] = static_cast<int32_t>(pnbr.size() + nl.off[i + 1]);
+        for (int32_t j : nl.idx) pnbr.push_back(soff[k] + j);
+        sh.insert(sh.end(), nl.shift.begin(), nl.shift.end());
+        if (pnbr.size() > static_cast<size_t>(INT32_MAX)) {
+            c->err = "more than 2^31 pairs in the batch";
+            return MLIP_ERR_ARG;
+        }
+    }
+    std::vector<int32_t> gof, gen;
+    build_gather(n, poff, pnbr, nullptr, gof, gen);
+    std::vector<double> xv(x, x + 3 * static_cast<size_t>(n));
+    c->nstruct = 0;
+    if (!setup_system(c, n, n, static_cast<int64_t>(pnbr.size()), xv, elem, elem, poff, pnbr, nullptr, &sh, gof, gen))
+        return MLIP_ERR_DEVICE;
+    if (!upload(c, c->soff, soff.data(), soff.size(), "structure offsets") || !c->sout.ensure(7 * nstruct))
+        return MLIP_ERR_DEVICE;
+    if (!hip_ok(c, hipStreamSynchronize(c->stream), "batch upload")) return MLIP_ERR_DEVICE;
+    c->nstruct = nstruct;
+    c->resident = true;
+    c->resident_natoms = n;
+    return MLIP_OK;
+}
+
+int mlip_random_structures(int nstruct, int nmin, int nmax, double density, double rmin, uint64_t seed,
+                           int* natoms, double* cells, double* x, int* type, int64_t cap_atoms) {
+    if (nstruct < 0 || nmin < 1 || nmax < nmin || !(density > 0) || !natoms) return MLIP_ERR_ARG;
+    int64_t total = 0;
+    for (int k = 0; k < nstruct; ++k) {
+        natoms[k] = random_structure_size(seed, k, nmin, nmax);
+        total += natoms[k];
+    }
+    if (!x) return MLIP_OK;  // sizes only
+    if (!cells || !type || total > cap_atoms) return MLIP_ERR_ARG;
+    int64_t at = 0;
+    for (int k = 0; k < nstruct; ++k) {
+        if (!random_structure(seed, k, natoms[k], density, rmin, cells + 9 * k, x + 3 * at, type + at))
+            return MLIP_ERR_ARG;
+        at += natoms[k];
+    }
+    return MLIP_OK;
+}
+
+int mlip_batch_get(mlip_ctx* c, double* energies, double* virials, double* f) {
+    if (!c) return MLIP_ERR_ARG;
+    if (c->nstruct <= 0) {
+        c->err = "no batch";
+        return MLIP_ERR_STATE;
+    }
+    std::vector<double> o(7 * static_cast<size_t>(c->nstruct));
+    if (!hip_ok(c, hipMemcpyAsync(o.data(), c->sout.p, o.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream),
+                "batch results"))
+        return MLIP_ERR_DEVICE;
+    if (f && !hip_ok(c, hipMemcpyAsync(f, c->f.p, 3 * sizeof(double) * c->resident_natoms, hipMemcpyDeviceToHost,
+                                       c->stream),
+                     "batch forces"))
+        return MLIP_ERR_DEVICE;
+    if (!hip_ok(c, hipStreamSynchronize(c->stream), "batch readback")) return MLIP_ERR_DEVICE;
+    for (int k = 0; k < c->nstruct; ++k) {
+        if (energies) energies[k] = o[7 * k];
+        if (virials)
+            for (int q = 0; q < 6; ++q) virials[6 * k + q] = o[7 * k + 1 + q];
+    }
+    return MLIP_OK;
+}
+
+int mlip_graph_mode(mlip_ctx* c, int on) {
+    if (!c) return MLIP_ERR_ARG;
+    if ((on != 0) != c->use_graph) drop_graph(c);
+    c->use_graph = on != 0;
+    return MLIP_OK;
+}
+
+int mlip_autotune(mlip_ctx* c, int reps, int* chosen) {
+    if (!c || reps < 1) return MLIP_ERR_ARG;
+    if (!c->resident) {
+        c->err = "autotune needs a resident structure or batch";
+        return MLIP_ERR_STATE;
+    }
+    const int cand[4] = {64, 128, 192, 256};
+    const bool prof = c->prof;
+    double best = 1e300;
+    int best_nt = c->force_threads;
+    for (int nt : cand) {
+        c->force_threads = nt;
+        hipEvent_t a, b;
+        if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return MLIP_ERR_DEVICE;
+        (void)hipEventRecord(a, c->stream);
+        for (int r = 0; r < reps; ++r) launch_force(c->dm, c->ds, nt, c->stream);
+        (void)hipEventRecord(b, c->stream);
+        float ms = 0.0f;
+        const bool ok = hipEventSynchronize(b) == hipSuccess && hipEventElapsedTime(&ms, a, b) == hipSuccess;
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        if (!ok) return MLIP_ERR_DEVICE;
+        if (ms < best) {
+            best = ms;
+            best_nt = nt;
+        }
+    }
+    c->force_threads = best_nt;
+    c->prof = prof;
+    drop_graph(c);
+    if (chosen) *chosen = best_nt;
+    return MLIP_OK;
+}
+
+int mlip_set_force_threads(mlip_ctx* c, int threads) {
+    if (!c) return MLIP_ERR_ARG;
+    if (threads != 64 && threads != 128 && threads != 192 && threads != 256) {
+        c->err = "force threads must be 64, 128, 192 or 256";
+        return MLIP_ERR_ARG;
+    }
+    if (threads != c->force_threads) drop_graph(c);
+    c->force_threads = threads;
     return MLIP_OK;
 }
 

@@ -329,11 +329,11 @@ __global__ __launch_bounds__(256) void k_poly(DevModel M, DevSystem S) {
 // recursion as k_anlm, contracted with
 //   Gp_lm = sum_n g'_n G_nlm,  Gpp_lm = sum_n g_n G_nlm
 //   dE/d(del) = sum_lm w_m Re(Gp Y) rhat + w_m Re(Gpp . grad Y),  w_0 = 1, w_{m<0} = 2
-template <int LMAX>
-__global__ __launch_bounds__(128) void k_force(DevModel M, DevSystem S) {
+template <int LMAX, int NT>
+__global__ __launch_bounds__(NT) void k_force(DevModel M, DevSystem S) {
     constexpr int NLMH = (LMAX + 1) * (LMAX + 2) / 2;
     extern __shared__ double Gs[];
-    __shared__ double vred[6][128];
+    __shared__ double vred[6][NT];
     const int slot = xcd_slot(blockIdx.x, S.nblocks);
     if (slot < 0) return;
     const int atom = S.atom_of_slot[slot];
@@ -483,7 +483,7 @@ __global__ __launch_bounds__(128) void k_force(DevModel M, DevSystem S) {
     __syncthreads();
     if (threadIdx.x < 6) {
         double s = 0.0;
-        for (int k = 0; k < (int)blockDim.x; ++k) s += vred[threadIdx.x][k];
+        for (int k = 0; k < NT; ++k) s += vred[threadIdx.x][k];
         S.vown[6 * atom + threadIdx.x] = s;
     }
 }
@@ -520,6 +520,27 @@ __global__ __launch_bounds__(256) void k_energy_slot(DevModel M, DevSystem S) {
         }
     }
     S.e_slot[slot] = e;
+}
+
+// Per-structure energy and virial of a batch: one workgroup per structure,
+// fixed-order tree over its (contiguous) atoms.
+__global__ __launch_bounds__(256) void k_struct_reduce(DevSystem S, const int32_t* soff, double* out) {
+    __shared__ double sh[256];
+    const int st = blockIdx.x;
+    const int a0 = soff[st], a1 = soff[st + 1];
+    for (int c = 0; c < 7; ++c) {
+        double v = 0.0;
+        for (int i = a0 + threadIdx.x; i < a1; i += 256)
+            v += c == 0 ? S.e_slot[S.slot_of_atom[i]] : S.vown[6 * (size_t)i + (c - 1)];
+        sh[threadIdx.x] = v;
+        __syncthreads();
+        for (int w = 128; w > 0; w >>= 1) {
+            if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) out[7 * st + c] = sh[0];
+        __syncthreads();
+    }
 }
 
 // Fixed-order two-level reduction: k_reduce1 sums contiguous chunks per
@@ -636,21 +657,31 @@ void launch_poly(const DevModel& m, const DevSystem& s, hipStream_t st) {
     check(hipGetLastError(), "k_poly launch");
 }
 
-void launch_force(const DevModel& m, const DevSystem& s, int, hipStream_t st) {
-    if (s.nblocks == 0) return;
+template <int NT>
+static void launch_force_nt(const DevModel& m, const DevSystem& s, hipStream_t st) {
     const int grid = slot_grid(s);
     const size_t lds = sizeof(double) * 2 * m.n_type * m.n_lmh * m.n_fn;
     switch (m.maxl) {
 #define X(L)                                                                                        \
     case L:                                                                                         \
-        check(hipFuncSetAttribute(reinterpret_cast<const void*>(k_force<L>),                        \
+        check(hipFuncSetAttribute(reinterpret_cast<const void*>(k_force<L, NT>),                    \
                                   hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)), \
               "k_force lds attr");                                                                  \
-        hipLaunchKernelGGL(k_force<L>, dim3(grid), dim3(128), lds, st, m, s);                       \
+        hipLaunchKernelGGL((k_force<L, NT>), dim3(grid), dim3(NT), lds, st, m, s);                  \
         break;
         MLIP_LMAX_CASES(X)
 #undef X
         default: break;
+    }
+}
+
+void launch_force(const DevModel& m, const DevSystem& s, int threads, hipStream_t st) {
+    if (s.nblocks == 0) return;
+    switch (threads) {
+        case 64: launch_force_nt<64>(m, s, st); break;
+        case 192: launch_force_nt<192>(m, s, st); break;
+        case 256: launch_force_nt<256>(m, s, st); break;
+        default: launch_force_nt<128>(m, s, st); break;
     }
     check(hipGetLastError(), "k_force launch");
 }
@@ -667,6 +698,12 @@ void launch_copy(void* dst, const void* src, int64_t nbytes, int reps, hipStream
         hipLaunchKernelGGL(k_copy, dim3(2048), dim3(256), 0, st, static_cast<double4*>(dst),
                            static_cast<const double4*>(src), n);
     check(hipGetLastError(), "k_copy launch");
+}
+
+void launch_struct_reduce(const DevSystem& s, const int32_t* soff, int nstruct, double* out, hipStream_t st) {
+    if (nstruct <= 0) return;
+    hipLaunchKernelGGL(k_struct_reduce, dim3(nstruct), dim3(256), 0, st, s, soff, out);
+    check(hipGetLastError(), "k_struct_reduce launch");
 }
 
 void launch_energy_virial(const DevModel& m, const DevSystem& s, hipStream_t st) {

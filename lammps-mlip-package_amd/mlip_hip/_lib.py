@@ -59,6 +59,13 @@ def library():
     L.mlip_profile_enable.argtypes = [vp, c_int]
     L.mlip_profile_read.argtypes = [vp, ctypes.POINTER(c_char_p), dp, ip, c_int]
     L.mlip_flops.argtypes = [vp, dp]
+    L.mlip_random_structures.argtypes = [c_int, c_int, c_int, c_double, c_double, ctypes.c_uint64, ip, dp, dp, ip,
+                                         ctypes.c_int64]
+    L.mlip_set_force_threads.argtypes = [vp, c_int]
+    L.mlip_batch_set.argtypes = [vp, c_int, ip, dp, dp, ip]
+    L.mlip_batch_get.argtypes = [vp, dp, dp, dp]
+    L.mlip_graph_mode.argtypes = [vp, c_int]
+    L.mlip_autotune.argtypes = [vp, c_int, ip]
     L.mlip_calibration_copy.argtypes = [vp, ctypes.c_int64, c_int]
     L.mlip_debug_fetch.argtypes = [vp, c_int, dp, ctypes.c_int64]
     L.mlip_debug_fetch.restype = ctypes.c_int64

@@ -121,7 +121,75 @@ class PairMLIPGtinv:
         return buf[:got]
 
 
-class ResidentSystem:
+class _ResidentBase:
+    def compute(self):
+        self.pair._check(self.lib.mlip_resident_compute(self.pair.ctx))
+
+    def sync(self):
+        self.pair._check(self.lib.mlip_resident_sync(self.pair.ctx))
+
+    def graph(self, on=True):
+        self.pair._check(self.lib.mlip_graph_mode(self.pair.ctx, 1 if on else 0))
+
+    def autotune(self, reps=3):
+        v = ctypes.c_int(0)
+        self.pair._check(self.lib.mlip_autotune(self.pair.ctx, int(reps), ctypes.byref(v)))
+        return v.value
+
+    def set_force_threads(self, threads):
+        self.pair._check(self.lib.mlip_set_force_threads(self.pair.ctx, int(threads)))
+
+    def profile(self, on=True):
+        self.lib.mlip_profile_enable(self.pair.ctx, 1 if on else 0)
+
+    def read_profile(self):
+        """{kernel: (total_ms, launches)} since profile(True)."""
+        names = (ctypes.c_char_p * 64)()
+        ms = np.zeros(64)
+        cnt = np.zeros(64, dtype=np.int32)
+        n = self.lib.mlip_profile_read(self.pair.ctx, names, _p(ms, ctypes.c_double), _p(cnt, ctypes.c_int), 64)
+        return {names[k].decode(): (float(ms[k]), int(cnt[k])) for k in range(n)}
+
+    def flops(self):
+        out = np.zeros(7)
+        self.pair._check(self.lib.mlip_flops(self.pair.ctx, _p(out, ctypes.c_double)))
+        keys = ["anlm", "q12", "q34_feat", "q34_adj", "poly", "force", "total"]
+        return dict(zip(keys, out.tolist()))
+
+    def calibration_copy(self, nbytes, reps=1):
+        self.pair._check(self.lib.mlip_calibration_copy(self.pair.ctx, int(nbytes), int(reps)))
+
+
+class BatchSystem(_ResidentBase):
+    """Batch of independent periodic structures evaluated together on the
+    device (structures: list of workloads.Structure)."""
+
+    def __init__(self, pair, structures):
+        self.pair = pair
+        self.lib = pair.lib
+        self.nstruct = len(structures)
+        nat = np.array([s.natoms for s in structures], dtype=np.int32)
+        cells = np.ascontiguousarray(np.concatenate([s.cell.reshape(9) for s in structures]), dtype=np.float64)
+        x = np.ascontiguousarray(np.concatenate([s.positions for s in structures]), dtype=np.float64)
+        t = np.ascontiguousarray(np.concatenate([s.types for s in structures]), dtype=np.int32)
+        self.offsets = np.concatenate([[0], np.cumsum(nat)])
+        self.natoms = int(self.offsets[-1])
+        pair._check(self.lib.mlip_batch_set(pair.ctx, self.nstruct, _p(nat, ctypes.c_int), _p(cells, ctypes.c_double),
+                                            _p(x, ctypes.c_double), _p(t, ctypes.c_int)))
+
+    def get(self, forces=True):
+        e = np.zeros(self.nstruct)
+        v = np.zeros((self.nstruct, 6))
+        f = np.zeros((self.natoms, 3)) if forces else None
+        self.pair._check(self.lib.mlip_batch_get(self.pair.ctx, _p(e, ctypes.c_double), _p(v, ctypes.c_double),
+                                                 _p(f, ctypes.c_double) if forces else _null()))
+        out = {"energies": e, "virials": v}
+        if forces:
+            out["forces"] = [f[self.offsets[k]:self.offsets[k + 1]] for k in range(self.nstruct)]
+        return out
+
+
+class ResidentSystem(_ResidentBase):
     """Device-resident structure: positions, types and a full neighbor list
     of owned atoms with image shifts stay in HBM; compute() only enqueues."""
 
@@ -144,12 +212,6 @@ class ResidentSystem:
         x = np.ascontiguousarray(x, dtype=np.float64)
         self.pair._check(self.lib.mlip_resident_update_positions(self.pair.ctx, _p(x, ctypes.c_double)))
 
-    def compute(self):
-        self.pair._check(self.lib.mlip_resident_compute(self.pair.ctx))
-
-    def sync(self):
-        self.pair._check(self.lib.mlip_resident_sync(self.pair.ctx))
-
     def get(self, eatom=False):
         f = np.zeros((self.n, 3))
         e = np.zeros(1)
@@ -163,25 +225,6 @@ class ResidentSystem:
             out["eatom"] = ea
         return out
 
-    def profile(self, on=True):
-        self.lib.mlip_profile_enable(self.pair.ctx, 1 if on else 0)
-
-    def read_profile(self):
-        """{kernel: (total_ms, launches)} since profile(True)."""
-        names = (ctypes.c_char_p * 64)()
-        ms = np.zeros(64)
-        cnt = np.zeros(64, dtype=np.int32)
-        n = self.lib.mlip_profile_read(self.pair.ctx, names, _p(ms, ctypes.c_double), _p(cnt, ctypes.c_int), 64)
-        return {names[k].decode(): (float(ms[k]), int(cnt[k])) for k in range(n)}
-
-    def flops(self):
-        out = np.zeros(7)
-        self.pair._check(self.lib.mlip_flops(self.pair.ctx, _p(out, ctypes.c_double)))
-        keys = ["anlm", "q12", "q34_feat", "q34_adj", "poly", "force", "total"]
-        return dict(zip(keys, out.tolist()))
-
-    def calibration_copy(self, nbytes, reps=1):
-        self.pair._check(self.lib.mlip_calibration_copy(self.pair.ctx, int(nbytes), int(reps)))
 
 
 def inspect_model(potfile, which, cap=1 << 26):

@@ -392,3 +392,32 @@ def resident_lists(nl):
     idx = nl.owner[slots].astype(np.int32)
     shift = -nl.shift[slots]
     return off, idx, np.ascontiguousarray(shift)
+
+
+def random_structures(nstruct, nmin, nmax, seed, density=0.06, rmin=1.8):
+    """Batched/served workload structures from the native generator
+    (mlip_random_structures): log-uniform sizes, random triclinic cells at
+    `density`, rmin rejection, two species with a random fraction."""
+    import ctypes
+    from ._lib import library
+    lib = library()
+    nat = np.zeros(nstruct, dtype=np.int32)
+    P = lambda a, t: a.ctypes.data_as(ctypes.POINTER(t))  # noqa: E731
+    rc = lib.mlip_random_structures(nstruct, nmin, nmax, density, rmin, seed, P(nat, ctypes.c_int), None, None,
+                                    None, 0)
+    if rc != 0:
+        raise ValueError("bad workload parameters")
+    tot = int(nat.sum())
+    cells = np.zeros((nstruct, 9))
+    x = np.zeros((tot, 3))
+    t = np.zeros(tot, dtype=np.int32)
+    rc = lib.mlip_random_structures(nstruct, nmin, nmax, density, rmin, seed, P(nat, ctypes.c_int),
+                                    P(cells, ctypes.c_double), P(x, ctypes.c_double), P(t, ctypes.c_int), tot)
+    if rc != 0:
+        raise RuntimeError("rejection sampling did not finish")
+    out, at = [], 0
+    for k in range(nstruct):
+        n = int(nat[k])
+        out.append(Structure(cells[k].reshape(3, 3), x[at:at + n], t[at:at + n]))
+        at += n
+    return out

@@ -1,0 +1,177 @@
+"""Batched evaluation of independent structures, HIP-graph replay and the
+force kernel's launch shapes (FP64), through the C ABI.
+
+Bar: per-structure energy within 1e-10 eV/atom of the oracle, forces within
+1e-8 of the structure's largest force component, virial within 1e-8 of its
+largest component (forces that vanish by symmetry: 1e-11 eV/A absolute).
+A structure evaluated alone and inside a batch gives
+the same numbers to 1e-12 (relative); graph replays equal the eager result
+bit for bit."""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from common import config_potential, ref, rel_force_err
+from mlip_hip import BatchSystem, PairMLIPGtinv, ResidentSystem, periodic_full_list
+from mlip_hip import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+E_TOL = 1e-10
+F_TOL = 1e-8
+V_TOL = 1e-8
+
+
+def gpu_pair(pot, elems):
+    p = PairMLIPGtinv()
+    p.settings()
+    p.coeff("*", "*", pot, *elems)
+    return p
+
+
+def one_atom_structure():
+    # a single atom whose only neighbors are its own periodic images
+    return W.Structure(np.diag([3.1, 3.3, 3.6]), np.array([[0.2, 0.1, 0.3]]), np.array([2], dtype=np.int32))
+
+
+def check_batch(pot, elems, structs, tag):
+    o = O.Oracle(pot)
+    p = gpu_pair(pot, elems)
+    b = BatchSystem(p, structs)
+    b.compute()
+    b.sync()
+    r = b.get()
+    worst = [0.0, 0.0, 0.0]
+    for k, s in enumerate(structs):
+        if s.natoms == 0:
+            assert r["energies"][k] == 0.0 and np.all(r["virials"][k] == 0.0)
+            continue
+        ro = o.compute(W.build_neighbor_list(s, o.cutoff), s.types - 1)
+        de = abs(r["energies"][k] - ro["energy"]) / s.natoms
+        # a structure whose forces vanish by symmetry (one atom with only
+        # its own images) is held to 1e-11 eV/A absolute
+        fe = np.abs(r["forces"][k] - ro["forces"]).max() / max(np.abs(ro["forces"]).max(), 1e-3)
+        ve = np.abs(r["virials"][k] - ro["virial"]).max() / max(np.abs(ro["virial"]).max(), 1e-300)
+        worst = [max(worst[0], de), max(worst[1], fe), max(worst[2], ve)]
+        assert de <= E_TOL and fe <= F_TOL and ve <= V_TOL, (tag, k, s.natoms, de, fe, ve)
+    print("%s: %d structures, worst dE/atom=%.2e relF=%.2e relV=%.2e" % (tag, len(structs), *worst), flush=True)
+    return p, b, r
+
+
+def test_batch_against_oracle_regression_potential():
+    structs = W.random_structures(14, 8, 72, seed=7)
+    structs.insert(3, one_atom_structure())
+    structs.insert(7, W.Structure(np.eye(3) * 5.0, np.zeros((0, 3)), np.zeros(0, dtype=np.int32)))
+    check_batch(ref("gtinv-820.mlip"), ["Ti", "Al"], structs, "gtinv-820 batch")
+
+
+def test_batch_against_oracle_order4_quadratic():
+    structs = W.random_structures(6, 8, 48, seed=11)
+    check_batch(config_potential("binary_order4_quadratic"), ["Ti", "Al"], structs, "order4 quadratic batch")
+
+
+def test_alone_equals_in_batch():
+    pot = ref("gtinv-434.mlip")
+    structs = W.random_structures(10, 8, 96, seed=3)
+    p, b, r = check_batch(pot, ["Ti", "Al"], structs, "gtinv-434 batch")
+    for k in (0, 4, 9):
+        q = gpu_pair(pot, ["Ti", "Al"])
+        alone = BatchSystem(q, [structs[k]])
+        alone.compute()
+        alone.sync()
+        ra = alone.get()
+        e_rel = abs(ra["energies"][0] - r["energies"][k]) / max(abs(r["energies"][k]), 1e-300)
+        f_rel = rel_force_err(ra["forces"][0], r["forces"][k])
+        v_rel = np.abs(ra["virials"][0] - r["virials"][k]).max() / np.abs(r["virials"][k]).max()
+        print("structure %d alone vs in batch: %.1e %.1e %.1e" % (k, e_rel, f_rel, v_rel), flush=True)
+        assert e_rel <= 1e-12 and f_rel <= 1e-12 and v_rel <= 1e-12
+
+
+def _resident(pot, s):
+    p = gpu_pair(pot, ["Ti", "Al"])
+    off, idx, sh = periodic_full_list(s.cell, s.positions, p.init_one())
+    return p, ResidentSystem(p, s.positions, s.types, off, idx, sh)
+
+
+def _eval(rs):
+    rs.compute()
+    rs.sync()
+    return rs.get()
+
+
+def _same(a, b):
+    return a["energy"] == b["energy"] and np.array_equal(a["forces"], b["forces"]) and np.array_equal(
+        a["virial"], b["virial"])
+
+
+def test_graph_replays_equal_eager():
+    pot = config_potential("binary_order4_quadratic")
+    s = W.fcc(3, 4.0, 0.08, 4, type_fraction=0.5, type_seed=4)
+    rng = np.random.default_rng(5)
+    x1 = s.positions + rng.normal(0.0, 0.02, s.positions.shape)
+    p, rs = _resident(pot, s)
+    eager0 = _eval(rs)
+    rs.update_positions(x1)
+    eager1 = _eval(rs)
+    assert not _same(eager0, eager1)
+    # other eager device work between replays: a second context's boundary call
+    q = gpu_pair(ref("gtinv-820.mlip"), ["Ti", "Al"])
+    s2 = W.read_lammps_data(ref("structure.lammps"))
+    nl2 = W.build_neighbor_list(s2, q.init_one(), full=True)
+    r2 = q.compute(nl2, nl2.types)
+    rs.graph(True)
+    for k in range(6):
+        x, want = (s.positions, eager0) if k % 2 == 0 else (x1, eager1)
+        rs.update_positions(x)
+        got = _eval(rs)
+        assert _same(got, want), "replay %d" % k
+        r2k = q.compute(nl2, nl2.types)
+        assert r2k["energy"] == r2["energy"]
+    rs.graph(False)
+    rs.update_positions(s.positions)
+    assert _same(_eval(rs), eager0)
+
+
+def test_graph_replay_batch():
+    pot = ref("gtinv-820.mlip")
+    structs = W.random_structures(8, 8, 64, seed=21)
+    p = gpu_pair(pot, ["Ti", "Al"])
+    b = BatchSystem(p, structs)
+    b.compute()
+    b.sync()
+    want = b.get()
+    b.graph(True)
+    for k in range(4):
+        b.compute()
+        b.sync()
+        got = b.get()
+        assert np.array_equal(got["energies"], want["energies"]), k
+        assert np.array_equal(got["virials"], want["virials"]), k
+        assert all(np.array_equal(a, c) for a, c in zip(got["forces"], want["forces"])), k
+
+
+@pytest.mark.parametrize("threads", [64, 128, 192, 256])
+def test_force_launch_shapes(threads):
+    pot = ref("gtinv-820.mlip")
+    s = W.fcc(3, 4.0, 0.08, 4, type_fraction=0.5, type_seed=4)
+    p, rs = _resident(pot, s)
+    base = _eval(rs)
+    rs.set_force_threads(threads)
+    r = _eval(rs)
+    assert r["energy"] == base["energy"]
+    assert np.array_equal(r["forces"], base["forces"])
+    assert np.abs(r["virial"] - base["virial"]).max() <= 1e-13 * np.abs(base["virial"]).max()
+
+
+def test_autotune_keeps_results():
+    pot = config_potential("binary_order4_quadratic")
+    s = W.fcc(3, 4.0, 0.08, 4, type_fraction=0.5, type_seed=4)
+    p, rs = _resident(pot, s)
+    base = _eval(rs)
+    nt = rs.autotune(reps=2)
+    print("autotune chose %d threads" % nt, flush=True)
+    assert nt in (64, 128, 192, 256)
+    r = _eval(rs)
+    assert r["energy"] == base["energy"]
+    assert np.array_equal(r["forces"], base["forces"])
+    assert np.abs(r["virial"] - base["virial"]).max() <= 1e-13 * np.abs(base["virial"]).max()
