@@ -70,6 +70,28 @@ def cpu_baseline(pot, budget_s):
                       "%.1f s of CPU work; per-atom cost is size-independent" % (n_eval, t_tot)}
 
 
+def timed_steps(rs, warmup, steps, tag):
+    """Warm up, pick the force kernel's launch shape with the autotuner, then
+    time `steps` evaluations (HIP events per kernel on the engine's stream)."""
+    for _ in range(max(warmup, 1)):
+        rs.compute()
+    rs.sync()
+    nt = rs.autotune(reps=2)
+    rs.compute()
+    rs.sync()
+    log("%s: warmup done, force kernel %d threads per workgroup" % (tag, nt))
+    rs.profile(True)
+    rs.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        rs.compute()
+    rs.sync()
+    dt = time.perf_counter() - t0
+    prof = rs.read_profile()
+    rs.profile(False)
+    return dt, prof, rs.get(), nt
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -79,6 +101,7 @@ def main():
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU-oracle work")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision timing")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="also launch a plain copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -100,23 +123,25 @@ def main():
         % (s.natoms, idx.size, idx.size / s.natoms, time.time() - t0))
     rs = ResidentSystem(pair, s.positions, s.types, off, idx, sh)
     del off, idx, sh
-    for _ in range(args.warmup):
-        rs.compute()
-    rs.sync()
-    log("warmup done")
-    rs.profile(True)
-    rs.sync()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        rs.compute()
-    rs.sync()
-    dt = time.perf_counter() - t0
-    prof = rs.read_profile()
-    rs.profile(False)
-    res = rs.get()
+    dt, prof, res, nt = timed_steps(rs, args.warmup, args.steps, args.precision)
     if not np.isfinite(res["energy"]) or not np.all(np.isfinite(res["forces"])):
         raise SystemExit("non-finite result")
     flops = rs.flops()
+    mixed = None
+    if args.precision == "fp64" and not args.no_mixed:
+        pair.set_precision("mixed")
+        dtm, profm, resm, ntm = timed_steps(rs, args.warmup, args.steps, "mixed")
+        pair.set_precision("fp64")
+        df = np.abs(resm["forces"] - res["forces"])
+        mixed = {"value": s.natoms * args.steps / dtm, "ms_per_step": 1e3 * dtm / args.steps,
+                 "speedup_vs_fp64": dt / dtm, "force_threads": ntm,
+                 "energy_err_per_atom": abs(resm["energy"] - res["energy"]) / s.natoms,
+                 "force_err_max": float(df.max()), "force_mae": float(df.mean()),
+                 "reference": "FP64 GPU result of the same step (itself within 1e-10 eV/atom of the CPU oracle)",
+                 "kernels_ms": {k: round(v[0] / max(v[1], 1), 4) for k, v in profm.items()}}
+        log("mixed: %.1f ms/step (%.2fx FP64), dE/atom %.2e, F max %.2e MAE %.2e eV/A"
+            % (mixed["ms_per_step"], mixed["speedup_vs_fp64"], mixed["energy_err_per_atom"],
+               mixed["force_err_max"], mixed["force_mae"]))
     if args.calibrate_copy_mib > 0:
         rs.calibration_copy(args.calibrate_copy_mib << 20, reps=3)
 
@@ -159,9 +184,11 @@ def main():
                                "[12,8,2], model_type 4 max_p 2 (239,184 coeffs), r_c 8 A, 12 Gaussians; "
                                "one E+F+virial evaluation per step, inputs resident in HBM"
                                % (args.ncell, args.ncell, args.ncell, natoms),
-                   "natoms": natoms, "pairs_per_atom": rs.npairs / natoms, "precision": args.precision},
+                   "natoms": natoms, "pairs_per_atom": rs.npairs / natoms, "precision": args.precision,
+                   "force_threads": nt},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "mixed": mixed,
         "kernels_ms": {k: round(v["avg_ms"], 4) for k, v in kernels.items()},
         "flops_per_step": flops,
         "step_tflops": total_rate,

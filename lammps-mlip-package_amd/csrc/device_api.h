@@ -12,9 +12,18 @@
 namespace mlip {
 
 constexpr int kBlock = 64;        // atoms per slot block (one wavefront of lanes)
-constexpr int kAnlmChunk = 32;    // neighbors staged per LDS round in the a_nlm kernel
+
 
 enum Precision : int { kFP64 = 0, kMixed = 1 };
+
+// Constants of the normalised Legendre recursion (see pair_kernels.hip).
+template <typename T>
+struct YTab {
+    T diag[kMaxL + 2];              // Qb_m^m
+    T alm[kMaxL + 1][kMaxL + 1];    // recursion factor a_{l,m}; alm[m+1][m] = sqrt(2m+3)
+    T ainv[kMaxL + 1][kMaxL + 1];   // 1 / a_{l-1,m} (0 for l < m + 2)
+    T grad[kMaxL + 1][kMaxL + 1];   // sqrt((l+m+1)(l-m))
+};
 
 struct DevModel {
     int n_type, n_fn, n_gtinv, n_lmh, n_lmh3, maxl, lmax3, n_pitc, n_p;
@@ -35,6 +44,8 @@ struct DevModel {
     const int32_t* c3_off;  // cubic terms transposed by target: offsets (n_p+1)
     const int32_t* c3_bc;   // 2 per entry
     const double* c3_w;
+    const YTab<double>* yt64;
+    const YTab<float>* yt32;
 };
 
 struct DevSystem {
@@ -70,8 +81,8 @@ struct DevSystem {
     double* out;     // [7]: energy, virial[6]
 };
 
-// Upload the constant recursion tables (once per process and device).
-void init_device_constants();
+// Host-side construction of the recursion tables (uploaded per context).
+void build_ylm_tables(YTab<double>& t64, YTab<float>& t32);
 
 void launch_anlm(const DevModel& m, const DevSystem& s, int precision, hipStream_t st);
 void launch_q12(const DevModel& m, const DevSystem& s, bool feat, hipStream_t st);
@@ -79,7 +90,7 @@ void launch_q34_adj(const DevModel& m, const DevSystem& s, hipStream_t st);
 void launch_q34_feat(const DevModel& m, const DevSystem& s, hipStream_t st);
 void launch_poly(const DevModel& m, const DevSystem& s, hipStream_t st);
 // threads per workgroup of the force kernel: 64, 128 (default), 192 or 256
-void launch_force(const DevModel& m, const DevSystem& s, int threads, hipStream_t st);
+void launch_force(const DevModel& m, const DevSystem& s, int precision, int threads, hipStream_t st);
 // per-structure energy + virial (7 doubles each) of a batch; soff = atom offsets
 void launch_struct_reduce(const DevSystem& s, const int32_t* soff, int nstruct, double* out, hipStream_t st);
 void launch_gather(const DevModel& m, const DevSystem& s, hipStream_t st);

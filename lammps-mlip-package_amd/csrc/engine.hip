@@ -67,6 +67,8 @@ struct mlip_ctx {
     DBuf<int32_t> d_q12off, d_c3off, d_c3bc;
     DBuf<VisitQ34> d_vadj, d_vfeat;
     DBuf<GroupQ34> d_gadj, d_gfeat;
+    DBuf<YTab<double>> d_yt64;
+    DBuf<YTab<float>> d_yt32;
     DevModel dm{};
     // structure buffers
     DBuf<double> x, pshift, A, G, D, Hq, H, Ep, Eq12, Eq34, gpair, vown, f, e_slot, red, out;
@@ -196,6 +198,16 @@ bool upload_model(mlip_ctx* c) {
               upload(c, c->d_gadj, gadj.data(), gadj.size(), "adjoint groups") &&
               upload(c, c->d_gfeat, gfeat.data(), gfeat.size(), "feature groups");
     if (!ok) return false;
+    {
+        YTab<double> t64;
+        YTab<float> t32;
+        build_ylm_tables(t64, t32);
+        if (!upload(c, c->d_yt64, &t64, 1, "Y_lm recursion table") ||
+            !upload(c, c->d_yt32, &t32, 1, "Y_lm recursion table (FP32)"))
+            return false;
+    }
+    dm.yt64 = c->d_yt64.p;
+    dm.yt32 = c->d_yt32.p;
     dm.coef = c->d_coef.p;
     dm.weight = c->d_weight.p;
     dm.q12 = c->d_q12.p;
@@ -385,7 +397,7 @@ bool run_pipeline(mlip_ctx* c) {
     }
     { Scope sc(c, "q12_adj"); launch_q12(m, s, false, st); }
     if (mm.has_order34) { Scope sc(c, "q34_adj"); launch_q34_adj(m, s, st); }
-    { Scope sc(c, "force"); launch_force(m, s, c->force_threads, st); }
+    { Scope sc(c, "force"); launch_force(m, s, c->precision, c->force_threads, st); }
     { Scope sc(c, "gather"); launch_gather(m, s, st); }
     { Scope sc(c, "energy_virial"); launch_energy_virial(m, s, st); }
     if (c->nstruct > 0) {
@@ -608,11 +620,6 @@ int mlip_create(const char* potfile, int ntypes, const char* const* elem_of_type
         return MLIP_ERR_DEVICE;
     }
     (void)hipGetDevice(&c->device);
-    static bool constants_done = false;
-    if (!constants_done) {
-        init_device_constants();
-        constants_done = true;
-    }
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
         rocblas_create_handle(&c->blas) != rocblas_status_success ||
         rocblas_set_stream(c->blas, c->stream) != rocblas_status_success) {
@@ -648,6 +655,8 @@ void mlip_destroy(mlip_ctx* c) {
     c->d_vfeat.release();
     c->d_gadj.release();
     c->d_gfeat.release();
+    c->d_yt64.release();
+    c->d_yt32.release();
     drop_graph(c);
     c->soff.release();
     c->sout.release();
@@ -897,7 +906,7 @@ int mlip_autotune(mlip_ctx* c, int reps, int* chosen) {
         hipEvent_t a, b;
         if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return MLIP_ERR_DEVICE;
         (void)hipEventRecord(a, c->stream);
-        for (int r = 0; r < reps; ++r) launch_force(c->dm, c->ds, nt, c->stream);
+        for (int r = 0; r < reps; ++r) launch_force(c->dm, c->ds, c->precision, nt, c->stream);
         (void)hipEventRecord(b, c->stream);
         float ms = 0.0f;
         const bool ok = hipEventSynchronize(b) == hipSuccess && hipEventElapsedTime(&ms, a, b) == hipSuccess;

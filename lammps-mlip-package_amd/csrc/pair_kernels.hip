@@ -1,0 +1,561 @@
+// Per-pair kernels of the gtinv path: a_nlm accumulation and pair forces,
+// each in FP64 and in the mixed mode (FP32 per-pair geometry, radial
+// functions, Y_lm and their gradients; FP64 accumulation of a_nlm, FP64
+// pair forces out of the kernel, FP64 virial).
+//
+// Reference: PairMLIPGtinv::compute_anlm (src/USER-MLIP/pair_mlip_gtinv.cpp:349-447)
+// and the force loop compute_energy_and_force_for_each_atom (:161-256);
+// formulas in docs/formulation.md:126-154,221-280,351-411.
+//
+// Y_l^m(del) = Qb_l^m u^m with u = (x + i y)/r and the normalised Legendre
+// recursion without the sin^m factor:
+//   Qb_m^m = diag[m],  Qb_l^m = alm[l][m] (z Qb_{l-1}^m - ainv[l][m] Qb_{l-2}^m)
+// where alm[m+1][m] = sqrt(2m+3) and Qb_{m-1}^m = 0 make the first step
+// the same formula.  Only m <= 0 entries are formed: Y_{l,-m} = (-1)^m Qb conj(u)^m.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+
+#include "device_api.h"
+#include "device_common.h"
+
+namespace mlip {
+
+namespace {
+
+using namespace dev;
+
+constexpr double kPiRef = 3.141592653589793;  // mlip_basis_function.cpp:26
+constexpr int kL1 = kMaxL + 1;
+
+template <typename T>
+struct Vec2;
+template <>
+struct Vec2<double> {
+    using type = double2;
+};
+template <>
+struct Vec2<float> {
+    using type = float2;
+};
+
+template <typename T>
+__device__ __forceinline__ const YTab<T>* ytab_of(const DevModel& M);
+template <>
+__device__ __forceinline__ const YTab<double>* ytab_of<double>(const DevModel& M) {
+    return M.yt64;
+}
+template <>
+__device__ __forceinline__ const YTab<float>* ytab_of<float>(const DevModel& M) {
+    return M.yt32;
+}
+
+__device__ __forceinline__ void sin_cos(double a, double* s, double* c) { sincos(a, s, c); }
+
+struct PairDel {
+    double dx, dy, dz, r;
+    bool valid;
+};
+
+// del = x_centre - x_neighbor (+ image shift), strict r < cutoff.  Always
+// FP64: the difference of absolute positions and the cutoff test must not
+// depend on the precision mode.
+__device__ __forceinline__ PairDel pair_del(const DevModel& M, const DevSystem& S, int owner, int64_t p) {
+    PairDel g;
+    const int nb = S.pnbr[p];
+    const int c = S.pcen ? S.pcen[p] : owner;
+    g.dx = S.x[3 * c] - S.x[3 * nb];
+    g.dy = S.x[3 * c + 1] - S.x[3 * nb + 1];
+    g.dz = S.x[3 * c + 2] - S.x[3 * nb + 2];
+    if (S.pshift) {
+        g.dx += S.pshift[3 * p];
+        g.dy += S.pshift[3 * p + 1];
+        g.dz += S.pshift[3 * p + 2];
+    }
+    g.r = sqrt(g.dx * g.dx + g.dy * g.dy + g.dz * g.dz);
+    g.valid = (g.r < M.cutoff) && (g.r > 0.0);
+    return g;
+}
+
+// ---------------------------------------------------------------- a_nlm
+// One workgroup of kAnlmThreads per centre.  Neighbors are taken in blocks
+// of kAnlmThreads (geometry once per neighbor into LDS), then in chunks of
+// kAnlmJ: phase 1 has kAnlmParts threads per neighbor evaluate g_n
+// (n = part, part + kAnlmParts) and the Y_lm columns m = part, part +
+// kAnlmParts into LDS; phase 2 has thread (lm, group) accumulate
+// a_{s,n,lm} for its NPT radial indices n = group*NPT + k and both species
+// in FP64 registers.
+constexpr int kAnlmThreads = 192;
+constexpr int kAnlmJ = 16;
+constexpr int kAnlmParts = kAnlmThreads / kAnlmJ;  // 12
+
+template <typename T>
+struct AnlmLds {
+    T diag[kL1 + 1];
+    T alm[kL1][kL1];
+    T ainv[kL1][kL1];
+    double beta[kMaxFn], mu[kMaxFn];
+    T ux[kAnlmThreads], uy[kAnlmThreads], uz[kAnlmThreads];
+    double rr[kAnlmThreads], fc[kAnlmThreads];
+    int spec[kAnlmThreads];
+    double g[kAnlmJ][kMaxFn];
+};
+
+template <int NPT, typename T>
+__global__ __launch_bounds__(kAnlmThreads) void k_anlm(DevModel M, DevSystem S, int ngroup) {
+    using T2 = typename Vec2<T>::type;
+    __shared__ AnlmLds<T> L;
+    extern __shared__ unsigned char anlm_dyn[];
+    T2* ys = reinterpret_cast<T2*>(anlm_dyn);  // [kAnlmJ][n_lmh]
+    const int slot = xcd_slot(blockIdx.x, S.nblocks);
+    if (slot < 0) return;
+    const int atom = S.atom_of_slot[slot];
+    if (atom < 0) return;
+    const int tid = threadIdx.x;
+    const int nfn = M.n_fn, nlmh = M.n_lmh, maxl = M.maxl;
+    {
+        const YTab<T>* yt = ytab_of<T>(M);
+        for (int k = tid; k < kL1 * kL1; k += kAnlmThreads) {
+            (&L.alm[0][0])[k] = (&yt->alm[0][0])[k];
+            (&L.ainv[0][0])[k] = (&yt->ainv[0][0])[k];
+        }
+        if (tid <= kL1) L.diag[tid] = yt->diag[tid];
+        if (tid < kMaxFn) {
+            L.beta[tid] = tid < nfn ? M.beta[tid] : 0.0;
+            L.mu[tid] = tid < nfn ? M.mu[tid] : 0.0;
+        }
+    }
+    const int p0 = S.poff[atom], p1 = S.poff[atom + 1];
+    // phase-2 role
+    const int lm2 = tid % nlmh, grp = tid / nlmh;
+    const bool act2 = grp < ngroup;
+    const int nbase = grp * NPT;
+    double a0r[NPT], a0i[NPT], a1r[NPT], a1i[NPT];
+#pragma unroll
+    for (int k = 0; k < NPT; ++k) a0r[k] = a0i[k] = a1r[k] = a1i[k] = 0.0;
+    // phase-1 role
+    const int j1 = tid / kAnlmParts, part = tid % kAnlmParts;
+    const double pir = kPiRef / M.cutoff;
+    __syncthreads();
+    for (int b0 = p0; b0 < p1; b0 += kAnlmThreads) {
+        const int nb = min(kAnlmThreads, p1 - b0);
+        if (tid < nb) {
+            const PairDel g = pair_del(M, S, atom, b0 + tid);
+            int spec = -1;
+            if (g.valid) {
+                spec = S.elem[S.pnbr[b0 + tid]];
+                const double ir = 1.0 / g.r;
+                L.ux[tid] = static_cast<T>(g.dx * ir);
+                L.uy[tid] = static_cast<T>(g.dy * ir);
+                L.uz[tid] = static_cast<T>(g.dz * ir);
+                L.rr[tid] = g.r;
+                L.fc[tid] = 0.5 * (cos(pir * g.r) + 1.0);
+            }
+            L.spec[tid] = spec;
+        }
+        __syncthreads();
+        for (int c0 = 0; c0 < nb; c0 += kAnlmJ) {
+            const int q = c0 + j1;
+            if (q < nb && L.spec[q] >= 0) {
+                // radial functions in FP64 in both modes: FP32 cannot carry
+                // the exponent beta (r - mu)^2 to the digits exp needs
+                const double r = L.rr[q], fcq = L.fc[q];
+                for (int n = part; n < kMaxFn; n += kAnlmParts) {
+                    const double d = r - L.mu[n];
+                    L.g[j1][n] = n < nfn ? exp(-L.beta[n] * (d * d)) * fcq : 0.0;
+                }
+                const T x = L.ux[q], y = L.uy[q], z = L.uz[q];
+                T2* yrow = ys + j1 * nlmh;
+                for (int m = part; m <= maxl; m += kAnlmParts) {
+                    // (-1)^m conj(u)^m by binary powering
+                    T pr = (m & 1) ? T(-1) : T(1), pi = T(0);
+                    T br = x, bi = -y;
+                    for (int e = m; e; e >>= 1) {
+                        if (e & 1) {
+                            const T t = pr * br - pi * bi;
+                            pi = pr * bi + pi * br;
+                            pr = t;
+                        }
+                        const T t2 = br * br - bi * bi;
+                        bi = T(2) * br * bi;
+                        br = t2;
+                    }
+                    T q2 = T(0), q1 = L.diag[m];
+                    int idx = m * (m + 1) / 2;  // lmh(m, -m)
+                    yrow[idx] = T2{q1 * pr, q1 * pi};
+                    for (int l = m + 1; l <= maxl; ++l) {
+                        const T qn = L.alm[l][m] * (z * q1 - L.ainv[l][m] * q2);
+                        idx += l + 1;  // lmh(l, -m) = lmh(l-1, -m) + l + 1
+                        yrow[idx] = T2{qn * pr, qn * pi};
+                        q2 = q1;
+                        q1 = qn;
+                    }
+                }
+            }
+            __syncthreads();
+            const int nc = min(kAnlmJ, nb - c0);
+            if (act2) {
+                for (int j = 0; j < nc; ++j) {
+                    const int spec = __builtin_amdgcn_readfirstlane(L.spec[c0 + j]);
+                    if (spec < 0) continue;
+                    const T2 yv = ys[j * nlmh + lm2];
+                    const double yr = static_cast<double>(yv.x), yi = static_cast<double>(yv.y);
+                    const double* gj = &L.g[j][nbase];
+                    if (spec == 0) {
+#pragma unroll
+                        for (int k = 0; k < NPT; ++k) {
+                            a0r[k] = fma(gj[k], yr, a0r[k]);
+                            a0i[k] = fma(gj[k], yi, a0i[k]);
+                        }
+                    } else {
+#pragma unroll
+                        for (int k = 0; k < NPT; ++k) {
+                            a1r[k] = fma(gj[k], yr, a1r[k]);
+                            a1i[k] = fma(gj[k], yi, a1i[k]);
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    if (act2) {
+        const int blk = slot / kBlock, lane = slot % kBlock;
+#pragma unroll
+        for (int k = 0; k < NPT; ++k) {
+            const int n = nbase + k;
+            if (n < nfn) {
+                S.A[a_index(M, blk, n, 0, lm2, 0, lane)] = a0r[k];
+                S.A[a_index(M, blk, n, 0, lm2, 1, lane)] = a0i[k];
+                if (M.n_type > 1) {
+                    S.A[a_index(M, blk, n, 1, lm2, 0, lane)] = a1r[k];
+                    S.A[a_index(M, blk, n, 1, lm2, 1, lane)] = a1i[k];
+                }
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------- forces
+// One workgroup per centre; G of the centre (both species) staged in LDS as
+// T [s][lm][n][re,im] with n padded to NF (zeros).  Thread = neighbor pair;
+// Y_lm and grad Y_lm by the recursion above (coefficients from LDS, read
+// as broadcasts), contracted with
+//   Gp_lm = sum_n g'_n G_nlm,  Gq_lm = sum_n g_n G_nlm
+//   dE/d(del) = sum_lm w_m Re(Gp Y) rhat + w_m Re(Gq . grad Y),  w_0 = 1, w_{m<0} = 2
+// grad Y_l^m * r = {(-B u^{m+1} + E u^{m-1})/2, (i/2)(B u^{m+1} + E u^{m-1}), -A u^m},
+//   A = m z Q + (1-z^2) c Q', B = m Q - z c Q', E = m (1+z^2) Q + z (1-z^2) c Q',
+//   Q = Qb_l^m, Q' = Qb_l^{m+1}, c = sqrt((l+m+1)(l-m)); for m = 0 the
+//   singular terms cancel to c Q' (z x, z y, -(1-z^2)).
+template <typename T>
+struct ForceLds {
+    double beta[kMaxFn], mu[kMaxFn];
+    T diag[kL1 + 1];
+    T alm[kL1][kL1];
+    T ainv[kL1][kL1];
+    T grad[kL1][kL1];
+};
+
+template <int NF, int NT, typename T>
+__global__ __launch_bounds__(NT) void k_force(DevModel M, DevSystem S) {
+    extern __shared__ unsigned char force_dyn[];
+    T* Gs = reinterpret_cast<T*>(force_dyn);  // [ns][n_lmh][NF][2]
+    __shared__ ForceLds<T> C;
+    __shared__ double vred[6][NT / 64];
+    const int slot = xcd_slot(blockIdx.x, S.nblocks);
+    if (slot < 0) return;
+    const int atom = S.atom_of_slot[slot];
+    if (atom < 0) return;
+    const int nfn = M.n_fn, ns = M.n_type, nlmh = M.n_lmh, maxl = M.maxl;
+    const int blk = slot / kBlock, lane = slot % kBlock;
+    for (int k = threadIdx.x; k < ns * nlmh * NF; k += NT) {
+        const int n = k % NF;
+        const int lm = (k / NF) % nlmh;
+        const int s = k / (NF * nlmh);
+        T gr = T(0), gi = T(0);
+        if (n < nfn) {
+            gr = static_cast<T>(S.G[a_index(M, blk, n, s, lm, 0, lane)]);
+            gi = static_cast<T>(S.G[a_index(M, blk, n, s, lm, 1, lane)]);
+        }
+        Gs[2 * k] = gr;
+        Gs[2 * k + 1] = gi;
+    }
+    {
+        const YTab<T>* yt = ytab_of<T>(M);
+        for (int k = threadIdx.x; k < kL1 * kL1; k += NT) {
+            (&C.alm[0][0])[k] = (&yt->alm[0][0])[k];
+            (&C.ainv[0][0])[k] = (&yt->ainv[0][0])[k];
+            (&C.grad[0][0])[k] = (&yt->grad[0][0])[k];
+        }
+        if (threadIdx.x <= kL1) C.diag[threadIdx.x] = yt->diag[threadIdx.x];
+        if (threadIdx.x < kMaxFn) {
+            C.beta[threadIdx.x] = threadIdx.x < nfn ? M.beta[threadIdx.x] : 0.0;
+            C.mu[threadIdx.x] = threadIdx.x < nfn ? M.mu[threadIdx.x] : 0.0;
+        }
+    }
+    __syncthreads();
+    const int p0 = S.poff[atom], p1 = S.poff[atom + 1];
+    const double pir = kPiRef / M.cutoff;
+    double v0 = 0, v1 = 0, v2 = 0, v3 = 0, v4 = 0, v5 = 0;
+    for (int64_t p = p0 + threadIdx.x; p < p1; p += NT) {
+        const PairDel pg = pair_del(M, S, atom, p);
+        double fx = 0.0, fy = 0.0, fz = 0.0;
+        if (pg.valid) {
+            const int spec = S.elem[S.pnbr[p]];
+            const T* Gsp = Gs + (size_t)spec * nlmh * NF * 2;
+            const double ird = 1.0 / pg.r;
+            const T x = static_cast<T>(pg.dx * ird), y = static_cast<T>(pg.dy * ird),
+                    z = static_cast<T>(pg.dz * ird);
+            // radial functions and the cutoff function in FP64 in both modes
+            const double arg = pir * pg.r;
+            double sa, ca;
+            sin_cos(arg, &sa, &ca);
+            const double fc = 0.5 * (ca + 1.0);
+            const double fcd = -0.5 * pir * sa;
+            T gn[NF], gd[NF];
+#pragma unroll
+            for (int n = 0; n < NF; ++n) {
+                // padded n >= n_fn meet zero G columns
+                const double dd = pg.r - C.mu[n];
+                const double be = C.beta[n];
+                const double b = exp(-be * (dd * dd));
+                gn[n] = static_cast<T>(b * fc);
+                gd[n] = static_cast<T>(b * (fcd - 2.0 * be * dd * fc));
+                // one radial function at a time: interleaving the twelve
+                // exponentials costs more registers than it hides latency
+                __builtin_amdgcn_sched_barrier(0);
+            }
+            T er = T(0);                      // sum w Re(Gp Y)
+            T tx = T(0), ty = T(0), tz = T(0);  // sum w Re(Gq grad Y) * r
+            const T zz = T(1) - z * z;
+            T cr = T(1), ci = T(0);  // conj(u)^m
+            T mr = T(0), mi = T(0);  // conj(u)^(m-1)
+#pragma unroll 1
+            for (int m = 0; m <= maxl; ++m) {
+                const T sg = (m & 1) ? T(-1) : T(1);
+                const T w = (m == 0) ? T(1) : T(2);
+                const T ws = w * sg;
+                const T pr = cr * x + ci * y, pi = ci * x - cr * y;  // conj(u)^(m+1)
+                const T fm = static_cast<T>(m);
+                T a1 = T(0), a2 = T(0);  // Qb_{l-1}^m, Qb_{l-2}^m
+                T b1 = T(0), b2 = T(0);  // Qb_{l-1}^{m+1}, Qb_{l-2}^{m+1}
+                int lm = m * (m + 1) / 2;  // lmh(m, -m)
+#pragma unroll 1
+                for (int l = m; l <= maxl; ++l) {
+                    // contraction over n first: independent of the recursion
+                    T gpr = T(0), gpi = T(0), gqr = T(0), gqi = T(0);
+                    const T* gl = Gsp + (size_t)lm * NF * 2;
+#pragma unroll
+                    for (int n = 0; n < NF; ++n) {
+                        const T grr = gl[2 * n], gii = gl[2 * n + 1];
+                        gpr = fma(gd[n], grr, gpr);
+                        gpi = fma(gd[n], gii, gpi);
+                        gqr = fma(gn[n], grr, gqr);
+                        gqi = fma(gn[n], gii, gqi);
+                    }
+                    const T q = (l == m) ? C.diag[m] : C.alm[l][m] * (z * a1 - C.ainv[l][m] * a2);
+                    a2 = a1;
+                    a1 = q;
+                    T qp = T(0);  // Qb_l^{m+1}
+                    if (l == m + 1)
+                        qp = C.diag[m + 1];
+                    else if (l > m + 1)
+                        qp = C.alm[l][m + 1] * (z * b1 - C.ainv[l][m + 1] * b2);
+                    b2 = b1;
+                    b1 = qp;
+                    // Y_{l,-m} = sg q conj(u)^m
+                    er = fma(w * sg * q, gpr * cr - gpi * ci, er);
+                    T gxr, gxi, gyr, gyi, gzr, gzi;
+                    if (m == 0) {
+                        const T c = C.grad[l][0] * qp;
+                        gxr = c * z * x;
+                        gxi = T(0);
+                        gyr = c * z * y;
+                        gyi = T(0);
+                        gzr = -c * zz;
+                        gzi = T(0);
+                    } else {
+                        const T c = C.grad[l][m];
+                        const T A = fm * z * q + zz * c * qp;
+                        const T B = fm * q - z * c * qp;
+                        const T E = fm * (T(1) + z * z) * q + z * zz * c * qp;
+                        gxr = T(0.5) * (-B * pr + E * mr);
+                        gxi = T(0.5) * (-B * pi + E * mi);
+                        const T zr = B * pr + E * mr, zi = B * pi + E * mi;
+                        gyr = T(0.5) * zi;
+                        gyi = T(-0.5) * zr;
+                        gzr = -A * cr;
+                        gzi = -A * ci;
+                    }
+                    tx = fma(ws, gqr * gxr - gqi * gxi, tx);
+                    ty = fma(ws, gqr * gyr - gqi * gyi, ty);
+                    tz = fma(ws, gqr * gzr - gqi * gzi, tz);
+                    lm += l + 2;  // lmh(l+1, -m) = lmh(l, -m) + l + 2
+                }
+                mr = cr;
+                mi = ci;
+                cr = pr;
+                ci = pi;
+            }
+            // dE/d(del) = er * rhat + t / r
+            const double ir = ird;
+            fx = static_cast<double>(er) * (pg.dx * ir) + static_cast<double>(tx) * ir;
+            fy = static_cast<double>(er) * (pg.dy * ir) + static_cast<double>(ty) * ir;
+            fz = static_cast<double>(er) * (pg.dz * ir) + static_cast<double>(tz) * ir;
+        }
+        S.gpair[3 * p] = fx;
+        S.gpair[3 * p + 1] = fy;
+        S.gpair[3 * p + 2] = fz;
+        // virial: del (x) F_centre with F_centre = -dE/d(del)
+        v0 -= pg.dx * fx;
+        v1 -= pg.dy * fy;
+        v2 -= pg.dz * fz;
+        v3 -= pg.dx * fy;
+        v4 -= pg.dx * fz;
+        v5 -= pg.dy * fz;
+    }
+    // fixed-order reduction: butterfly within each wave, then over waves
+    for (int off = 32; off > 0; off >>= 1) {
+        v0 += __shfl_xor(v0, off);
+        v1 += __shfl_xor(v1, off);
+        v2 += __shfl_xor(v2, off);
+        v3 += __shfl_xor(v3, off);
+        v4 += __shfl_xor(v4, off);
+        v5 += __shfl_xor(v5, off);
+    }
+    const int wv = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        vred[0][wv] = v0;
+        vred[1][wv] = v1;
+        vred[2][wv] = v2;
+        vred[3][wv] = v3;
+        vred[4][wv] = v4;
+        vred[5][wv] = v5;
+    }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        double s = 0.0;
+        for (int k = 0; k < NT / 64; ++k) s += vred[threadIdx.x][k];
+        S.vown[6 * atom + threadIdx.x] = s;
+    }
+}
+
+void check(hipError_t e, const char* what) {
+    if (e != hipSuccess) std::fprintf(stderr, "mlip_hip: %s failed: %s\n", what, hipGetErrorString(e));
+}
+
+int slot_grid(const DevSystem& s) { return 8 * kBlock * ((s.nblocks + 7) / 8); }
+
+// radial indices per phase-2 thread of the a_nlm kernel
+int anlm_npt(const DevModel& m, int* ngroup) {
+    const int gmax = kAnlmThreads / m.n_lmh;
+    for (int npt : {2, 4, 6, 8, 16}) {
+        const int g = (m.n_fn + npt - 1) / npt;
+        if (g <= gmax) {
+            *ngroup = g;
+            return npt;
+        }
+    }
+    *ngroup = 1;
+    return 16;
+}
+
+template <typename T>
+void launch_anlm_t(const DevModel& m, const DevSystem& s, hipStream_t st) {
+    int ngroup = 1;
+    const int npt = anlm_npt(m, &ngroup);
+    const size_t lds = sizeof(typename Vec2<T>::type) * kAnlmJ * m.n_lmh;
+    const dim3 grid(slot_grid(s)), block(kAnlmThreads);
+    switch (npt) {
+        case 2: hipLaunchKernelGGL((k_anlm<2, T>), grid, block, lds, st, m, s, ngroup); break;
+        case 4: hipLaunchKernelGGL((k_anlm<4, T>), grid, block, lds, st, m, s, ngroup); break;
+        case 6: hipLaunchKernelGGL((k_anlm<6, T>), grid, block, lds, st, m, s, ngroup); break;
+        case 8: hipLaunchKernelGGL((k_anlm<8, T>), grid, block, lds, st, m, s, ngroup); break;
+        default: hipLaunchKernelGGL((k_anlm<16, T>), grid, block, lds, st, m, s, ngroup); break;
+    }
+}
+
+template <int NF, int NT, typename T>
+void launch_force_k(const DevModel& m, const DevSystem& s, hipStream_t st) {
+    const size_t lds = sizeof(T) * 2 * m.n_type * m.n_lmh * NF;
+    check(hipFuncSetAttribute(reinterpret_cast<const void*>(k_force<NF, NT, T>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)),
+          "k_force lds attr");
+    hipLaunchKernelGGL((k_force<NF, NT, T>), dim3(slot_grid(s)), dim3(NT), lds, st, m, s);
+}
+
+template <int NT, typename T>
+void launch_force_nt(const DevModel& m, const DevSystem& s, hipStream_t st) {
+    if (m.n_fn <= 4)
+        launch_force_k<4, NT, T>(m, s, st);
+    else if (m.n_fn <= 8)
+        launch_force_k<8, NT, T>(m, s, st);
+    else if (m.n_fn <= 12)
+        launch_force_k<12, NT, T>(m, s, st);
+    else
+        launch_force_k<16, NT, T>(m, s, st);
+}
+
+template <typename T>
+void launch_force_t(const DevModel& m, const DevSystem& s, int threads, hipStream_t st) {
+    switch (threads) {
+        case 64: launch_force_nt<64, T>(m, s, st); break;
+        case 192: launch_force_nt<192, T>(m, s, st); break;
+        case 256: launch_force_nt<256, T>(m, s, st); break;
+        default: launch_force_nt<128, T>(m, s, st); break;
+    }
+}
+
+}  // namespace
+
+void build_ylm_tables(YTab<double>& t64, YTab<float>& t32) {
+    long double dg = 1.0L / sqrtl(4.0L * 3.14159265358979323846264338327950288L);
+    for (int m = 0; m < kL1 + 1; ++m) {
+        if (m > 0) dg *= -sqrtl((2.0L * m + 1.0L) / (2.0L * m));
+        t64.diag[m] = static_cast<double>(dg);
+    }
+    for (int l = 0; l < kL1; ++l)
+        for (int m = 0; m < kL1; ++m) {
+            long double a = 0.0L, ai = 0.0L, g = 0.0L;
+            if (l == m + 1) {
+                a = sqrtl(2.0L * m + 3.0L);
+            } else if (l >= m + 2) {
+                a = sqrtl((4.0L * l * l - 1.0L) / ((long double)l * l - (long double)m * m));
+                const long double a1 =
+                    sqrtl((4.0L * (l - 1) * (l - 1) - 1.0L) / ((long double)(l - 1) * (l - 1) - (long double)m * m));
+                ai = 1.0L / a1;
+            }
+            if (m <= l) g = sqrtl((long double)(l + m + 1) * (l - m));
+            t64.alm[l][m] = static_cast<double>(a);
+            t64.ainv[l][m] = static_cast<double>(ai);
+            t64.grad[l][m] = static_cast<double>(g);
+        }
+    for (int m = 0; m < kL1 + 1; ++m) t32.diag[m] = static_cast<float>(t64.diag[m]);
+    for (int l = 0; l < kL1; ++l)
+        for (int m = 0; m < kL1; ++m) {
+            t32.alm[l][m] = static_cast<float>(t64.alm[l][m]);
+            t32.ainv[l][m] = static_cast<float>(t64.ainv[l][m]);
+            t32.grad[l][m] = static_cast<float>(t64.grad[l][m]);
+        }
+}
+
+void launch_anlm(const DevModel& m, const DevSystem& s, int precision, hipStream_t st) {
+    if (s.nblocks == 0) return;
+    if (precision == kMixed)
+        launch_anlm_t<float>(m, s, st);
+    else
+        launch_anlm_t<double>(m, s, st);
+    check(hipGetLastError(), "k_anlm launch");
+}
+
+void launch_force(const DevModel& m, const DevSystem& s, int precision, int threads, hipStream_t st) {
+    if (s.nblocks == 0) return;
+    if (precision == kMixed)
+        launch_force_t<float>(m, s, threads, st);
+    else
+        launch_force_t<double>(m, s, threads, st);
+    check(hipGetLastError(), "k_force launch");
+}
+
+}  // namespace mlip
