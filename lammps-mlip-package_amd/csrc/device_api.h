@@ -11,7 +11,7 @@
 
 namespace mlip {
 
-constexpr int kBlock = 64;        // atoms per slot block (one wavefront of lanes)
+constexpr int kBlock = kBlockLanes;  // atoms per slot block (one wavefront of lanes)
 
 
 enum Precision : int { kFP64 = 0, kMixed = 1 };
@@ -44,6 +44,8 @@ struct DevModel {
     const int32_t* c3_off;  // cubic terms transposed by target: offsets (n_p+1)
     const int32_t* c3_bc;   // 2 per entry
     const double* c3_w;
+    int q34_red;            // doubles of LDS partial rows of split adjoint groups
+    int q34_dyn_adj, q34_dyn_feat;  // visits for the general-loop order-3/4 kernel
     const YTab<double>* yt64;
     const YTab<float>* yt32;
 };
@@ -88,6 +90,9 @@ void launch_anlm(const DevModel& m, const DevSystem& s, int precision, hipStream
 void launch_q12(const DevModel& m, const DevSystem& s, bool feat, hipStream_t st);
 void launch_q34_adj(const DevModel& m, const DevSystem& s, hipStream_t st);
 void launch_q34_feat(const DevModel& m, const DevSystem& s, hipStream_t st);
+// general-loop passes for visits outside the unrolled signatures (launched
+// by the two functions above when the model has such visits)
+void launch_q34_dyn(const DevModel& m, const DevSystem& s, bool feat, hipStream_t st);
 void launch_poly(const DevModel& m, const DevSystem& s, hipStream_t st);
 // threads per workgroup of the force kernel: 64, 128 (default), 192 or 256
 void launch_force(const DevModel& m, const DevSystem& s, int precision, int threads, hipStream_t st);

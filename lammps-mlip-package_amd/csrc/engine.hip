@@ -135,6 +135,9 @@ bool upload_model(mlip_ctx* c) {
     dm.n_pitc = m.n_pitc;
     dm.n_p = m.n_p;
     dm.has_q34 = m.has_order34 ? 1 : 0;
+    dm.q34_red = m.q34_red;
+    dm.q34_dyn_adj = m.q34_dyn_adj ? 1 : 0;
+    dm.q34_dyn_feat = m.q34_dyn_feat ? 1 : 0;
     dm.cutoff = m.cutoff;
     for (int n = 0; n < m.n_fn; ++n) {
         dm.beta[n] = m.beta[n];

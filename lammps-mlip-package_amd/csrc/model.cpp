@@ -323,7 +323,7 @@ void build_type_tables(Model& m, int t, const std::vector<std::map<std::vector<i
     tt.q12_off[m.maxl + 1] = static_cast<int>(tt.visits_q12.size());
     // group by (S, L), balance groups over the waves of a workgroup
     auto make_groups = [&](std::vector<Pending>& pend, std::vector<GroupQ34>& groups,
-                           std::vector<VisitQ34>& visits) {
+                           std::vector<VisitQ34>& visits, bool adjoint, int& red_total, bool& dyn) {
         // group by (S, L); inside a group, visits with the same code path
         // (signature and flags) are adjacent for instruction-cache locality
         std::stable_sort(pend.begin(), pend.end(), [](const Pending& a, const Pending& b) {
@@ -333,21 +333,58 @@ void build_type_tables(Model& m, int t, const std::vector<std::map<std::vector<i
             const auto kb = std::make_tuple(b.v.flags & 4, b.v.order, b.v.la, b.v.lb, b.v.lc, b.v.flags & 1);
             return ka < kb;
         });
+        long total = 0;
+        for (const Pending& q : pend) {
+            total += q.cost;
+            if (!(q.v.flags & 4)) dyn = true;
+        }
+        // rows heavier than a wave's share are split into balanced parts
+        const long share = std::max<long>(1, total / kWavesPerWg);
         std::vector<long> gcost;
+        int red = 0;
         for (size_t k = 0; k < pend.size();) {
             size_t e = k;
             long c = 0;
             while (e < pend.size() && pend[e].S == pend[k].S && pend[e].L == pend[k].L) c += pend[e++].cost;
-            GroupQ34 g{};
-            g.S = pend[k].S;
-            g.L = pend[k].L;
-            g.vbeg = static_cast<int>(visits.size());
-            for (size_t j = k; j < e; ++j) visits.push_back(pend[j].v);
-            g.vend = static_cast<int>(visits.size());
-            groups.push_back(g);
-            gcost.push_back(c);
+            const int L = pend[k].L;
+            int nparts = 1;
+            if (c > share) nparts = static_cast<int>(std::min<long>((c + share - 1) / share, kWavesPerWg));
+            nparts = static_cast<int>(std::min<size_t>(nparts, e - k));
+            long acc = 0;
+            size_t j = k;
+            for (int part = 0; part < nparts; ++part) {
+                GroupQ34 g{};
+                g.S = pend[k].S;
+                g.L = L;
+                g.part = part;
+                g.nparts = nparts;
+                g.red = (adjoint && nparts > 1) ? red + part * (L + 1) * 2 * kBlockLanes : 0;
+                g.vbeg = static_cast<int>(visits.size());
+                long pc = 0;
+                if (part == nparts - 1) {
+                    while (j < e) {
+                        pc += pend[j].cost;
+                        visits.push_back(pend[j++].v);
+                    }
+                } else {
+                    // at least one visit, then up to this part's cumulative
+                    // share, leaving one visit for each later part
+                    const long goal = c * (part + 1) / nparts;
+                    const size_t keep = static_cast<size_t>(nparts - 1 - part);
+                    do {
+                        pc += pend[j].cost;
+                        visits.push_back(pend[j++].v);
+                    } while (j < e && acc + pc < goal && e - j > keep);
+                }
+                acc += pc;
+                g.vend = static_cast<int>(visits.size());
+                groups.push_back(g);
+                gcost.push_back(pc);
+            }
+            if (adjoint && nparts > 1) red += nparts * (L + 1) * 2 * kBlockLanes;
             k = e;
         }
+        red_total = std::max(red_total, red);
         std::vector<size_t> order(groups.size());
         for (size_t k = 0; k < order.size(); ++k) order[k] = k;
         std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return gcost[a] > gcost[b]; });
@@ -358,8 +395,9 @@ void build_type_tables(Model& m, int t, const std::vector<std::map<std::vector<i
             load[w] += gcost[k] + 16;
         }
     };
-    make_groups(adj, tt.groups_adj, tt.visits_adj);
-    make_groups(feat, tt.groups_feat, tt.visits_feat);
+    int feat_red = 0;
+    make_groups(adj, tt.groups_adj, tt.visits_adj, true, m.q34_red, m.q34_dyn_adj);
+    make_groups(feat, tt.groups_feat, tt.visits_feat, false, feat_red, m.q34_dyn_feat);
     m.types[t] = std::move(tt);
 }
 

@@ -20,7 +20,8 @@ namespace mlip {
 constexpr int kMaxFn = 16;      // radial functions supported by the kernels
 constexpr int kMaxL = 12;       // max angular momentum of a_nlm
 constexpr int kMaxOrderGpu = 4; // highest invariant order of the HIP path
-constexpr int kWavesPerWg = 8;  // waves sharing one a_nlm tile in the adjoint kernel
+constexpr int kWavesPerWg = 12; // waves sharing one a_nlm tile in the order-3/4 kernel
+constexpr int kBlockLanes = 64; // centres per slot block (the lanes of one wave)
 constexpr int kUnrollCap3 = 8;  // order-3 visits with all l <= 8 run fully unrolled (q34_signatures.inc)
 constexpr int kUnrollCap4 = 2;  // order-4 visits with all l <= 2 run fully unrolled
 
@@ -73,11 +74,17 @@ struct VisitQ34 {
 
 // Output group: all visits of one (species S, angular momentum L) handled by
 // one wave of the adjoint kernel with a statically unrolled G_L accumulator.
+// Output-stationary group of the order-3/4 kernel: visits that all add into
+// G_{S,L,m}.  A heavy (S, L) row is split into nparts sub-groups on
+// different waves; each writes its partial row to LDS at `red` (doubles
+// from the partial area) and the wave of part 0 sums them in part order.
 struct GroupQ34 {
     int32_t S, L;
     int32_t vbeg, vend;
     int32_t wave;      // wave of the workgroup that owns the group
-    int32_t pad;
+    int32_t part;      // sub-group index within its (S, L) row
+    int32_t nparts;    // sub-groups of the row (1: not split)
+    int32_t red;       // partial-row offset of this sub-group (split rows only)
 };
 
 struct TypeTables {
@@ -122,6 +129,9 @@ struct Model {
     std::vector<TypeTables> types;
     bool has_order34 = false;
     bool has_feat34 = false;
+    int q34_red = 0;           // doubles of LDS partial rows (adjoint pass), max over types
+    bool q34_dyn_adj = false;  // visits outside the unrolled signatures (general loop kernel)
+    bool q34_dyn_feat = false;
 };
 
 // Parse a potential file (format of mlip_model.h:45-101) and build every

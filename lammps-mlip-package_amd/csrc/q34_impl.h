@@ -1,15 +1,20 @@
 // Order-3/4 invariants of the gtinv model on CDNA4: polynomial features
 // (FEAT = true) or the adjoint G = dE/da (FEAT = false).
 //
-// Workgroup = 8 waves on one (block of 64 same-type centres, radial index
-// n).  The a_nlm tile (l <= lmax3, both species) sits in LDS as
-// [s][lm][re,im][lane]; each wave owns whole (S, L) output groups and keeps
-// G_{S,L,m<=0} in registers.  Reference: compute_anlm_uniq_products and the
+// Workgroup = kWavesPerWg waves on one (block of 64 same-type centres,
+// radial index n).  The a_nlm tile (l <= lmax3, both species) sits in LDS as
+// [s][lm][re,im][lane]; each wave owns (S, L) output groups (or balanced
+// parts of heavy ones) and keeps G_{S,L,m<=0} in registers.  Visits with
+// fully unrolled signatures run in one kernel, the rest in a second
+// general-loop kernel (only launched for models that have such visits), so
+// the common kernel's register budget allows more waves per SIMD.  Reference: compute_anlm_uniq_products and the
 // linear/polynomial adjoint terms of src/USER-MLIP/pair_mlip_gtinv.cpp
 // (:260-343, :451-521), docs/formulation.md (adjoint G).
 #pragma once
 
 #include <hip/hip_runtime.h>
+
+#include <cstdio>
 
 #include "device_api.h"
 #include "device_common.h"
@@ -21,9 +26,6 @@ using namespace dev;
 
 constexpr int kQ34Threads = 64 * kWavesPerWg;
 
-// Workgroup = 8 waves on one (block of 64 same-type centres, n).  The a_nlm
-// tile (l <= lmax3, both species) sits in LDS as [s][lm][re,im][lane]; each
-// wave owns whole (S, L) output groups and keeps G_{S,L,m<=0} in registers.
 struct Tile {
     const double* As;
     int nlmh3;
@@ -121,23 +123,33 @@ __device__ __forceinline__ void visit4(const Tile& T, const DevModel& M, const V
 }
 
 // Fully unrolled visits for the signatures of q34_signatures.inc.  The b
-// (and c) rows are loaded once per visit into registers, with the m > 0
-// entries formed by the static conjugation a_{l,m} = (-1)^m conj(a_{l,-m});
-// the a factor is read once per m_a.  Coefficients are compact (valid terms
-// in loop order), so every coefficient offset is a compile-time constant.
+// (and c) rows are loaded once per visit into registers, m <= 0 entries
+// only: an m > 0 factor is a_{l,m} = (-1)^m conj(a_{l,-m}), whose sign and
+// conjugation fold into the (fully unrolled, so compile-time) product.  The
+// a factor is read once per m_a.  Coefficients are compact (valid terms in
+// loop order), so every coefficient offset is a compile-time constant.
 template <int LB>
-__device__ __forceinline__ void load_row(const Tile& T, int s, double (&r)[2 * LB + 1], double (&i)[2 * LB + 1]) {
+__device__ __forceinline__ void load_row(const Tile& T, int s, double (&r)[LB + 1], double (&i)[LB + 1]) {
 #pragma unroll
-    for (int k = 0; k <= LB; ++k) {
-        double re, im;
-        T.get(s, LB, -k, re, im);
-        r[LB - k] = re;
-        i[LB - k] = im;
-        if (k > 0) {
-            const double sg = (k & 1) ? -1.0 : 1.0;
-            r[LB + k] = sg * re;
-            i[LB + k] = -sg * im;
-        }
+    for (int k = 0; k <= LB; ++k) T.get(s, LB, -k, r[k], i[k]);
+}
+
+// a_{l,m} from a stored row of m <= 0 entries (m is a compile-time value
+// after unrolling, so the selects fold away)
+template <int LB>
+__device__ __forceinline__ void factor(const double (&r)[LB + 1], const double (&i)[LB + 1], int m, double& yr,
+                                       double& yi) {
+    const int am = m < 0 ? -m : m;
+    const double re = r[am], im = i[am];
+    if (m <= 0) {
+        yr = re;
+        yi = im;
+    } else if (m & 1) {
+        yr = -re;
+        yi = im;
+    } else {
+        yr = re;
+        yi = -im;
     }
 }
 
@@ -145,14 +157,17 @@ template <int L, int LA, int LB, bool DOG, bool DOD>
 __device__ __forceinline__ void visit3u(const Tile& T, int S, int sa, int sb, cptr<double> cg,
                                         cptr<double> cd, double H, double (&gr)[L + 1],
                                         double (&gi)[L + 1], double& d) {
-    double br[2 * LB + 1], bi[2 * LB + 1];
+    double br[LB + 1], bi[LB + 1];
     load_row<LB>(T, sb, br, bi);
     int t = 0;
+    double xr, xi;
+    T.get(sa, LA, -LA, xr, xi);
 #pragma unroll
     for (int ia = 0; ia <= 2 * LA; ++ia) {
         const int ma = ia - LA;
-        double xr, xi;
-        T.get(sa, LA, ma, xr, xi);
+        // next a factor requested one m_a ahead
+        double nxr = 0.0, nxi = 0.0;
+        if (ia < 2 * LA) T.get(sa, LA, ma + 1, nxr, nxi);
         // d-forming visits read the output factor a_{S,L,-k} term by term; the
         // barrier keeps the compiler from hoisting those reads into registers
         if (DOD) asm volatile("" ::: "memory");
@@ -160,7 +175,8 @@ __device__ __forceinline__ void visit3u(const Tile& T, int S, int sa, int sb, cp
         for (int k = 0; k <= L; ++k) {
             const int mb = k - ma;
             if (mb < -LB || mb > LB) continue;
-            const double yr = br[mb + LB], yi = bi[mb + LB];
+            double yr, yi;
+            factor<LB>(br, bi, mb, yr, yi);
             const double pr = xr * yr - xi * yi;
             const double pi = xr * yi + xi * yr;
             if (DOG) {
@@ -175,6 +191,11 @@ __device__ __forceinline__ void visit3u(const Tile& T, int S, int sa, int sb, cp
             }
             ++t;
         }
+        // one m_a per scheduling region: interleaving all of them costs more
+        // registers (and occupancy) than it hides latency
+        __builtin_amdgcn_sched_barrier(0);
+        xr = nxr;
+        xi = nxi;
     }
 }
 
@@ -182,7 +203,7 @@ template <int L, int LA, int LB, int LC, bool DOG, bool DOD>
 __device__ __forceinline__ void visit4u(const Tile& T, int S, int sa, int sb, int sc, cptr<double> cg,
                                         cptr<double> cd, double H, double (&gr)[L + 1],
                                         double (&gi)[L + 1], double& d) {
-    double br[2 * LB + 1], bi[2 * LB + 1], cr[2 * LC + 1], ci[2 * LC + 1];
+    double br[LB + 1], bi[LB + 1], cr[LC + 1], ci[LC + 1];
     load_row<LB>(T, sb, br, bi);
     load_row<LC>(T, sc, cr, ci);
     int t = 0;
@@ -194,14 +215,17 @@ __device__ __forceinline__ void visit4u(const Tile& T, int S, int sa, int sb, in
 #pragma unroll
         for (int ib = 0; ib <= 2 * LB; ++ib) {
             const int mb = ib - LB;
-            const double qr = xr * br[ib] - xi * bi[ib];
-            const double qi = xr * bi[ib] + xi * br[ib];
+            double yr, yi;
+            factor<LB>(br, bi, mb, yr, yi);
+            const double qr = xr * yr - xi * yi;
+            const double qi = xr * yi + xi * yr;
             if (DOD) asm volatile("" ::: "memory");
 #pragma unroll
             for (int k = 0; k <= L; ++k) {
                 const int mc = k - ma - mb;
                 if (mc < -LC || mc > LC) continue;
-                const double zr = cr[mc + LC], zi = ci[mc + LC];
+                double zr, zi;
+                factor<LC>(cr, ci, mc, zr, zi);
                 const double pr = qr * zr - qi * zi;
                 const double pi = qr * zi + qi * zr;
                 if (DOG) {
@@ -216,6 +240,7 @@ __device__ __forceinline__ void visit4u(const Tile& T, int S, int sa, int sb, in
                 }
                 ++t;
             }
+            __builtin_amdgcn_sched_barrier(0);
         }
     }
 }
@@ -263,9 +288,9 @@ struct Unrolled {
     }
 };
 
-template <int L, bool FEAT>
-__device__ __forceinline__ void q34_group(const Tile& T, const DevModel& M, const DevSystem& S, const GroupQ34& g, int blk,
-                          int n, double& eacc) {
+template <int L, bool FEAT, bool DYN>
+__device__ __forceinline__ void q34_group(const Tile& T, const DevModel& M, const DevSystem& S, const GroupQ34& g,
+                                          int blk, int n, double* red, double& eacc) {
     double gr[L + 1], gi[L + 1];
     const int Sx = uniform(g.S);
 #pragma unroll
@@ -276,18 +301,21 @@ __device__ __forceinline__ void q34_group(const Tile& T, const DevModel& M, cons
     for (int vi = vb; vi < ve; ++vi) {
         const VisitQ34 v = vis[vi];
         const int order = uniform(v.order), flags = uniform(v.flags), pslot = uniform(v.pslot);
+        const bool fast = (flags & 4) != 0;
+        if (fast == DYN) continue;  // the other kernel's visit
         const int itc = uniform(v.itc);
         double d = 0.0;
-        const bool fast = (flags & 4) != 0;
         const int la = uniform(v.la), lb = uniform(v.lb), lc = uniform(v.lc);
         cptr<double> cg = as_const(M.coef) + uniform(v.coef);
         cptr<double> cd = as_const(M.coef) + uniform(v.dcoef);
         if (FEAT) {
-            if (!(fast && Unrolled<L, false, true>::run(T, v, Sx, order, la, lb, lc, cg, cd, 0.0, gr, gi, d))) {
+            if constexpr (DYN) {
                 if (order == 3)
                     visit3<L, false, true>(T, M, v, Sx, 0.0, gr, gi, d);
                 else
                     visit4<L, false, true>(T, M, v, Sx, 0.0, gr, gi, d);
+            } else {
+                Unrolled<L, false, true>::run(T, v, Sx, order, la, lb, lc, cg, cd, 0.0, gr, gi, d);
             }
             S.D[(size_t)slot * M.n_p + n * M.n_pitc + pslot] = d;
             continue;
@@ -295,35 +323,68 @@ __device__ __forceinline__ void q34_group(const Tile& T, const DevModel& M, cons
         const double H =
             pslot >= 0 ? S.H[(size_t)slot * M.n_p + n * M.n_pitc + pslot] : as_const(M.weight)[n * M.n_gtinv + itc];
         if (flags & 1) {
-            if (!(fast && Unrolled<L, true, true>::run(T, v, Sx, order, la, lb, lc, cg, cd, H, gr, gi, d))) {
+            if constexpr (DYN) {
                 if (order == 3)
                     visit3<L, true, true>(T, M, v, Sx, H, gr, gi, d);
                 else
                     visit4<L, true, true>(T, M, v, Sx, H, gr, gi, d);
+            } else {
+                Unrolled<L, true, true>::run(T, v, Sx, order, la, lb, lc, cg, cd, H, gr, gi, d);
             }
             eacc = fma(as_const(M.weight)[n * M.n_gtinv + itc], d, eacc);
         } else {
-            if (!(fast && Unrolled<L, true, false>::run(T, v, Sx, order, la, lb, lc, cg, cd, H, gr, gi, d))) {
+            if constexpr (DYN) {
                 if (order == 3)
                     visit3<L, true, false>(T, M, v, Sx, H, gr, gi, d);
                 else
                     visit4<L, true, false>(T, M, v, Sx, H, gr, gi, d);
+            } else {
+                Unrolled<L, true, false>::run(T, v, Sx, order, la, lb, lc, cg, cd, H, gr, gi, d);
             }
         }
     }
     if (!FEAT) {
+        if (uniform(g.nparts) == 1) {
 #pragma unroll
-        for (int k = 0; k <= L; ++k) {
-            const size_t ir = a_index(M, blk, n, Sx, lmh(L, -k), 0, T.lane);
-            const size_t ii = a_index(M, blk, n, Sx, lmh(L, -k), 1, T.lane);
-            S.G[ir] += gr[k];
-            S.G[ii] += gi[k];
+            for (int k = 0; k <= L; ++k) {
+                const size_t ir = a_index(M, blk, n, Sx, lmh(L, -k), 0, T.lane);
+                const size_t ii = a_index(M, blk, n, Sx, lmh(L, -k), 1, T.lane);
+                S.G[ir] += gr[k];
+                S.G[ii] += gi[k];
+            }
+        } else {
+            double* pr = red + uniform(g.red);
+#pragma unroll
+            for (int k = 0; k <= L; ++k) {
+                pr[(2 * k) * kBlock + T.lane] = gr[k];
+                pr[(2 * k + 1) * kBlock + T.lane] = gi[k];
+            }
         }
     }
 }
 
+// Sum of the partial rows of a split (S, L) row, in part order, into G.
+__device__ __forceinline__ void q34_combine(const DevModel& M, const DevSystem& S, const GroupQ34& g, int blk, int n,
+                                            int lane, const double* red) {
+    const int L = uniform(g.L), Sx = uniform(g.S), np = uniform(g.nparts);
+    const double* base = red + uniform(g.red);
+    const int stride = (L + 1) * 2 * kBlock;
+    for (int k = 0; k <= L; ++k) {
+        double sr = 0.0, si = 0.0;
+        for (int j = 0; j < np; ++j) {
+            sr += base[j * stride + (2 * k) * kBlock + lane];
+            si += base[j * stride + (2 * k + 1) * kBlock + lane];
+        }
+        S.G[a_index(M, blk, n, Sx, lmh(L, -k), 0, lane)] += sr;
+        S.G[a_index(M, blk, n, Sx, lmh(L, -k), 1, lane)] += si;
+    }
+}
 
-template <bool FEAT>
+// largest output L with unrolled signatures (groups above it hold only
+// general-loop visits)
+constexpr int kMaxLUnrolled = MLIP_CAP3;
+
+template <bool FEAT, bool DYN>
 __global__ __launch_bounds__(kQ34Threads) void k_q34(DevModel M, DevSystem S) {
     extern __shared__ double As[];
     const int blk = blockIdx.x, n = blockIdx.y;
@@ -333,6 +394,7 @@ __global__ __launch_bounds__(kQ34Threads) void k_q34(DevModel M, DevSystem S) {
         const double* src = S.A + a_index(M, blk, n, s, 0, 0, 0);
         for (int k = threadIdx.x; k < per_s; k += kQ34Threads) As[s * per_s + k] = src[k];
     }
+    double* red = As + M.n_type * per_s;  // partial rows of split groups
     __syncthreads();
     const int wave = uniform(threadIdx.x >> 6);
     Tile T{As, M.n_lmh3, static_cast<int>(threadIdx.x & 63)};
@@ -343,24 +405,56 @@ __global__ __launch_bounds__(kQ34Threads) void k_q34(DevModel M, DevSystem S) {
     for (int gi = gb; gi < ge; ++gi) {
         const GroupQ34 g = groups[gi];
         if (uniform(g.wave) != wave) continue;
-        switch (uniform(g.L)) {
-            case 0: q34_group<0, FEAT>(T, M, S, g, blk, n, eacc); break;
-            case 1: q34_group<1, FEAT>(T, M, S, g, blk, n, eacc); break;
-            case 2: q34_group<2, FEAT>(T, M, S, g, blk, n, eacc); break;
-            case 3: q34_group<3, FEAT>(T, M, S, g, blk, n, eacc); break;
-            case 4: q34_group<4, FEAT>(T, M, S, g, blk, n, eacc); break;
-            case 5: q34_group<5, FEAT>(T, M, S, g, blk, n, eacc); break;
-            case 6: q34_group<6, FEAT>(T, M, S, g, blk, n, eacc); break;
-            case 7: q34_group<7, FEAT>(T, M, S, g, blk, n, eacc); break;
-            case 8: q34_group<8, FEAT>(T, M, S, g, blk, n, eacc); break;
-            case 9: q34_group<9, FEAT>(T, M, S, g, blk, n, eacc); break;
-            case 10: q34_group<10, FEAT>(T, M, S, g, blk, n, eacc); break;
+        const int L = uniform(g.L);
+        if (!DYN && L > kMaxLUnrolled) continue;
+        switch (L) {
+            case 0: q34_group<0, FEAT, DYN>(T, M, S, g, blk, n, red, eacc); break;
+            case 1: q34_group<1, FEAT, DYN>(T, M, S, g, blk, n, red, eacc); break;
+            case 2: q34_group<2, FEAT, DYN>(T, M, S, g, blk, n, red, eacc); break;
+            case 3: q34_group<3, FEAT, DYN>(T, M, S, g, blk, n, red, eacc); break;
+            case 4: q34_group<4, FEAT, DYN>(T, M, S, g, blk, n, red, eacc); break;
+            case 5: q34_group<5, FEAT, DYN>(T, M, S, g, blk, n, red, eacc); break;
+            case 6: q34_group<6, FEAT, DYN>(T, M, S, g, blk, n, red, eacc); break;
+            case 7: q34_group<7, FEAT, DYN>(T, M, S, g, blk, n, red, eacc); break;
+            case 8: q34_group<8, FEAT, DYN>(T, M, S, g, blk, n, red, eacc); break;
+            case 9:
+                if constexpr (DYN) q34_group<9, FEAT, DYN>(T, M, S, g, blk, n, red, eacc);
+                break;
+            case 10:
+                if constexpr (DYN) q34_group<10, FEAT, DYN>(T, M, S, g, blk, n, red, eacc);
+                break;
             default: break;
         }
     }
-    if (!FEAT) S.Eq34[((size_t)(blk * kBlock + T.lane) * M.n_fn + n) * kWavesPerWg + wave] = eacc;
+    if (!FEAT) {
+        if (M.q34_red > 0) {
+            __syncthreads();
+            for (int gi = gb; gi < ge; ++gi) {
+                const GroupQ34 g = groups[gi];
+                if (uniform(g.wave) != wave || uniform(g.nparts) == 1 || uniform(g.part) != 0) continue;
+                if (!DYN && uniform(g.L) > kMaxLUnrolled) continue;
+                q34_combine(M, S, g, blk, n, T.lane, red);
+            }
+        }
+        double* e = &S.Eq34[((size_t)(blk * kBlock + T.lane) * M.n_fn + n) * kWavesPerWg + wave];
+        if (DYN)
+            *e += eacc;
+        else
+            *e = eacc;
+    }
 }
 
+template <bool FEAT, bool DYN>
+void launch(const DevModel& m, const DevSystem& s, hipStream_t st) {
+    const size_t lds = sizeof(double) * (static_cast<size_t>(m.n_type) * m.n_lmh3 * 2 * kBlock + (FEAT ? 0 : m.q34_red));
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k_q34<FEAT, DYN>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL((k_q34<FEAT, DYN>), dim3(s.nblocks, m.n_fn), dim3(kQ34Threads), lds, st, m, s);
+        e = hipGetLastError();
+    }
+    if (e != hipSuccess) std::fprintf(stderr, "mlip_hip: k_q34 launch failed: %s\n", hipGetErrorString(e));
+}
 
 }  // namespace q34
 }  // namespace mlip
