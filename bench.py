@@ -30,7 +30,8 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "lammps-mlip-package_amd"))
 
-from mlip_hip import PairMLIPGtinv, ResidentSystem, inspect_model, periodic_full_list  # noqa: E402
+from mlip_hip import BatchSystem, PairMLIPGtinv, ResidentSystem, inspect_model, periodic_full_list  # noqa: E402
+from mlip_hip import serve as SV  # noqa: E402
 from mlip_hip import workloads as W  # noqa: E402
 
 FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 (vector and matrix)
@@ -92,6 +93,75 @@ def timed_steps(rs, warmup, steps, tag):
     return dt, prof, rs.get(), nt
 
 
+def make_pair(pot, precision):
+    p = PairMLIPGtinv()
+    p.settings()
+    p.coeff("*", "*", pot, "Ti", "Al")
+    p.set_precision(precision)
+    return p
+
+
+def batched_section(pot, steps, precision="mixed"):
+    """4096 independent structures (log-uniform 8-512 atoms, seed 6) in one
+    batched evaluation, replayed as a captured HIP graph; force launch shape
+    from the autotuner."""
+    p = make_pair(pot, precision)
+    structs = W.random_structures(4096, 8, 512, seed=6)
+    natoms = sum(s.natoms for s in structs)
+    t0 = time.time()
+    b = BatchSystem(p, structs)
+    t_setup = time.time() - t0
+    b.compute()
+    b.sync()
+    nt = b.autotune(reps=2)
+    b.graph(True)
+    for _ in range(2):  # eager after the change, then the capture
+        b.compute()
+    b.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        b.compute()
+    b.sync()
+    dt = time.perf_counter() - t0
+    r = b.get(forces=False)
+    if not np.all(np.isfinite(r["energies"])):
+        raise SystemExit("non-finite batched result")
+    out = {"value": natoms * steps / dt, "unit": "atom-evals/s", "ms_per_eval": 1e3 * dt / steps,
+           "nstruct": len(structs), "natoms": natoms, "precision": precision, "graph": True,
+           "force_threads": nt, "host_setup_s": t_setup,
+           "workload": "4096 random triclinic cells, 8-512 atoms log-uniform (seed 6), 0.06 atoms/A^3, "
+                       "1.8 A rejection, two species with random fraction; same model"}
+    log("batched %s: %d structures, %d atoms, %.2f ms/eval, %.4g atom-evals/s (setup %.1fs)"
+        % (precision, len(structs), natoms, out["ms_per_eval"], out["value"], t_setup))
+    del b
+    return out
+
+
+def served_section(pot, rates, windows, budget_gb):
+    """Poisson request streams (seed 7) through the continuous-batching pool
+    in mixed precision, and the same streams one request at a time in FP64."""
+    masses = [47.88, 26.98]
+    p = make_pair(pot, "mixed")
+    pool, cap = SV.pool_for_budget(p, int(budget_gb * (1 << 30)), max_requests=8192)
+    SV.serve(pool, SV.make_stream(1000, 0.05, 1, masses))  # warm-up stream
+    q = make_pair(pot, "fp64")
+    seq_pool = SV.RequestPool(q, 8192, 8192 * 200, 4)
+    SV.serve_sequential(seq_pool, SV.make_stream(100, 0.05, 1, masses))
+    out = {"pool_atoms": cap, "hbm_budget_gb": budget_gb, "precision": "mixed", "rates": {}}
+    for rate, win in zip(rates, windows):
+        stream = SV.make_stream(rate, win, 7, masses)
+        r = SV.serve(pool, stream)
+        seq = SV.serve_sequential(seq_pool, SV.make_stream(rate, win, 7, masses))
+        r["arrival_window_s"] = win
+        r["fp64_sequential"] = seq
+        out["rates"][str(rate)] = r
+        log("served %d req/s (%d requests): p50 %.1f ms p99 %.1f ms, %.4g atom-evals/s; "
+            "FP64 one-at-a-time: p50 %.1f ms p99 %.1f ms, %.4g atom-evals/s"
+            % (rate, len(stream), r["latency_p50_ms"], r["latency_p99_ms"], r["atom_evals_per_s"],
+               seq["latency_p50_ms"], seq["latency_p99_ms"], seq["atom_evals_per_s"]))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -102,6 +172,11 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU-oracle work")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision timing")
+    ap.add_argument("--no-batched", action="store_true", help="skip the 4096-structure batched evaluation")
+    ap.add_argument("--no-served", action="store_true", help="skip the served request streams")
+    ap.add_argument("--serve-windows", type=str, default="1.0,0.5,0.25",
+                    help="arrival windows (s) for the 200/1000/5000 req/s streams")
+    ap.add_argument("--pool-gb", type=float, default=16.0, help="HBM budget of the request pool")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="also launch a plain copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -145,6 +220,14 @@ def main():
     if args.calibrate_copy_mib > 0:
         rs.calibration_copy(args.calibrate_copy_mib << 20, reps=3)
 
+    npairs = rs.npairs
+    del rs
+    batched = None if args.no_batched else batched_section(pot, max(args.steps, 3))
+    served = None
+    if not args.no_served:
+        served = served_section(pot, [200, 1000, 5000], [float(w) for w in args.serve_windows.split(",")],
+                                args.pool_gb)
+
     natoms = s.natoms
     value = natoms * args.steps / dt
     kernels = {k: {"avg_ms": v[0] / max(v[1], 1), "launches": v[1]} for k, v in prof.items()}
@@ -184,11 +267,13 @@ def main():
                                "[12,8,2], model_type 4 max_p 2 (239,184 coeffs), r_c 8 A, 12 Gaussians; "
                                "one E+F+virial evaluation per step, inputs resident in HBM"
                                % (args.ncell, args.ncell, args.ncell, natoms),
-                   "natoms": natoms, "pairs_per_atom": rs.npairs / natoms, "precision": args.precision,
+                   "natoms": natoms, "pairs_per_atom": npairs / natoms, "precision": args.precision,
                    "force_threads": nt},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "mixed": mixed,
+        "batched": batched,
+        "served": served,
         "kernels_ms": {k: round(v["avg_ms"], 4) for k, v in kernels.items()},
         "flops_per_step": flops,
         "step_tflops": total_rate,

@@ -26,7 +26,8 @@ enum {
     MLIP_ERR_FILE = 2,    /* potential file missing / malformed */
     MLIP_ERR_MODEL = 3,   /* model outside what the HIP path evaluates */
     MLIP_ERR_DEVICE = 4,  /* HIP or rocBLAS failure */
-    MLIP_ERR_STATE = 5    /* call order (e.g. resident compute before resident set) */
+    MLIP_ERR_STATE = 5,   /* call order (e.g. resident compute before resident set) */
+    MLIP_ERR_FULL = 6     /* request pool: no room for the request now (retry after completions) */
 };
 
 enum { MLIP_FP64 = 0, MLIP_MIXED = 1 };
@@ -166,6 +167,52 @@ void mlip_nlist_free(mlip_nlist* nl);
  * of coefficients the model defines (the file's own count is not checked).  Returns the
  * number of ints written or a negative value with errbuf set. */
 int mlip_model_inspect(const char* potfile, int which, int* out, int64_t cap, char* errbuf, size_t errlen);
+
+
+/* ---- Request pool: continuous batching (serving) --------------------------
+ * No counterpart in the reference (a pair style evaluates one LAMMPS
+ * system); these entry points keep the per-structure semantics of
+ * mlip_compute for many independent structures resident on the device.
+ *
+ * mlip_pool_init reserves device capacity for max_atoms atoms, max_pairs
+ * directed pairs (neighbor list with skin) and max_requests requests;
+ * mlip_pool_bytes gives the HBM that capacity takes, so a caller can size the
+ * pool from a memory budget.  mlip_pool_admit copies one structure (cell rows
+ * a, b, c; 1-based types; optional velocities in A/ps) into the pool with its
+ * neighbor list built at cutoff + skin: nsteps = 0 asks for one
+ * energy/force/virial evaluation, nsteps > 0 for that many velocity-Verlet
+ * steps of dt ps (metal units, masses from the potential file).  It returns
+ * MLIP_ERR_FULL when the capacity left cannot hold the request.
+ * mlip_pool_tick enqueues one evaluation of every resident request as one
+ * batch (and the NVE half steps) and returns without waiting; admissions may
+ * continue while it runs and join the next tick.  mlip_pool_sync waits for
+ * the tick and retires requests that finished; mlip_pool_done lists them and
+ * mlip_pool_result copies one out and forgets it.  Per evaluation the pool
+ * records 13 numbers: energy, virial[6] (xx, yy, zz, xy, xz, yz, eV) and the
+ * kinetic-energy tensor sum m v_a v_b [6] (eV).  A request's results do not
+ * depend on the other requests in its batches. */
+int64_t mlip_pool_bytes(const mlip_ctx* ctx, int64_t max_atoms, int64_t max_pairs, int max_requests);
+int mlip_pool_init(mlip_ctx* ctx, int64_t max_atoms, int64_t max_pairs, int max_requests);
+int mlip_pool_admit(mlip_ctx* ctx, int natoms, const double* cell, const double* x, const int* type, const double* v,
+                    int nsteps, double dt, double skin, int* handle);
+/* Admits requests 0..count-1 in order (FIFO) until one does not fit;
+ * structures concatenated (natoms[k] atoms each, cells 9 per request, v
+ * NULL or concatenated), neighbor lists built on nthreads host threads.
+ * Returns the number admitted (negative MLIP_ERR_* on error). */
+int mlip_pool_admit_many(mlip_ctx* ctx, int count, const int* natoms, const double* cells, const double* x,
+                         const int* type, const double* v, const int* nsteps, double dt, const double* skin,
+                         int nthreads, int* handles);
+int mlip_pool_tick(mlip_ctx* ctx, int* nbatch);
+int mlip_pool_sync(mlip_ctx* ctx, int* ncompleted);
+/* returns the number of handles written (negative MLIP_ERR_* on error) */
+int mlip_pool_done(mlip_ctx* ctx, int* handles, int cap);
+/* energy/virial of the last evaluation; f, x, v [natoms*3] at the end
+ * (nullable); thermo [13 per evaluation] (nullable, thermo_cap doubles).
+ * Returns the number of evaluations (negative MLIP_ERR_* on error). */
+int mlip_pool_result(mlip_ctx* ctx, int handle, double* energy, double* virial, double* f, double* x, double* v,
+                     double* thermo, int thermo_cap);
+/* used atoms, used pairs, atom capacity, pair capacity, requests held, requests running */
+int mlip_pool_stats(const mlip_ctx* ctx, int64_t* out6);
 
 #ifdef __cplusplus
 }

@@ -58,7 +58,8 @@ struct DevSystem {
     int max_neigh;
     const double* x;          // [nall][3]
     const int32_t* elem;      // [nall] element index
-    const int32_t* poff;      // [nown+1] directed pairs grouped by centre
+    const int32_t* poff;      // [nown] first pair of each centre
+    const int32_t* pend;      // [nown] end of each centre's pairs (poff + 1 for one CSR)
     const int32_t* pnbr;      // [npairs] neighbor position index
     const int32_t* pcen;      // [npairs] centre position index, or null (= owner)
     const double* pshift;     // [npairs][3] added to x_c - x_n, or null
@@ -75,7 +76,8 @@ struct DevSystem {
     double* Eq34;    // [slot][n_fn][kWavesPerWg]
     double* gpair;   // [npairs][3] dE_centre/d(del)
     double* vown;    // [nown][6]
-    const int32_t* goff;   // [nall+1] force gather CSR
+    const int32_t* goff;   // [nall] force gather lists: begin
+    const int32_t* gend;   // [nall] end (goff + 1 for one CSR)
     const int32_t* gent;   // signed pair entries: p+1 (neighbor side) / -(p+1) (centre side)
     double* f;       // [nall][3]
     double* e_slot;  // [nblocks*kBlock]
@@ -97,9 +99,11 @@ void launch_poly(const DevModel& m, const DevSystem& s, hipStream_t st);
 // threads per workgroup of the force kernel: 64, 128 (default), 192 or 256
 void launch_force(const DevModel& m, const DevSystem& s, int precision, int threads, hipStream_t st);
 // per-structure energy + virial (7 doubles each) of a batch; soff = atom offsets
-void launch_struct_reduce(const DevSystem& s, const int32_t* soff, int nstruct, double* out, hipStream_t st);
+void launch_struct_reduce(const DevSystem& s, const int32_t* sbeg, const int32_t* send, int nstruct, double* out,
+                          hipStream_t st);
 void launch_gather(const DevModel& m, const DevSystem& s, hipStream_t st);
-void launch_energy_virial(const DevModel& m, const DevSystem& s, hipStream_t st);
+// per-slot energies; with totals, also the fixed-order whole-system sums
+void launch_energy_virial(const DevModel& m, const DevSystem& s, bool totals, hipStream_t st);
 // plain streaming copy used to calibrate hardware byte counters
 void launch_copy(void* dst, const void* src, int64_t nbytes, int reps, hipStream_t st);
 

@@ -5,6 +5,8 @@
 #include <rocblas/rocblas.h>
 
 #include <algorithm>
+#include <atomic>
+#include <thread>
 #include <cstdio>
 #include <cstring>
 #include <numeric>
@@ -12,94 +14,13 @@
 #include <unordered_map>
 #include <vector>
 
-#include "../../include/mlip_hip.h"
-#include "device_api.h"
-#include "model.h"
+#include "engine_internal.h"
 #include "neighbor.h"
 #include "workload.h"
 
 using namespace mlip;
 
-namespace {
-
-template <typename T>
-struct DBuf {
-    T* p = nullptr;
-    size_t cap = 0;
-    bool ensure(size_t n) {
-        if (n <= cap && p) return true;
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        cap = 0;
-        if (n == 0) n = 1;
-        if (hipMalloc(&p, n * sizeof(T)) != hipSuccess) {
-            p = nullptr;
-            return false;
-        }
-        cap = n;
-        return true;
-    }
-    void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        cap = 0;
-    }
-};
-
-struct ProfEvent {
-    const char* name;
-    hipEvent_t a, b;
-};
-
-}  // namespace
-
-struct mlip_ctx {
-    Model model;
-    std::vector<int> elem_of_type;  // LAMMPS type-1 -> element (or -1)
-    int precision = kFP64;
-    std::string err;
-    hipStream_t stream = nullptr;
-    rocblas_handle blas = nullptr;
-    int device = 0;
-    // model tables
-    DBuf<double> d_coef, d_weight, d_wmat, d_linp, d_c3w;
-    DBuf<VisitQ12> d_q12;
-    DBuf<int32_t> d_q12off, d_c3off, d_c3bc;
-    DBuf<VisitQ34> d_vadj, d_vfeat;
-    DBuf<GroupQ34> d_gadj, d_gfeat;
-    DBuf<YTab<double>> d_yt64;
-    DBuf<YTab<float>> d_yt32;
-    DevModel dm{};
-    // structure buffers
-    DBuf<double> x, pshift, A, G, D, Hq, H, Ep, Eq12, Eq34, gpair, vown, f, e_slot, red, out;
-    DBuf<int32_t> elem, poff, pnbr, pcen, atom_of_slot, slot_of_atom, block_type, goff, gent;
-    DevSystem ds{};
-    bool resident = false;
-    int resident_natoms = 0;
-    // profiling: events recorded around every launch while enabled, folded
-    // into per-name totals at each synchronisation
-    bool prof = false;
-    std::vector<ProfEvent> events;
-    size_t ev_used = 0;
-    std::vector<std::string> prof_names;
-    std::vector<double> prof_ms;
-    std::vector<int> prof_count;
-    // algorithmic work of the current structure (see mlip_flops)
-    std::vector<int64_t> owners_per_type;
-    int64_t npairs = 0;
-    // launch shape chosen by mlip_autotune
-    int force_threads = 128;
-    // batch of independent structures (0 = single structure)
-    int nstruct = 0;
-    DBuf<int32_t> soff;
-    DBuf<double> sout;
-    // one evaluation captured as a HIP graph and replayed
-    bool use_graph = false;
-    bool graph_warm = false;  // one eager evaluation ran since the last change
-    hipGraphExec_t gexec = nullptr;
-};
-
-namespace {
+namespace mlip {
 
 bool hip_ok(mlip_ctx* c, hipError_t e, const char* what) {
     if (e == hipSuccess) return true;
@@ -107,15 +28,9 @@ bool hip_ok(mlip_ctx* c, hipError_t e, const char* what) {
     return false;
 }
 
-template <typename T>
-bool upload(mlip_ctx* c, DBuf<T>& b, const T* h, size_t n, const char* what) {
-    if (!b.ensure(n)) {
-        c->err = std::string("device allocation failed for ") + what;
-        return false;
-    }
-    if (n == 0) return true;
-    return hip_ok(c, hipMemcpyAsync(b.p, h, n * sizeof(T), hipMemcpyHostToDevice, c->stream), what);
-}
+}  // namespace mlip
+
+namespace {
 
 void set_err(char* buf, size_t len, const std::string& msg) {
     if (buf && len) std::snprintf(buf, len, "%s", msg.c_str());
@@ -226,8 +141,6 @@ bool upload_model(mlip_ctx* c) {
     return hip_ok(c, hipStreamSynchronize(c->stream), "model upload");
 }
 
-void drop_graph(mlip_ctx* c);
-
 // Slot assignment: centres sorted by element (stable), each element padded
 // to whole blocks of kBlock lanes.
 void assign_slots(const mlip_ctx* c, int nown, const std::vector<int>& elem_of_owner,
@@ -262,6 +175,9 @@ bool setup_system(mlip_ctx* c, int nall, int nown, int64_t npairs, const std::ve
                   const std::vector<int32_t>& gent) {
     const Model& m = c->model;
     drop_graph(c);
+    c->nstruct = 0;
+    c->sres = nullptr;
+    c->total_reduce = true;
     std::vector<int32_t> atom_of_slot, slot_of_atom, block_type;
     assign_slots(c, nown, elem_own, atom_of_slot, slot_of_atom, block_type);
     const int nblocks = static_cast<int>(block_type.size());
@@ -304,6 +220,7 @@ bool setup_system(mlip_ctx* c, int nall, int nown, int64_t npairs, const std::ve
     s.x = c->x.p;
     s.elem = c->elem.p;
     s.poff = c->poff.p;
+    s.pend = c->poff.p + 1;
     s.pnbr = c->pnbr.p;
     s.pcen = pcen ? c->pcen.p : nullptr;
     s.pshift = pshift ? c->pshift.p : nullptr;
@@ -321,6 +238,7 @@ bool setup_system(mlip_ctx* c, int nall, int nown, int64_t npairs, const std::ve
     s.gpair = c->gpair.p;
     s.vown = c->vown.p;
     s.goff = c->goff.p;
+    s.gend = c->goff.p + 1;
     s.gent = c->gent.p;
     s.f = c->f.p;
     s.e_slot = c->e_slot.p;
@@ -328,6 +246,10 @@ bool setup_system(mlip_ctx* c, int nall, int nown, int64_t npairs, const std::ve
     s.out = c->out.p;
     return true;
 }
+
+}  // namespace
+
+namespace mlip {
 
 // Force gather lists: for every position k, the pairs whose centre (-) or
 // neighbor (+) is k, in increasing pair order.
@@ -353,6 +275,10 @@ void build_gather(int nall, const std::vector<int32_t>& poff, const std::vector<
     }
 }
 
+}  // namespace mlip
+
+namespace {
+
 void prof_mark(mlip_ctx* c, const char* name, bool start) {
     if (!c->prof) return;
     if (start) {
@@ -375,6 +301,10 @@ struct Scope {
     Scope(mlip_ctx* cc, const char* n) : c(cc) { prof_mark(c, n, true); }
     ~Scope() { prof_mark(c, nullptr, false); }
 };
+
+}  // namespace
+
+namespace mlip {
 
 bool run_pipeline(mlip_ctx* c) {
     const DevModel& m = c->dm;
@@ -402,10 +332,10 @@ bool run_pipeline(mlip_ctx* c) {
     if (mm.has_order34) { Scope sc(c, "q34_adj"); launch_q34_adj(m, s, st); }
     { Scope sc(c, "force"); launch_force(m, s, c->precision, c->force_threads, st); }
     { Scope sc(c, "gather"); launch_gather(m, s, st); }
-    { Scope sc(c, "energy_virial"); launch_energy_virial(m, s, st); }
+    { Scope sc(c, "energy_virial"); launch_energy_virial(m, s, c->total_reduce, st); }
     if (c->nstruct > 0) {
         Scope sc(c, "struct_reduce");
-        launch_struct_reduce(s, c->soff.p, c->nstruct, c->sout.p, st);
+        launch_struct_reduce(s, c->sbeg, c->send, c->nstruct, c->sres, st);
     }
     return hip_ok(c, hipGetLastError(), "kernel launch");
 }
@@ -463,6 +393,10 @@ void collect_profile(mlip_ctx* c) {
     }
     c->ev_used = 0;
 }
+
+}  // namespace mlip
+
+namespace {
 
 int lammps_compute(mlip_ctx* c, int nlocal, int nghost, const double* x, const int* type, const int64_t* tag,
                    int inum, const int* ilist, const int* numneigh, const int* const* firstneigh,
@@ -661,6 +595,7 @@ void mlip_destroy(mlip_ctx* c) {
     c->d_yt64.release();
     c->d_yt32.release();
     drop_graph(c);
+    pool_free(c);
     c->soff.release();
     c->sout.release();
     if (c->blas) rocblas_destroy_handle(c->blas);
@@ -815,13 +750,29 @@ int mlip_batch_set(mlip_ctx* c, int nstruct, const int* natoms, const double* ce
         }
         elem[i] = c->elem_of_type[t - 1];
     }
+    // per-structure neighbor lists on host threads, concatenated in order
+    std::vector<NeighborList> lists(nstruct);
+    std::vector<std::string> errs(nstruct);
+    std::vector<char> bad(nstruct, 0);
+    {
+        std::atomic<int> next{0};
+        auto work = [&]() {
+            for (int k = next++; k < nstruct; k = next++)
+                bad[k] = !build_full_list(cells + 9 * k, natoms[k], x + 3 * static_cast<size_t>(soff[k]),
+                                          c->model.cutoff, lists[k], errs[k]);
+        };
+        const int nt = std::max(1, std::min<int>(nstruct, std::min(16u, std::max(1u, std::thread::hardware_concurrency()))));
+        std::vector<std::thread> th;
+        for (int t = 1; t < nt; ++t) th.emplace_back(work);
+        work();
+        for (auto& t : th) t.join();
+    }
     for (int k = 0; k < nstruct; ++k) {
-        NeighborList nl;
-        std::string e;
-        if (!build_full_list(cells + 9 * k, natoms[k], x + 3 * static_cast<size_t>(soff[k]), c->model.cutoff, nl, e)) {
-            c->err = "structure " + std::to_string(k) + ": " + e;
+        if (bad[k]) {
+            c->err = "structure " + std::to_string(k) + ": " + errs[k];
             return MLIP_ERR_ARG;
         }
+        NeighborList nl = std::move(lists[k]);
         for (int i = 0; i < natoms[k]; ++i) poff[soff[k] + i + 1] = static_cast<int32_t>(pnbr.size() + nl.off[i + 1]);
         for (int32_t j : nl.idx) pnbr.push_back(soff[k] + j);
         sh.insert(sh.end(), nl.shift.begin(), nl.shift.end());
@@ -840,6 +791,9 @@ int mlip_batch_set(mlip_ctx* c, int nstruct, const int* natoms, const double* ce
         return MLIP_ERR_DEVICE;
     if (!hip_ok(c, hipStreamSynchronize(c->stream), "batch upload")) return MLIP_ERR_DEVICE;
     c->nstruct = nstruct;
+    c->sbeg = c->soff.p;
+    c->send = c->soff.p + 1;
+    c->sres = c->sout.p;
     c->resident = true;
     c->resident_natoms = n;
     return MLIP_OK;

@@ -1,0 +1,127 @@
+// Internal types of the host engine shared by its translation units
+// (engine.hip, pool.hip): device buffers, the context behind the C ABI and
+// the pipeline entry points.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/mlip_hip.h"
+#include "device_api.h"
+#include "model.h"
+
+namespace mlip {
+
+template <typename T>
+struct DBuf {
+    T* p = nullptr;
+    size_t cap = 0;
+    bool ensure(size_t n) {
+        if (n <= cap && p) return true;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+        if (n == 0) n = 1;
+        if (hipMalloc(&p, n * sizeof(T)) != hipSuccess) {
+            p = nullptr;
+            return false;
+        }
+        cap = n;
+        return true;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
+struct ProfEvent {
+    const char* name;
+    hipEvent_t a, b;
+};
+
+struct Pool;  // device-resident request pool (pool.hip)
+
+}  // namespace mlip
+
+struct mlip_ctx {
+    mlip::Model model;
+    std::vector<int> elem_of_type;  // LAMMPS type-1 -> element (or -1)
+    int precision = mlip::kFP64;
+    std::string err;
+    hipStream_t stream = nullptr;
+    rocblas_handle blas = nullptr;
+    int device = 0;
+    // model tables
+    mlip::DBuf<double> d_coef, d_weight, d_wmat, d_linp, d_c3w;
+    mlip::DBuf<mlip::VisitQ12> d_q12;
+    mlip::DBuf<int32_t> d_q12off, d_c3off, d_c3bc;
+    mlip::DBuf<mlip::VisitQ34> d_vadj, d_vfeat;
+    mlip::DBuf<mlip::GroupQ34> d_gadj, d_gfeat;
+    mlip::DBuf<mlip::YTab<double>> d_yt64;
+    mlip::DBuf<mlip::YTab<float>> d_yt32;
+    mlip::DevModel dm{};
+    // structure buffers
+    mlip::DBuf<double> x, pshift, A, G, D, Hq, H, Ep, Eq12, Eq34, gpair, vown, f, e_slot, red, out;
+    mlip::DBuf<int32_t> elem, poff, pnbr, pcen, atom_of_slot, slot_of_atom, block_type, goff, gent;
+    mlip::DevSystem ds{};
+    bool resident = false;
+    int resident_natoms = 0;
+    // profiling: events recorded around every launch while enabled, folded
+    // into per-name totals at each synchronisation
+    bool prof = false;
+    std::vector<mlip::ProfEvent> events;
+    size_t ev_used = 0;
+    std::vector<std::string> prof_names;
+    std::vector<double> prof_ms;
+    std::vector<int> prof_count;
+    // algorithmic work of the current structure (see mlip_flops)
+    std::vector<int64_t> owners_per_type;
+    int64_t npairs = 0;
+    // launch shape chosen by mlip_autotune
+    int force_threads = 128;
+    // batch of independent structures (0 = single structure): per-structure
+    // atom ranges [sbeg, send) and 7 results each
+    int nstruct = 0;
+    mlip::DBuf<int32_t> soff;
+    const int32_t* sbeg = nullptr;
+    const int32_t* send = nullptr;
+    mlip::DBuf<double> sout;
+    double* sres = nullptr;  // where the per-structure results go
+    // whole-system energy/virial reductions (off for the request pool)
+    bool total_reduce = true;
+    // one evaluation captured as a HIP graph and replayed
+    bool use_graph = false;
+    bool graph_warm = false;  // one eager evaluation ran since the last change
+    hipGraphExec_t gexec = nullptr;
+    // device-resident request pool, when initialised
+    mlip::Pool* pool = nullptr;
+};
+
+namespace mlip {
+
+bool hip_ok(mlip_ctx* c, hipError_t e, const char* what);
+
+template <typename T>
+bool upload(mlip_ctx* c, DBuf<T>& b, const T* h, size_t n, const char* what) {
+    if (!b.ensure(n)) {
+        c->err = std::string("device allocation failed for ") + what;
+        return false;
+    }
+    if (n == 0) return true;
+    return hip_ok(c, hipMemcpyAsync(b.p, h, n * sizeof(T), hipMemcpyHostToDevice, c->stream), what);
+}
+
+void drop_graph(mlip_ctx* c);
+bool run_pipeline(mlip_ctx* c);
+bool run_eval(mlip_ctx* c);
+void collect_profile(mlip_ctx* c);
+void build_gather(int nall, const std::vector<int32_t>& poff, const std::vector<int32_t>& pnbr,
+                  const std::vector<int32_t>* pcen, std::vector<int32_t>& goff, std::vector<int32_t>& gent);
+void pool_free(mlip_ctx* c);
+
+}  // namespace mlip

@@ -180,7 +180,7 @@ __global__ __launch_bounds__(256) void k_gather(DevSystem S) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= S.nall) return;
     double fx = 0.0, fy = 0.0, fz = 0.0;
-    for (int e = S.goff[k]; e < S.goff[k + 1]; ++e) {
+    for (int e = S.goff[k]; e < S.gend[k]; ++e) {
         const int code = S.gent[e];
         const int64_t p = code > 0 ? code - 1 : -code - 1;
         const double sg = code > 0 ? 1.0 : -1.0;
@@ -211,10 +211,11 @@ __global__ __launch_bounds__(256) void k_energy_slot(DevModel M, DevSystem S) {
 
 // Per-structure energy and virial of a batch: one workgroup per structure,
 // fixed-order tree over its (contiguous) atoms.
-__global__ __launch_bounds__(256) void k_struct_reduce(DevSystem S, const int32_t* soff, double* out) {
+__global__ __launch_bounds__(256) void k_struct_reduce(DevSystem S, const int32_t* sbeg, const int32_t* send,
+                                                       double* out) {
     __shared__ double sh[256];
     const int st = blockIdx.x;
-    const int a0 = soff[st], a1 = soff[st + 1];
+    const int a0 = sbeg[st], a1 = send[st];
     for (int c = 0; c < 7; ++c) {
         double v = 0.0;
         for (int i = a0 + threadIdx.x; i < a1; i += 256)
@@ -313,15 +314,20 @@ void launch_copy(void* dst, const void* src, int64_t nbytes, int reps, hipStream
     check(hipGetLastError(), "k_copy launch");
 }
 
-void launch_struct_reduce(const DevSystem& s, const int32_t* soff, int nstruct, double* out, hipStream_t st) {
+void launch_struct_reduce(const DevSystem& s, const int32_t* sbeg, const int32_t* send, int nstruct, double* out,
+                          hipStream_t st) {
     if (nstruct <= 0) return;
-    hipLaunchKernelGGL(k_struct_reduce, dim3(nstruct), dim3(256), 0, st, s, soff, out);
+    hipLaunchKernelGGL(k_struct_reduce, dim3(nstruct), dim3(256), 0, st, s, sbeg, send, out);
     check(hipGetLastError(), "k_struct_reduce launch");
 }
 
-void launch_energy_virial(const DevModel& m, const DevSystem& s, hipStream_t st) {
+void launch_energy_virial(const DevModel& m, const DevSystem& s, bool totals, hipStream_t st) {
     const int nslot = s.nblocks * kBlock;
     if (nslot > 0) hipLaunchKernelGGL(k_energy_slot, dim3((nslot + 255) / 256), dim3(256), 0, st, m, s);
+    if (!totals) {
+        check(hipGetLastError(), "energy launch");
+        return;
+    }
     const int parts = 120;
     hipLaunchKernelGGL(k_reduce1, dim3(parts), dim3(kRedThreads), 0, st, s.e_slot, nslot, 1, 1, s.red);
     hipLaunchKernelGGL(k_reduce2, dim3(1), dim3(kRedThreads), 0, st, s.red, parts, 1, s.out);

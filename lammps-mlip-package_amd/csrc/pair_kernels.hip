@@ -126,7 +126,7 @@ __global__ __launch_bounds__(kAnlmThreads) void k_anlm(DevModel M, DevSystem S, 
             L.mu[tid] = tid < nfn ? M.mu[tid] : 0.0;
         }
     }
-    const int p0 = S.poff[atom], p1 = S.poff[atom + 1];
+    const int p0 = S.poff[atom], p1 = S.pend[atom];
     // phase-2 role
     const int lm2 = tid % nlmh, grp = tid / nlmh;
     const bool act2 = grp < ngroup;
@@ -295,7 +295,7 @@ __global__ __launch_bounds__(NT) void k_force(DevModel M, DevSystem S) {
         }
     }
     __syncthreads();
-    const int p0 = S.poff[atom], p1 = S.poff[atom + 1];
+    const int p0 = S.poff[atom], p1 = S.pend[atom];
     const double pir = kPiRef / M.cutoff;
     double v0 = 0, v1 = 0, v2 = 0, v3 = 0, v4 = 0, v5 = 0;
     for (int64_t p = p0 + threadIdx.x; p < p1; p += NT) {
@@ -326,8 +326,9 @@ __global__ __launch_bounds__(NT) void k_force(DevModel M, DevSystem S) {
                 // exponentials costs more registers than it hides latency
                 __builtin_amdgcn_sched_barrier(0);
             }
-            T er = T(0);                      // sum w Re(Gp Y)
-            T tx = T(0), ty = T(0), tz = T(0);  // sum w Re(Gq grad Y) * r
+            // sums over lm in FP64 in both modes (terms in T)
+            double er = 0.0;                        // sum w Re(Gp Y)
+            double tx = 0.0, ty = 0.0, tz = 0.0;    // sum w Re(Gq grad Y) * r
             const T zz = T(1) - z * z;
             T cr = T(1), ci = T(0);  // conj(u)^m
             T mr = T(0), mi = T(0);  // conj(u)^(m-1)
@@ -365,7 +366,7 @@ __global__ __launch_bounds__(NT) void k_force(DevModel M, DevSystem S) {
                     b2 = b1;
                     b1 = qp;
                     // Y_{l,-m} = sg q conj(u)^m
-                    er = fma(w * sg * q, gpr * cr - gpi * ci, er);
+                    er += static_cast<double>(w * sg * q * (gpr * cr - gpi * ci));
                     T gxr, gxi, gyr, gyi, gzr, gzi;
                     if (m == 0) {
                         const T c = C.grad[l][0] * qp;
@@ -388,9 +389,9 @@ __global__ __launch_bounds__(NT) void k_force(DevModel M, DevSystem S) {
                         gzr = -A * cr;
                         gzi = -A * ci;
                     }
-                    tx = fma(ws, gqr * gxr - gqi * gxi, tx);
-                    ty = fma(ws, gqr * gyr - gqi * gyi, ty);
-                    tz = fma(ws, gqr * gzr - gqi * gzi, tz);
+                    tx += static_cast<double>(ws * (gqr * gxr - gqi * gxi));
+                    ty += static_cast<double>(ws * (gqr * gyr - gqi * gyi));
+                    tz += static_cast<double>(ws * (gqr * gzr - gqi * gzi));
                     lm += l + 2;  // lmh(l+1, -m) = lmh(l, -m) + l + 2
                 }
                 mr = cr;
@@ -400,9 +401,9 @@ __global__ __launch_bounds__(NT) void k_force(DevModel M, DevSystem S) {
             }
             // dE/d(del) = er * rhat + t / r
             const double ir = ird;
-            fx = static_cast<double>(er) * (pg.dx * ir) + static_cast<double>(tx) * ir;
-            fy = static_cast<double>(er) * (pg.dy * ir) + static_cast<double>(ty) * ir;
-            fz = static_cast<double>(er) * (pg.dz * ir) + static_cast<double>(tz) * ir;
+            fx = er * (pg.dx * ir) + tx * ir;
+            fy = er * (pg.dy * ir) + ty * ir;
+            fz = er * (pg.dz * ir) + tz * ir;
         }
         S.gpair[3 * p] = fx;
         S.gpair[3 * p + 1] = fy;

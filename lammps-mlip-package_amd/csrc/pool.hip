@@ -1,0 +1,670 @@
+// Device-resident request pool: continuous batching of independent
+// structures (single energy/force/stress evaluations and short NVE runs).
+//
+// Every admitted request owns an atom range and a pair range of the pool's
+// device arrays (positions, velocities, element, per-centre pair ranges with
+// image shifts, force-gather lists), filled once at admission from a native
+// neighbor list built with a skin.  A tick evaluates all resident requests
+// as one batch: the slot blocks of the pipeline mix atoms of different
+// requests (same species per block), pairs and gather entries stay in each
+// request's own ranges, and per-request energy, virial and kinetic-energy
+// tensor come from fixed-order reductions over the request's atoms, so a
+// request's numbers do not depend on what else is in the batch.  NVE
+// requests advance one velocity-Verlet step per tick (LAMMPS fix nve:
+// half kick and drift before the force evaluation, half kick after) in
+// metal units.  Admission is by capacity: a request that does not fit the
+// pool's atom or pair budget is refused with MLIP_ERR_FULL and can be
+// offered again after completions free space.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <atomic>
+#include <map>
+#include <memory>
+#include <thread>
+#include <string>
+#include <vector>
+
+#include "engine_internal.h"
+#include "neighbor.h"
+
+namespace mlip {
+
+namespace {
+
+// LAMMPS metal units (force.cpp / update.cpp conventions)
+constexpr double kMvv2e = 1.0364269e-4;        // g/mol A^2/ps^2 -> eV
+constexpr double kFtm2v = 1.0 / 1.0364269e-4;  // eV/A / (g/mol) -> A/ps^2
+
+struct NveDesc {
+    int32_t a0, n;
+    double dtf, dtv;  // 0.5 dt ftm2v, dt
+};
+
+__global__ void k_nve_first(const NveDesc* d, const int32_t* elem, const double* mass, const double* f, double* v,
+                            double* x) {
+    const NveDesc q = d[blockIdx.y];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= q.n) return;
+    const int a = q.a0 + i;
+    const double dtfm = q.dtf / mass[elem[a]];
+    for (int c = 0; c < 3; ++c) {
+        v[3 * a + c] += dtfm * f[3 * a + c];
+        x[3 * a + c] += q.dtv * v[3 * a + c];
+    }
+}
+
+__global__ void k_nve_final(const NveDesc* d, const int32_t* elem, const double* mass, const double* f, double* v) {
+    const NveDesc q = d[blockIdx.y];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= q.n) return;
+    const int a = q.a0 + i;
+    const double dtfm = q.dtf / mass[elem[a]];
+    for (int c = 0; c < 3; ++c) v[3 * a + c] += dtfm * f[3 * a + c];
+}
+
+// kinetic-energy tensor m v_a v_b (xx, yy, zz, xy, xz, yz) of each request,
+// one workgroup per request, fixed-order tree
+__global__ __launch_bounds__(256) void k_ke(const int32_t* sbeg, const int32_t* send, const int32_t* elem,
+                                            const double* mass, const double* v, double* out) {
+    __shared__ double sh[256];
+    const int r = blockIdx.x;
+    const int a0 = sbeg[r], a1 = send[r];
+    for (int c = 0; c < 6; ++c) {
+        const int p = c < 3 ? c : (c == 5 ? 1 : 0);
+        const int q = c < 3 ? c : (c == 3 ? 1 : 2);
+        double s = 0.0;
+        for (int a = a0 + threadIdx.x; a < a1; a += 256) s += mass[elem[a]] * v[3 * a + p] * v[3 * a + q];
+        sh[threadIdx.x] = s;
+        __syncthreads();
+        for (int w = 128; w > 0; w >>= 1) {
+            if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) out[6 * r + c] = kMvv2e * sh[0];
+        __syncthreads();
+    }
+}
+
+__global__ void k_scatter_slots(const int32_t* atom_of_slot, int nslot, int32_t* slot_of_atom) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nslot) return;
+    const int a = atom_of_slot[s];
+    if (a >= 0) slot_of_atom[a] = s;
+}
+
+// First-fit allocator of index ranges with coalescing of freed ranges.
+class RangeAlloc {
+  public:
+    void init(int64_t cap) {
+        cap_ = cap;
+        used_ = 0;
+        free_.clear();
+        if (cap > 0) free_[0] = cap;
+    }
+    int64_t take(int64_t n) {
+        if (n <= 0) return 0;
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second < n) continue;
+            const int64_t off = it->first, rem = it->second - n;
+            free_.erase(it);
+            if (rem > 0) free_[off + n] = rem;
+            used_ += n;
+            return off;
+        }
+        return -1;
+    }
+    void give(int64_t off, int64_t n) {
+        if (n <= 0) return;
+        used_ -= n;
+        auto it = free_.emplace(off, n).first;
+        auto nx = std::next(it);
+        if (nx != free_.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            free_.erase(nx);
+        }
+        if (it != free_.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                free_.erase(it);
+            }
+        }
+    }
+    int64_t used() const { return used_; }
+    int64_t cap() const { return cap_; }
+    // end of the highest allocated range
+    int64_t high() const {
+        if (free_.empty()) return cap_;
+        const auto& last = *free_.rbegin();
+        return last.first + last.second == cap_ ? last.first : cap_;
+    }
+
+  private:
+    int64_t cap_ = 0, used_ = 0;
+    std::map<int64_t, int64_t> free_;
+};
+
+struct Request {
+    int id = 0;
+    int natoms = 0;
+    int64_t a0 = 0, p0 = 0, np = 0;
+    int nsteps = 0, step = 0;
+    double dt = 0.0;
+    std::vector<int32_t> elem;
+    bool done = false;
+    // per evaluation: energy, virial[6], kinetic-energy tensor[6]
+    std::vector<double> thermo;
+    std::vector<double> f, x, v;
+};
+
+constexpr int kThermo = 13;
+
+}  // namespace
+
+struct Pool {
+    int64_t cap_atoms = 0, cap_pairs = 0, cap_slots = 0;
+    int cap_req = 0;
+    RangeAlloc atoms, pairs;
+    DBuf<double> x, v, f, pshift, A, G, D, Hq, H, Ep, Eq12, Eq34, gpair, vown, e_slot, red, out, sout, ke, mass;
+    DBuf<int32_t> elem, pbeg, pend, pnbr, gbeg, gend, gent, atom_of_slot, slot_of_atom, block_type, sbeg, send;
+    DBuf<NveDesc> first, final_;
+    std::map<int, Request> req;
+    int next_id = 1;
+    std::vector<int> done_ids;
+    // the tick in flight: requests in batch order and host staging
+    bool in_flight = false;
+    std::vector<int> batch;
+    double* h_out = nullptr;  // pinned: 7 per request, then 6 kinetic per request
+    std::vector<std::vector<char>> staging;
+};
+
+void pool_free(mlip_ctx* c) {
+    Pool* p = c->pool;
+    if (!p) return;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    DBuf<double>* dbs[] = {&p->x,  &p->v,      &p->f,    &p->pshift, &p->A,   &p->G,   &p->D,
+                           &p->Hq, &p->H,      &p->Ep,   &p->Eq12,   &p->Eq34, &p->gpair, &p->vown,
+                           &p->e_slot, &p->red, &p->out, &p->sout,   &p->ke,  &p->mass};
+    for (auto* b : dbs) b->release();
+    DBuf<int32_t>* ibs[] = {&p->elem, &p->pbeg, &p->pend, &p->pnbr, &p->gbeg, &p->gend, &p->gent,
+                            &p->atom_of_slot, &p->slot_of_atom, &p->block_type, &p->sbeg, &p->send};
+    for (auto* b : ibs) b->release();
+    p->first.release();
+    p->final_.release();
+    if (p->h_out) (void)hipHostFree(p->h_out);
+    delete p;
+    c->pool = nullptr;
+}
+
+namespace {
+
+template <typename T>
+bool stage_upload(mlip_ctx* c, Pool* p, T* dst, const T* src, size_t n, const char* what) {
+    if (n == 0) return true;
+    std::vector<char> buf(reinterpret_cast<const char*>(src), reinterpret_cast<const char*>(src) + n * sizeof(T));
+    const bool ok = hip_ok(c, hipMemcpyAsync(dst, buf.data(), n * sizeof(T), hipMemcpyHostToDevice, c->stream), what);
+    p->staging.push_back(std::move(buf));
+    return ok;
+}
+
+// point the pipeline's system view at the pool arrays for one tick
+void bind_pool(mlip_ctx* c, Pool* p, int nblocks) {
+    DevSystem& s = c->ds;
+    std::memset(&s, 0, sizeof s);
+    s.nall = static_cast<int>(p->atoms.high());
+    s.nown = s.nall;
+    s.nblocks = nblocks;
+    s.npairs = p->pairs.high();
+    s.x = p->x.p;
+    s.elem = p->elem.p;
+    s.poff = p->pbeg.p;
+    s.pend = p->pend.p;
+    s.pnbr = p->pnbr.p;
+    s.pcen = nullptr;
+    s.pshift = p->pshift.p;
+    s.atom_of_slot = p->atom_of_slot.p;
+    s.slot_of_atom = p->slot_of_atom.p;
+    s.block_type = p->block_type.p;
+    s.A = p->A.p;
+    s.G = p->G.p;
+    s.D = p->D.p;
+    s.Hq = p->Hq.p;
+    s.H = p->H.p;
+    s.Ep = p->Ep.p;
+    s.Eq12 = p->Eq12.p;
+    s.Eq34 = p->Eq34.p;
+    s.gpair = p->gpair.p;
+    s.vown = p->vown.p;
+    s.goff = p->gbeg.p;
+    s.gend = p->gend.p;
+    s.gent = p->gent.p;
+    s.f = p->f.p;
+    s.e_slot = p->e_slot.p;
+    s.red = p->red.p;
+    s.out = p->out.p;
+    c->npairs = s.npairs;
+}
+
+}  // namespace
+
+}  // namespace mlip
+
+using namespace mlip;
+
+extern "C" {
+
+int mlip_pool_init(mlip_ctx* c, int64_t max_atoms, int64_t max_pairs, int max_requests) {
+    if (!c || max_atoms < 1 || max_pairs < 1 || max_requests < 1 || max_atoms > INT32_MAX / 4 ||
+        max_pairs > INT32_MAX / 4)
+        return MLIP_ERR_ARG;
+    pool_free(c);
+    drop_graph(c);
+    auto* p = new Pool();
+    c->pool = p;
+    const Model& m = c->model;
+    p->cap_atoms = max_atoms;
+    p->cap_pairs = max_pairs;
+    p->cap_req = max_requests;
+    // every species pads its last slot block
+    p->cap_slots = (max_atoms + static_cast<int64_t>(m.n_type) * kBlock + kBlock - 1) / kBlock * kBlock;
+    p->atoms.init(max_atoms);
+    p->pairs.init(max_pairs);
+    const size_t na = static_cast<size_t>(max_atoms), npr = static_cast<size_t>(max_pairs);
+    const size_t ns = static_cast<size_t>(p->cap_slots), nr = static_cast<size_t>(max_requests);
+    const size_t asz = ns * m.n_fn * m.n_type * m.n_lmh * 2, np = static_cast<size_t>(m.n_p);
+    const bool ok = p->x.ensure(3 * na) && p->v.ensure(3 * na) && p->f.ensure(3 * na) && p->elem.ensure(na) &&
+                    p->pbeg.ensure(na) && p->pend.ensure(na) && p->gbeg.ensure(na) && p->gend.ensure(na) &&
+                    p->slot_of_atom.ensure(na) && p->vown.ensure(6 * na) && p->pnbr.ensure(npr) &&
+                    p->pshift.ensure(3 * npr) && p->gpair.ensure(3 * npr) && p->gent.ensure(2 * npr) &&
+                    p->A.ensure(asz) && p->G.ensure(asz) && p->D.ensure(ns * np) && p->Hq.ensure(ns * np) &&
+                    p->H.ensure(ns * np) && p->Ep.ensure(ns) && p->Eq12.ensure(ns * m.n_fn) &&
+                    p->Eq34.ensure(ns * m.n_fn * kWavesPerWg) && p->e_slot.ensure(ns) &&
+                    p->atom_of_slot.ensure(ns) && p->block_type.ensure(ns / kBlock + 1) && p->red.ensure(7 * 128) &&
+                    p->out.ensure(8) && p->sout.ensure(7 * nr) && p->ke.ensure(6 * nr) && p->sbeg.ensure(nr) &&
+                    p->send.ensure(nr) && p->first.ensure(nr) && p->final_.ensure(nr) &&
+                    p->mass.ensure(static_cast<size_t>(m.n_type));
+    if (!ok || hipHostMalloc(reinterpret_cast<void**>(&p->h_out), 13 * nr * sizeof(double)) != hipSuccess) {
+        c->err = "device allocation failed for the request pool";
+        pool_free(c);
+        return MLIP_ERR_DEVICE;
+    }
+    // empty ranges everywhere; a_nlm of padding lanes starts finite
+    std::vector<double> mass(m.masses.begin(), m.masses.end());
+    mass.resize(m.n_type, 1.0);
+    if (!hip_ok(c, hipMemsetAsync(p->pbeg.p, 0, na * 4, c->stream), "pool clear") ||
+        !hip_ok(c, hipMemsetAsync(p->pend.p, 0, na * 4, c->stream), "pool clear") ||
+        !hip_ok(c, hipMemsetAsync(p->gbeg.p, 0, na * 4, c->stream), "pool clear") ||
+        !hip_ok(c, hipMemsetAsync(p->gend.p, 0, na * 4, c->stream), "pool clear") ||
+        !hip_ok(c, hipMemsetAsync(p->elem.p, 0, na * 4, c->stream), "pool clear") ||
+        !hip_ok(c, hipMemsetAsync(p->x.p, 0, 3 * na * 8, c->stream), "pool clear") ||
+        !hip_ok(c, hipMemsetAsync(p->A.p, 0, asz * 8, c->stream), "pool clear") ||
+        !hip_ok(c, hipMemcpyAsync(p->mass.p, mass.data(), mass.size() * 8, hipMemcpyHostToDevice, c->stream),
+                "pool masses") ||
+        !hip_ok(c, hipStreamSynchronize(c->stream), "pool init")) {
+        pool_free(c);
+        return MLIP_ERR_DEVICE;
+    }
+    c->resident = false;
+    return MLIP_OK;
+}
+
+}  // extern "C"
+
+namespace mlip {
+namespace {
+
+// Host half of an admission that needs no pool state (runs on worker
+// threads): element mapping and the neighbor list at cutoff + skin.
+struct Prepared {
+    int rc = MLIP_OK;
+    std::string err;
+    std::vector<int32_t> elem;
+    NeighborList nl;
+};
+
+void prepare(const mlip_ctx* c, int natoms, const double* cell, const double* x, const int* type, double skin,
+             Prepared& out) {
+    out.elem.resize(natoms);
+    for (int i = 0; i < natoms; ++i) {
+        const int t = type[i];
+        if (t < 1 || t > static_cast<int>(c->elem_of_type.size()) || c->elem_of_type[t - 1] < 0) {
+            out.rc = MLIP_ERR_ARG;
+            out.err = "atom type without an element mapping";
+            return;
+        }
+        out.elem[i] = c->elem_of_type[t - 1];
+    }
+    std::string e;
+    if (!build_full_list(cell, natoms, x, c->model.cutoff + skin, out.nl, e)) {
+        out.rc = MLIP_ERR_ARG;
+        out.err = "request structure: " + e;
+    }
+}
+
+// Pool half: ranges and uploads.  MLIP_ERR_FULL leaves the pool unchanged.
+int commit(mlip_ctx* c, Pool* p, Prepared& pr, int natoms, const double* x, const double* v, int nsteps, double dt,
+           int* handle) {
+    if (static_cast<int>(p->req.size()) >= p->cap_req) {
+        c->err = "request pool: all request slots in use";
+        return MLIP_ERR_FULL;
+    }
+    Request r;
+    r.natoms = natoms;
+    r.nsteps = nsteps;
+    r.dt = dt;
+    r.elem = std::move(pr.elem);
+    const NeighborList& nl = pr.nl;
+    r.np = static_cast<int64_t>(nl.idx.size());
+    r.a0 = p->atoms.take(natoms);
+    if (r.a0 < 0) {
+        pr.elem = std::move(r.elem);
+        c->err = "request pool: atom budget exhausted";
+        return MLIP_ERR_FULL;
+    }
+    r.p0 = p->pairs.take(std::max<int64_t>(r.np, 1));
+    if (r.p0 < 0) {
+        p->atoms.give(r.a0, natoms);
+        pr.elem = std::move(r.elem);
+        c->err = "request pool: pair budget exhausted";
+        return MLIP_ERR_FULL;
+    }
+    // per-centre pair ranges, neighbors and gather lists in pool indices
+    std::vector<int32_t> pbeg(natoms), pend(natoms), pnbr(r.np), gbeg(natoms), gend(natoms);
+    std::vector<int32_t> loff(natoms + 1), lnbr(nl.idx.begin(), nl.idx.end());
+    for (int i = 0; i <= natoms; ++i) loff[i] = static_cast<int32_t>(nl.off[i]);
+    for (int i = 0; i < natoms; ++i) {
+        pbeg[i] = static_cast<int32_t>(r.p0 + loff[i]);
+        pend[i] = static_cast<int32_t>(r.p0 + loff[i + 1]);
+    }
+    for (int64_t k = 0; k < r.np; ++k) pnbr[k] = static_cast<int32_t>(r.a0 + lnbr[k]);
+    std::vector<int32_t> goff, gent;
+    build_gather(natoms, loff, lnbr, nullptr, goff, gent);
+    const int64_t g0 = 2 * r.p0;
+    for (auto& en : gent) en = en > 0 ? static_cast<int32_t>(en + r.p0) : static_cast<int32_t>(en - r.p0);
+    for (int i = 0; i < natoms; ++i) {
+        gbeg[i] = static_cast<int32_t>(g0 + goff[i]);
+        gend[i] = static_cast<int32_t>(g0 + goff[i + 1]);
+    }
+    std::vector<double> vz;
+    if (!v) vz.assign(3 * static_cast<size_t>(natoms), 0.0);
+    const size_t a0 = static_cast<size_t>(r.a0), p0 = static_cast<size_t>(r.p0);
+    const bool ok = stage_upload(c, p, p->x.p + 3 * a0, x, 3 * static_cast<size_t>(natoms), "pool positions") &&
+                    stage_upload(c, p, p->v.p + 3 * a0, v ? v : vz.data(), 3 * static_cast<size_t>(natoms),
+                                 "pool velocities") &&
+                    stage_upload(c, p, p->elem.p + a0, r.elem.data(), natoms, "pool elements") &&
+                    stage_upload(c, p, p->pbeg.p + a0, pbeg.data(), natoms, "pool pair ranges") &&
+                    stage_upload(c, p, p->pend.p + a0, pend.data(), natoms, "pool pair ranges") &&
+                    stage_upload(c, p, p->gbeg.p + a0, gbeg.data(), natoms, "pool gather ranges") &&
+                    stage_upload(c, p, p->gend.p + a0, gend.data(), natoms, "pool gather ranges") &&
+                    stage_upload(c, p, p->pnbr.p + p0, pnbr.data(), pnbr.size(), "pool neighbors") &&
+                    stage_upload(c, p, p->pshift.p + 3 * p0, nl.shift.data(), nl.shift.size(), "pool shifts") &&
+                    stage_upload(c, p, p->gent.p + 2 * p0, gent.data(), gent.size(), "pool gather entries");
+    if (!ok) {
+        p->atoms.give(r.a0, natoms);
+        p->pairs.give(r.p0, std::max<int64_t>(r.np, 1));
+        return MLIP_ERR_DEVICE;
+    }
+    r.id = p->next_id++;
+    *handle = r.id;
+    p->req.emplace(r.id, std::move(r));
+    return MLIP_OK;
+}
+
+}  // namespace
+}  // namespace mlip
+
+extern "C" {
+
+int mlip_pool_admit(mlip_ctx* c, int natoms, const double* cell, const double* x, const int* type, const double* v,
+                    int nsteps, double dt, double skin, int* handle) {
+    if (!c || !c->pool || natoms < 1 || !cell || !x || !type || !handle || nsteps < 0 || skin < 0.0 ||
+        (nsteps > 0 && !(dt > 0.0)))
+        return MLIP_ERR_ARG;
+    Prepared pr;
+    prepare(c, natoms, cell, x, type, skin, pr);
+    if (pr.rc != MLIP_OK) {
+        c->err = pr.err;
+        return pr.rc;
+    }
+    return commit(c, c->pool, pr, natoms, x, v, nsteps, dt, handle);
+}
+
+int mlip_pool_admit_many(mlip_ctx* c, int count, const int* natoms, const double* cells, const double* x,
+                         const int* type, const double* v, const int* nsteps, double dt, const double* skin,
+                         int nthreads, int* handles) {
+    if (!c || !c->pool || count < 0 || (count > 0 && (!natoms || !cells || !x || !type || !nsteps || !skin ||
+                                                      !handles)))
+        return -MLIP_ERR_ARG;
+    std::vector<int64_t> at(count + 1, 0);
+    for (int k = 0; k < count; ++k) {
+        if (natoms[k] < 1 || nsteps[k] < 0 || skin[k] < 0.0 || (nsteps[k] > 0 && !(dt > 0.0))) return -MLIP_ERR_ARG;
+        at[k + 1] = at[k] + natoms[k];
+    }
+    // neighbor lists on worker threads, then FIFO commits
+    std::vector<Prepared> pr(count);
+    const int nt = std::max(1, std::min(nthreads, count));
+    std::atomic<int> next{0};
+    auto work = [&]() {
+        for (int k = next++; k < count; k = next++)
+            prepare(c, natoms[k], cells + 9 * k, x + 3 * at[k], type + at[k], skin[k], pr[k]);
+    };
+    std::vector<std::thread> th;
+    for (int t = 1; t < nt; ++t) th.emplace_back(work);
+    work();
+    for (auto& t : th) t.join();
+    int admitted = 0;
+    for (int k = 0; k < count; ++k) {
+        if (pr[k].rc != MLIP_OK) {
+            c->err = pr[k].err;
+            return admitted > 0 ? admitted : -pr[k].rc;
+        }
+        const int rc = commit(c, c->pool, pr[k], natoms[k], x + 3 * at[k], v ? v + 3 * at[k] : nullptr,
+                              nsteps[k], dt, &handles[k]);
+        if (rc == MLIP_ERR_FULL) break;
+        if (rc != MLIP_OK) return admitted > 0 ? admitted : -rc;
+        ++admitted;
+    }
+    return admitted;
+}
+
+int mlip_pool_tick(mlip_ctx* c, int* nbatch) {
+    if (!c || !c->pool) return MLIP_ERR_ARG;
+    Pool* p = c->pool;
+    if (p->in_flight) {
+        c->err = "pool tick already in flight (call mlip_pool_sync)";
+        return MLIP_ERR_STATE;
+    }
+    p->batch.clear();
+    for (const auto& kv : p->req)
+        if (!kv.second.done) p->batch.push_back(kv.first);
+    if (nbatch) *nbatch = static_cast<int>(p->batch.size());
+    if (p->batch.empty()) return MLIP_OK;
+    const Model& m = c->model;
+    const int nb = static_cast<int>(p->batch.size());
+    // slot blocks by species over the batch, request ranges, NVE steps
+    std::vector<int32_t> aos, btype, sbeg(nb), send(nb);
+    std::vector<NveDesc> first, final_;
+    for (int t = 0; t < m.n_type; ++t) {
+        size_t k = 0;
+        for (int id : p->batch) {
+            const Request& r = p->req[id];
+            for (int i = 0; i < r.natoms; ++i)
+                if (r.elem[i] == t) {
+                    if (k % kBlock == 0) btype.push_back(t);
+                    aos.push_back(static_cast<int32_t>(r.a0 + i));
+                    ++k;
+                }
+        }
+        while (k % kBlock) {
+            aos.push_back(-1);
+            ++k;
+        }
+    }
+    for (int b = 0; b < nb; ++b) {
+        const Request& r = p->req[p->batch[b]];
+        sbeg[b] = static_cast<int32_t>(r.a0);
+        send[b] = static_cast<int32_t>(r.a0 + r.natoms);
+        if (r.nsteps > 0 && r.step > 0) {
+            const NveDesc d{static_cast<int32_t>(r.a0), r.natoms, 0.5 * r.dt * kFtm2v, r.dt};
+            first.push_back(d);
+            final_.push_back(d);
+        }
+    }
+    const int nslot = static_cast<int>(aos.size());
+    if (nslot > p->cap_slots) {
+        c->err = "request pool: slot capacity exceeded";
+        return MLIP_ERR_STATE;
+    }
+    int maxn = 0;
+    for (const auto& d : first) maxn = std::max(maxn, d.n);
+    bool ok = stage_upload(c, p, p->atom_of_slot.p, aos.data(), aos.size(), "pool slots") &&
+              stage_upload(c, p, p->block_type.p, btype.data(), btype.size(), "pool block types") &&
+              stage_upload(c, p, p->sbeg.p, sbeg.data(), sbeg.size(), "pool request ranges") &&
+              stage_upload(c, p, p->send.p, send.data(), send.size(), "pool request ranges") &&
+              stage_upload(c, p, p->first.p, first.data(), first.size(), "pool nve") &&
+              stage_upload(c, p, p->final_.p, final_.data(), final_.size(), "pool nve");
+    if (!ok) return MLIP_ERR_DEVICE;
+    hipStream_t st = c->stream;
+    if (!first.empty())
+        hipLaunchKernelGGL(k_nve_first, dim3((maxn + 255) / 256, static_cast<unsigned>(first.size())), dim3(256), 0,
+                           st, p->first.p, p->elem.p, p->mass.p, p->f.p, p->v.p, p->x.p);
+    hipLaunchKernelGGL(k_scatter_slots, dim3((nslot + 255) / 256), dim3(256), 0, st, p->atom_of_slot.p, nslot,
+                       p->slot_of_atom.p);
+    bind_pool(c, p, static_cast<int>(btype.size()));
+    c->nstruct = nb;
+    c->sbeg = p->sbeg.p;
+    c->send = p->send.p;
+    c->sres = p->sout.p;
+    c->total_reduce = false;
+    // work accounting for mlip_flops
+    c->owners_per_type.assign(m.n_type, 0);
+    int64_t npr = 0;
+    for (int id : p->batch) {
+        for (int32_t e : p->req[id].elem) ++c->owners_per_type[e];
+        npr += p->req[id].np;
+    }
+    c->npairs = npr;
+    if (!run_pipeline(c)) return MLIP_ERR_DEVICE;
+    if (!final_.empty())
+        hipLaunchKernelGGL(k_nve_final, dim3((maxn + 255) / 256, static_cast<unsigned>(final_.size())), dim3(256), 0,
+                           st, p->final_.p, p->elem.p, p->mass.p, p->f.p, p->v.p);
+    hipLaunchKernelGGL(k_ke, dim3(nb), dim3(256), 0, st, p->sbeg.p, p->send.p, p->elem.p, p->mass.p, p->v.p, p->ke.p);
+    if (!hip_ok(c, hipGetLastError(), "pool kernels") ||
+        !hip_ok(c, hipMemcpyAsync(p->h_out, p->sout.p, 7 * nb * sizeof(double), hipMemcpyDeviceToHost, st),
+                "pool results") ||
+        !hip_ok(c, hipMemcpyAsync(p->h_out + 7 * nb, p->ke.p, 6 * nb * sizeof(double), hipMemcpyDeviceToHost, st),
+                "pool kinetic"))
+        return MLIP_ERR_DEVICE;
+    p->in_flight = true;
+    return MLIP_OK;
+}
+
+int mlip_pool_sync(mlip_ctx* c, int* ncompleted) {
+    if (!c || !c->pool) return MLIP_ERR_ARG;
+    Pool* p = c->pool;
+    if (ncompleted) *ncompleted = 0;
+    if (!hip_ok(c, hipStreamSynchronize(c->stream), "pool sync")) return MLIP_ERR_DEVICE;
+    p->staging.clear();
+    if (!p->in_flight) return MLIP_OK;
+    p->in_flight = false;
+    if (c->prof) collect_profile(c);
+    const int nb = static_cast<int>(p->batch.size());
+    int ndone = 0;
+    for (int b = 0; b < nb; ++b) {
+        Request& r = p->req[p->batch[b]];
+        for (int k = 0; k < 7; ++k) r.thermo.push_back(p->h_out[7 * b + k]);
+        for (int k = 0; k < 6; ++k) r.thermo.push_back(p->h_out[7 * nb + 6 * b + k]);
+        const bool last = r.nsteps == 0 || r.step == r.nsteps;
+        if (!last) {
+            ++r.step;
+            continue;
+        }
+        const size_t n3 = 3 * static_cast<size_t>(r.natoms);
+        r.f.resize(n3);
+        r.x.resize(n3);
+        r.v.resize(n3);
+        const size_t a3 = 3 * static_cast<size_t>(r.a0);
+        if (!hip_ok(c, hipMemcpy(r.f.data(), p->f.p + a3, n3 * 8, hipMemcpyDeviceToHost), "pool forces") ||
+            !hip_ok(c, hipMemcpy(r.x.data(), p->x.p + a3, n3 * 8, hipMemcpyDeviceToHost), "pool positions") ||
+            !hip_ok(c, hipMemcpy(r.v.data(), p->v.p + a3, n3 * 8, hipMemcpyDeviceToHost), "pool velocities"))
+            return MLIP_ERR_DEVICE;
+        r.done = true;
+        // the freed atoms own no pairs or gather entries any more
+        const size_t a0 = static_cast<size_t>(r.a0), n = static_cast<size_t>(r.natoms);
+        if (!hip_ok(c, hipMemsetAsync(p->pbeg.p + a0, 0, n * 4, c->stream), "pool release") ||
+            !hip_ok(c, hipMemsetAsync(p->pend.p + a0, 0, n * 4, c->stream), "pool release") ||
+            !hip_ok(c, hipMemsetAsync(p->gbeg.p + a0, 0, n * 4, c->stream), "pool release") ||
+            !hip_ok(c, hipMemsetAsync(p->gend.p + a0, 0, n * 4, c->stream), "pool release"))
+            return MLIP_ERR_DEVICE;
+        p->atoms.give(r.a0, r.natoms);
+        p->pairs.give(r.p0, std::max<int64_t>(r.np, 1));
+        p->done_ids.push_back(r.id);
+        ++ndone;
+    }
+    if (ncompleted) *ncompleted = ndone;
+    return MLIP_OK;
+}
+
+int mlip_pool_done(mlip_ctx* c, int* ids, int cap) {
+    if (!c || !c->pool || (!ids && cap > 0)) return -MLIP_ERR_ARG;
+    Pool* p = c->pool;
+    const int n = std::min<int>(cap, static_cast<int>(p->done_ids.size()));
+    for (int k = 0; k < n; ++k) ids[k] = p->done_ids[k];
+    p->done_ids.erase(p->done_ids.begin(), p->done_ids.begin() + n);
+    return n;
+}
+
+int mlip_pool_result(mlip_ctx* c, int id, double* energy, double* virial, double* f, double* x, double* v,
+                     double* thermo, int thermo_cap) {
+    if (!c || !c->pool) return -MLIP_ERR_ARG;
+    Pool* p = c->pool;
+    auto it = p->req.find(id);
+    if (it == p->req.end() || !it->second.done) {
+        c->err = "request not completed";
+        return -MLIP_ERR_STATE;
+    }
+    const Request& r = it->second;
+    const size_t nev = r.thermo.size() / kThermo;
+    const double* lastv = r.thermo.data() + (nev - 1) * kThermo;
+    if (energy) *energy = lastv[0];
+    if (virial)
+        for (int k = 0; k < 6; ++k) virial[k] = lastv[1 + k];
+    if (f) std::copy(r.f.begin(), r.f.end(), f);
+    if (x) std::copy(r.x.begin(), r.x.end(), x);
+    if (v) std::copy(r.v.begin(), r.v.end(), v);
+    if (thermo) std::copy(r.thermo.begin(), r.thermo.begin() + std::min<size_t>(r.thermo.size(), thermo_cap),
+                          thermo);
+    const int nret = static_cast<int>(nev);
+    p->req.erase(it);
+    return nret;
+}
+
+int mlip_pool_stats(const mlip_ctx* c, int64_t* out6) {
+    if (!c || !c->pool || !out6) return MLIP_ERR_ARG;
+    const Pool* p = c->pool;
+    int64_t active = 0;
+    for (const auto& kv : p->req) active += kv.second.done ? 0 : 1;
+    out6[0] = p->atoms.used();
+    out6[1] = p->pairs.used();
+    out6[2] = p->cap_atoms;
+    out6[3] = p->cap_pairs;
+    out6[4] = static_cast<int64_t>(p->req.size());
+    out6[5] = active;
+    return MLIP_OK;
+}
+
+int64_t mlip_pool_bytes(const mlip_ctx* c, int64_t max_atoms, int64_t max_pairs, int max_requests) {
+    if (!c || max_atoms < 1 || max_pairs < 1 || max_requests < 1) return -1;
+    const Model& m = c->model;
+    const int64_t slots = (max_atoms + static_cast<int64_t>(m.n_type) * kBlock + kBlock - 1) / kBlock * kBlock;
+    const int64_t per_slot = 8 * (2 * static_cast<int64_t>(m.n_fn) * m.n_type * m.n_lmh * 2 + 3 * m.n_p + 3 +
+                                  static_cast<int64_t>(m.n_fn) * (1 + kWavesPerWg)) +
+                             8;
+    const int64_t per_atom = 8 * (3 * 3 + 6) + 4 * 7;
+    const int64_t per_pair = 4 + 8 * 3 + 8 * 3 + 2 * 4;
+    return slots * per_slot + max_atoms * per_atom + max_pairs * per_pair + max_requests * (13 * 8 + 8 + 2 * 24);
+}
+
+}  // extern "C"

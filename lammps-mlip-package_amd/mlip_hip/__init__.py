@@ -6,4 +6,5 @@ reference pair-style interface (``pair_style mlip_gtinv`` /
 ``pair_coeff * * file elements``) for Python callers.
 """
 from ._lib import LIB_PATH, MlipError, library, build_library  # noqa: F401
-from .pair import BatchSystem, PairMLIPGtinv, ResidentSystem, inspect_model, periodic_full_list  # noqa: F401
+from .pair import (BatchSystem, PairMLIPGtinv, RequestPool, ResidentSystem, inspect_model,  # noqa: F401
+                   periodic_full_list)

@@ -7,7 +7,8 @@ PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(PKG_DIR, "_build", "libmlip_hip.so")
 
 MLIP_OK = 0
-ERRORS = {1: "argument", 2: "file", 3: "model", 4: "device", 5: "state"}
+ERRORS = {1: "argument", 2: "file", 3: "model", 4: "device", 5: "state", 6: "full"}
+MLIP_ERR_FULL = 6
 
 
 class MlipError(RuntimeError):
@@ -62,6 +63,17 @@ def library():
     L.mlip_random_structures.argtypes = [c_int, c_int, c_int, c_double, c_double, ctypes.c_uint64, ip, dp, dp, ip,
                                          ctypes.c_int64]
     L.mlip_set_force_threads.argtypes = [vp, c_int]
+    i64 = ctypes.c_int64
+    L.mlip_pool_bytes.argtypes = [vp, i64, i64, c_int]
+    L.mlip_pool_bytes.restype = i64
+    L.mlip_pool_init.argtypes = [vp, i64, i64, c_int]
+    L.mlip_pool_admit.argtypes = [vp, c_int, dp, dp, ip, dp, c_int, c_double, c_double, ip]
+    L.mlip_pool_admit_many.argtypes = [vp, c_int, ip, dp, dp, ip, dp, ip, c_double, dp, c_int, ip]
+    L.mlip_pool_tick.argtypes = [vp, ip]
+    L.mlip_pool_sync.argtypes = [vp, ip]
+    L.mlip_pool_done.argtypes = [vp, ip, c_int]
+    L.mlip_pool_result.argtypes = [vp, c_int, dp, dp, dp, dp, dp, dp, c_int]
+    L.mlip_pool_stats.argtypes = [vp, i64p]
     L.mlip_batch_set.argtypes = [vp, c_int, ip, dp, dp, ip]
     L.mlip_batch_get.argtypes = [vp, dp, dp, dp]
     L.mlip_graph_mode.argtypes = [vp, c_int]

@@ -10,7 +10,7 @@ import ctypes
 
 import numpy as np
 
-from ._lib import MLIP_OK, MlipError, library
+from ._lib import MLIP_ERR_FULL, MLIP_OK, MlipError, library
 
 INFO_KEYS = ["n_type", "n_fn", "n_gtinv", "n_des", "n_coeff", "maxl", "lmax3", "n_poly_features", "n_irreps",
              "gtinv_order", "model_type", "maxp"]
@@ -225,6 +225,112 @@ class ResidentSystem(_ResidentBase):
             out["eatom"] = ea
         return out
 
+
+
+class RequestPool:
+    """Device-resident request pool with continuous batching (see the
+    mlip_pool_* entry points in include/mlip_hip.h).  Structures are
+    workloads.Structure objects; results come back per request."""
+
+    THERMO = 13  # energy, virial[6], kinetic-energy tensor[6] per evaluation
+
+    def __init__(self, pair, max_atoms, max_pairs, max_requests):
+        self.pair = pair
+        self.lib = pair.lib
+        self.natoms = {}
+        pair._check(self.lib.mlip_pool_init(pair.ctx, int(max_atoms), int(max_pairs), int(max_requests)))
+
+    @staticmethod
+    def bytes_for(pair, max_atoms, max_pairs, max_requests):
+        return int(pair.lib.mlip_pool_bytes(pair.ctx, int(max_atoms), int(max_pairs), int(max_requests)))
+
+    def admit(self, s, nsteps=0, dt=0.001, skin=0.0, velocities=None):
+        """Returns a handle, or None when the pool has no room now."""
+        cell = np.ascontiguousarray(s.cell, dtype=np.float64).reshape(9)
+        x = np.ascontiguousarray(s.positions, dtype=np.float64)
+        t = np.ascontiguousarray(s.types, dtype=np.int32)
+        v = None if velocities is None else np.ascontiguousarray(velocities, dtype=np.float64)
+        h = ctypes.c_int(0)
+        rc = self.lib.mlip_pool_admit(self.pair.ctx, int(s.natoms), _p(cell, ctypes.c_double), _p(x, ctypes.c_double),
+                                      _p(t, ctypes.c_int), _p(v, ctypes.c_double) if v is not None else _null(),
+                                      int(nsteps), float(dt), float(skin), ctypes.byref(h))
+        if rc == MLIP_ERR_FULL:
+            return None
+        self.pair._check(rc)
+        self.natoms[h.value] = int(s.natoms)
+        return h.value
+
+    def admit_many(self, structures, nsteps, dt=0.001, skins=None, velocities=None, nthreads=16):
+        """FIFO admission of several structures (neighbor lists built on
+        `nthreads` host threads); returns the handles of those admitted (a
+        prefix of the list)."""
+        n = len(structures)
+        if n == 0:
+            return []
+        nat = np.array([s.natoms for s in structures], dtype=np.int32)
+        cells = np.ascontiguousarray(np.concatenate([s.cell.reshape(9) for s in structures]), dtype=np.float64)
+        x = np.ascontiguousarray(np.concatenate([s.positions for s in structures]), dtype=np.float64)
+        t = np.ascontiguousarray(np.concatenate([s.types for s in structures]), dtype=np.int32)
+        ns = np.ascontiguousarray(nsteps, dtype=np.int32)
+        sk = np.ascontiguousarray(np.zeros(n) if skins is None else skins, dtype=np.float64)
+        v = None
+        if velocities is not None:
+            v = np.ascontiguousarray(np.concatenate([np.zeros((s.natoms, 3)) if vv is None else vv
+                                                     for s, vv in zip(structures, velocities)]), dtype=np.float64)
+        hs = np.zeros(n, dtype=np.int32)
+        k = self.lib.mlip_pool_admit_many(self.pair.ctx, n, _p(nat, ctypes.c_int), _p(cells, ctypes.c_double),
+                                          _p(x, ctypes.c_double), _p(t, ctypes.c_int),
+                                          _p(v, ctypes.c_double) if v is not None else _null(), _p(ns, ctypes.c_int),
+                                          float(dt), _p(sk, ctypes.c_double), int(nthreads), _p(hs, ctypes.c_int))
+        if k < 0:
+            self.pair._check(-k)
+        for j in range(k):
+            self.natoms[int(hs[j])] = int(nat[j])
+        return hs[:k].tolist()
+
+    def tick(self):
+        """Enqueue one batched evaluation of every resident request; returns
+        the batch size (0: nothing resident)."""
+        n = ctypes.c_int(0)
+        self.pair._check(self.lib.mlip_pool_tick(self.pair.ctx, ctypes.byref(n)))
+        return n.value
+
+    def sync(self):
+        n = ctypes.c_int(0)
+        self.pair._check(self.lib.mlip_pool_sync(self.pair.ctx, ctypes.byref(n)))
+        return n.value
+
+    def done(self, cap=4096):
+        ids = np.zeros(cap, dtype=np.int32)
+        n = self.lib.mlip_pool_done(self.pair.ctx, _p(ids, ctypes.c_int), cap)
+        if n < 0:
+            self.pair._check(-n)
+        return ids[:n].tolist()
+
+    def result(self, handle, max_evals=4096):
+        n = self.natoms.pop(handle)
+        e = np.zeros(1)
+        vir = np.zeros(6)
+        f = np.zeros((n, 3))
+        x = np.zeros((n, 3))
+        v = np.zeros((n, 3))
+        th = np.zeros(max_evals * self.THERMO)
+        nev = self.lib.mlip_pool_result(self.pair.ctx, int(handle), _p(e, ctypes.c_double), _p(vir, ctypes.c_double),
+                                        _p(f, ctypes.c_double), _p(x, ctypes.c_double), _p(v, ctypes.c_double),
+                                        _p(th, ctypes.c_double), th.size)
+        if nev < 0:
+            self.pair._check(-nev)
+        return {"energy": float(e[0]), "virial": vir, "forces": f, "positions": x, "velocities": v,
+                "thermo": th[:nev * self.THERMO].reshape(nev, self.THERMO)}
+
+    def stats(self):
+        out = np.zeros(6, dtype=np.int64)
+        self.pair._check(self.lib.mlip_pool_stats(self.pair.ctx, _p(out, ctypes.c_int64)))
+        return dict(zip(["atoms_used", "pairs_used", "atom_capacity", "pair_capacity", "requests", "running"],
+                        out.tolist()))
+
+    def profile(self, on=True):
+        self.lib.mlip_profile_enable(self.pair.ctx, 1 if on else 0)
 
 
 def inspect_model(potfile, which, cap=1 << 26):
