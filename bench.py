@@ -93,6 +93,19 @@ def timed_steps(rs, warmup, steps, tag):
     return dt, prof, rs.get(), nt
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the calibrated FETCH_SIZE /
+    WRITE_SIZE counter passes summarised in profiles/ (None if absent)."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_256k_fp64.json")
+    if not os.path.exists(path):
+        return None
+    data = json.load(open(path))
+    k = data.get("kernels", {}).get(kernel)
+    if not k:
+        return None
+    return {"bytes_per_launch": k["bytes"], "source": "profiles/hbm_traffic_256k_fp64.json"}
+
+
 def make_pair(pot, precision):
     p = PairMLIPGtinv()
     p.settings()
@@ -202,6 +215,7 @@ def main():
     if not np.isfinite(res["energy"]) or not np.all(np.isfinite(res["forces"])):
         raise SystemExit("non-finite result")
     flops = rs.flops()
+    flops_ref = rs.flops_reference()
     mixed = None
     if args.precision == "fp64" and not args.no_mixed:
         pair.set_precision("mixed")
@@ -235,12 +249,23 @@ def main():
     fl_key = {"q34_adj": "q34_adj", "q34_feat": "q34_feat", "anlm": "anlm", "force": "force",
               "poly_gemm": "poly", "q12_adj": "q12", "q12_feat": "q12"}.get(dom)
     dom_ms = kernels[dom]["avg_ms"]
-    achieved = flops[fl_key] / (dom_ms * 1e-3) / 1e12 if fl_key else None
+    ref_fl = flops_ref[fl_key] if fl_key else None
+    exe_fl = flops[fl_key] if fl_key else None
+    achieved = ref_fl / (dom_ms * 1e-3) / 1e12 if fl_key else None
+    traffic = measured_traffic(dom)
     roofline = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": (achieved / FP64_PEAK_TFLOPS) if achieved else None, "traffic": None,
+                "frac": (achieved / FP64_PEAK_TFLOPS) if achieved else None,
+                "traffic": traffic["bytes_per_launch"] if traffic else None,
                 "kernel": dom, "kernel_avg_ms": dom_ms,
-                "algorithmic_flops_per_launch": flops[fl_key] if fl_key else None,
-                "note": "FP64 VALU work; the MI355X FP64 peak is the same for vector and matrix units"}
+                "algorithmic_flops_per_launch": ref_fl,
+                "rule": "reference formulation: per half pair 40 + 12 n_fn + 38 n_lmh + 52 n_fn n_lmh; per atom "
+                        "n_fn sum T(6(q-1)+3) invariants + n_fn sum T q(6(q-1)+4) adjoint + 2 sum|p| polynomial",
+                "executed_flops_per_launch": exe_fl,
+                "executed_tflops": exe_fl / (dom_ms * 1e-3) / 1e12 if fl_key else None,
+                "traffic_source": traffic["source"] if traffic else None,
+                "note": "FP64 VALU work (the MI355X FP64 peak is the same for vector and matrix units); the "
+                        "kernel pairs m with -m and shares partial products, so it executes fewer operations "
+                        "than the reference formulation counts"}
     total_rate = flops["total"] * args.steps / dt / 1e12
     log("steps %d in %.3fs -> %.1f ms/step, %.4g atom-evals/s; whole-step FP64 rate %.2f TFLOP/s"
         % (args.steps, dt, 1e3 * dt / args.steps, value, total_rate))
@@ -276,6 +301,7 @@ def main():
         "served": served,
         "kernels_ms": {k: round(v["avg_ms"], 4) for k, v in kernels.items()},
         "flops_per_step": flops,
+        "flops_reference_per_step": flops_ref,
         "step_tflops": total_rate,
         "energy_per_atom": res["energy"] / natoms,
     }

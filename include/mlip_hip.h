@@ -136,6 +136,14 @@ int mlip_profile_read(mlip_ctx* ctx, const char** names, double* ms, int* count,
  * structure, per kernel: a_nlm, order 1/2, order 3/4 features, order 3/4
  * adjoint, polynomial, forces, total (see engine.hip for the rule). */
 int mlip_flops(const mlip_ctx* ctx, double* out7);
+/* The same seven entries by the reference-formulation rule: per half pair
+ * 40 + 12 n_fn + 38 n_lmh + 8 n_fn n_lmh (a_nlm) and 44 n_fn n_lmh (force);
+ * per atom n_fn sum T (6(q-1)+3) (invariants), n_fn sum T q (6(q-1)+4)
+ * (adjoint), 2 sum |p| (polynomial) over the itcs / terms the centre's
+ * species sees (T = CG terms of the irrep, q its order).  This is the
+ * algorithmic count that roofline figures are quoted against; mlip_flops
+ * counts what this implementation's kernels execute. */
+int mlip_flops_reference(const mlip_ctx* ctx, double* out7);
 
 /* Launches a plain device copy of nbytes `reps` times (byte-counter calibration). */
 int mlip_calibration_copy(mlip_ctx* ctx, int64_t nbytes, int reps);

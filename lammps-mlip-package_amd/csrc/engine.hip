@@ -974,6 +974,56 @@ int mlip_flops(const mlip_ctx* c, double* out7) {
     return MLIP_OK;
 }
 
+// Algorithmic FLOPs of the current structure by the reference-formulation
+// rule (counting weights: add or mul 1, FMA 2, complex x complex 6,
+// complex x real 2; transcendental or division 1):
+//   per half pair   40 + 12 n_fn + 38 n_lmh          geometry, radial, Y_lm
+//                   + 8 n_fn n_lmh                    a_nlm accumulation
+//                   + 44 n_fn n_lmh                   pair forces
+//   per atom        n_fn sum_{active itc} T (6(q-1) + 3)     invariants
+//                   n_fn sum_{active itc} T q (6(q-1) + 4)   adjoint G
+//                   2 sum_{active poly terms} |p|            E and H
+// T = CG terms of the itc's irrep, q its order.  Counted by the kernel that
+// carries the work: a_nlm (geometry + accumulation), q12 (orders <= 2),
+// q34_feat / q34_adj (orders >= 3: polynomial features / the rest with G),
+// poly, force; out7[6] = total.
+int mlip_flops_reference(const mlip_ctx* c, double* out7) {
+    if (!c || !out7) return MLIP_ERR_ARG;
+    const Model& m = c->model;
+    for (int k = 0; k < 7; ++k) out7[k] = 0.0;
+    const double half = 0.5 * static_cast<double>(c->npairs);
+    const double nl = m.n_lmh, nf = m.n_fn;
+    out7[0] = half * (40.0 + 12.0 * nf + 38.0 * nl + 8.0 * nf * nl);
+    out7[5] = half * 44.0 * nf * nl;
+    std::vector<char> in_poly(m.n_gtinv, 0);
+    if (m.maxp > 1)
+        for (int f : m.poly_features) in_poly[f % m.n_gtinv] = 1;
+    for (int t = 0; t < m.n_type && t < static_cast<int>(c->owners_per_type.size()); ++t) {
+        const double na = static_cast<double>(c->owners_per_type[t]);
+        if (na == 0.0) continue;
+        for (int i = 0; i < m.n_gtinv; ++i) {
+            const Itc& it = m.itcs[i];
+            if (!((it.type_mask >> t) & 1u)) continue;
+            const double T = static_cast<double>(m.irreps[it.irrep].m.size()), q = it.order;
+            const double inv = nf * T * (6.0 * (q - 1.0) + 3.0), adj = nf * T * q * (6.0 * (q - 1.0) + 4.0);
+            if (it.order <= 2)
+                out7[1] += na * (inv + adj);
+            else if (in_poly[i])
+                out7[2] += na * inv, out7[3] += na * adj;
+            else
+                out7[3] += na * (inv + adj);
+        }
+        auto active = [&](int f) { return ((m.itcs[f % m.n_gtinv].type_mask >> t) & 1u) != 0; };
+        double terms = 0.0;
+        for (int f = 0; f < m.n_des; ++f) terms += active(f) ? 1.0 : 0.0;
+        for (const auto& p2 : m.comb2) terms += (active(p2[0]) && active(p2[1])) ? 2.0 : 0.0;
+        for (const auto& p3 : m.comb3) terms += (active(p3[0]) && active(p3[1]) && active(p3[2])) ? 3.0 : 0.0;
+        out7[4] += na * 2.0 * terms;
+    }
+    for (int k = 0; k < 6; ++k) out7[6] += out7[k];
+    return MLIP_OK;
+}
+
 // Plain device copy of nbytes (calibrates HBM byte counters against a
 // kernel whose traffic is known exactly).
 int mlip_calibration_copy(mlip_ctx* c, int64_t nbytes, int reps) {

@@ -63,6 +63,7 @@ def library():
     L.mlip_random_structures.argtypes = [c_int, c_int, c_int, c_double, c_double, ctypes.c_uint64, ip, dp, dp, ip,
                                          ctypes.c_int64]
     L.mlip_set_force_threads.argtypes = [vp, c_int]
+    L.mlip_flops_reference.argtypes = [vp, dp]
     i64 = ctypes.c_int64
     L.mlip_pool_bytes.argtypes = [vp, i64, i64, c_int]
     L.mlip_pool_bytes.restype = i64

@@ -156,6 +156,14 @@ class _ResidentBase:
         keys = ["anlm", "q12", "q34_feat", "q34_adj", "poly", "force", "total"]
         return dict(zip(keys, out.tolist()))
 
+    def flops_reference(self):
+        """Algorithmic FLOPs by the reference-formulation rule (see
+        mlip_flops_reference)."""
+        out = np.zeros(7)
+        self.pair._check(self.lib.mlip_flops_reference(self.pair.ctx, _p(out, ctypes.c_double)))
+        keys = ["anlm", "q12", "q34_feat", "q34_adj", "poly", "force", "total"]
+        return dict(zip(keys, out.tolist()))
+
     def calibration_copy(self, nbytes, reps=1):
         self.pair._check(self.lib.mlip_calibration_copy(self.pair.ctx, int(nbytes), int(reps)))
 
