@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <thread>
 
 namespace mlip {
 
@@ -43,7 +44,8 @@ void cross3(const double* a, const double* b, double* o) {
 
 }  // namespace
 
-bool build_full_list(const double cell[9], int n, const double* x, double rc, NeighborList& out, std::string& err) {
+bool build_full_list(const double cell[9], int n, const double* x, double rc, NeighborList& out, std::string& err,
+                     int nthreads) {
     out.off.assign(static_cast<size_t>(n) + 1, 0);
     out.idx.clear();
     out.shift.clear();
@@ -80,9 +82,10 @@ bool build_full_list(const double cell[9], int n, const double* x, double rc, Ne
             f[3 * i + k] = v;
         }
     }
-    // bins along each lattice direction at least rc wide
+    // bins along each lattice direction at least rc/2 wide (fewer distance
+    // tests than rc-wide bins: a (5/2)^3 instead of 3^3 stencil volume)
     int nb[3];
-    for (int k = 0; k < 3; ++k) nb[k] = std::max(1, static_cast<int>(std::floor(w[k] / rc)));
+    for (int k = 0; k < 3; ++k) nb[k] = std::max(1, static_cast<int>(std::floor(2.0 * w[k] / rc)));
     const int nbins = nb[0] * nb[1] * nb[2];
     std::vector<int> bin_of(n), head(nbins + 1, 0), members(n);
     for (int i = 0; i < n; ++i) {
@@ -108,42 +111,93 @@ bool build_full_list(const double cell[9], int n, const double* x, double rc, Ne
     out.wrap_shift.assign(3 * static_cast<size_t>(n), 0.0);
     for (int i = 0; i < n; ++i)
         for (int k = 0; k < 3; ++k) out.wrap_shift[3 * i + k] = x[3 * i + k] - xw[3 * i + k];
-    std::vector<std::pair<int, std::array<int, 3>>> cand;
-    for (int i = 0; i < n; ++i) {
-        const int bi = bin_of[i];
+    // expected neighbors per atom, for reserving
+    const double per_atom = 4.18879020478639 * rc * rc * rc * n / vol;
+    // per bin, the stencil's neighbor bins and their lattice images, once
+    const int sx = 2 * reach[0] + 1, sy = 2 * reach[1] + 1, sz = 2 * reach[2] + 1;
+    const int ns = sx * sy * sz;
+    std::vector<int> sbin(static_cast<size_t>(nbins) * ns);
+    std::vector<double> ssh(3 * static_cast<size_t>(nbins) * ns);
+    for (int bi = 0; bi < nbins; ++bi) {
         const int b0 = bi / (nb[1] * nb[2]), b1 = (bi / nb[2]) % nb[1], b2 = bi % nb[2];
-        const double* xi = &xw[3 * i];
+        int t = 0;
         for (int d0 = -reach[0]; d0 <= reach[0]; ++d0)
             for (int d1 = -reach[1]; d1 <= reach[1]; ++d1)
-                for (int d2 = -reach[2]; d2 <= reach[2]; ++d2) {
+                for (int d2 = -reach[2]; d2 <= reach[2]; ++d2, ++t) {
                     const int c0 = b0 + d0, c1 = b1 + d1, c2 = b2 + d2;
                     // bin index and the lattice image it lies in
-                    const int i0 = ((c0 % nb[0]) + nb[0]) % nb[0], s0 = (c0 - i0) / nb[0];
-                    const int i1 = ((c1 % nb[1]) + nb[1]) % nb[1], s1 = (c1 - i1) / nb[1];
-                    const int i2 = ((c2 % nb[2]) + nb[2]) % nb[2], s2 = (c2 - i2) / nb[2];
-                    const int bj = (i0 * nb[1] + i1) * nb[2] + i2;
-                    double sv[3];
-                    for (int k = 0; k < 3; ++k) sv[k] = s0 * cell[k] + s1 * cell[3 + k] + s2 * cell[6 + k];
-                    for (int q = head[bj]; q < head[bj + 1]; ++q) {
-                        const int j = members[q];
-                        // neighbor at x_j + S: del = x_i - x_j - S
-                        const double dx = xi[0] - xw[3 * j] - sv[0];
-                        const double dy = xi[1] - xw[3 * j + 1] - sv[1];
-                        const double dz = xi[2] - xw[3 * j + 2] - sv[2];
-                        const double r2 = dx * dx + dy * dy + dz * dz;
-                        if (r2 < rc2 && r2 > 0.0) {
-                            out.idx.push_back(j);
-                            // del = x_i - x_j + shift on the caller's (unwrapped) positions
-                            const double* wi = &out.wrap_shift[3 * i];
-                            const double* wj = &out.wrap_shift[3 * j];
-                            out.shift.push_back(-sv[0] - wi[0] + wj[0]);
-                            out.shift.push_back(-sv[1] - wi[1] + wj[1]);
-                            out.shift.push_back(-sv[2] - wi[2] + wj[2]);
-                        }
-                    }
+                    const int j0 = ((c0 % nb[0]) + nb[0]) % nb[0], s0 = (c0 - j0) / nb[0];
+                    const int j1 = ((c1 % nb[1]) + nb[1]) % nb[1], s1 = (c1 - j1) / nb[1];
+                    const int j2 = ((c2 % nb[2]) + nb[2]) % nb[2], s2 = (c2 - j2) / nb[2];
+                    const size_t e = static_cast<size_t>(bi) * ns + t;
+                    sbin[e] = (j0 * nb[1] + j1) * nb[2] + j2;
+                    for (int k = 0; k < 3; ++k) ssh[3 * e + k] = s0 * cell[k] + s1 * cell[3 + k] + s2 * cell[6 + k];
                 }
-        out.off[i + 1] = static_cast<int64_t>(out.idx.size());
     }
+    int max_occ = 0;
+    for (int b = 0; b < nbins; ++b) max_occ = std::max(max_occ, head[b + 1] - head[b]);
+    const size_t max_cand = static_cast<size_t>(ns) * max_occ;
+    auto scan = [&](int i0, int i1, std::vector<int32_t>& idx, std::vector<double>& shift, int64_t* cnt) {
+        idx.reserve(static_cast<size_t>((i1 - i0) * per_atom * 1.15) + 16);
+        shift.reserve(3 * idx.capacity());
+        // branch-free candidate pass into a small buffer, then append hits
+        std::vector<int32_t> cj(max_cand + 1);
+        std::vector<double> cs(3 * (max_cand + 1));
+        for (int i = i0; i < i1; ++i) {
+            const size_t e0 = static_cast<size_t>(bin_of[i]) * ns;
+            const double* xi = &xw[3 * i];
+            const double* wi = &out.wrap_shift[3 * i];
+            size_t k = 0;
+            for (int t = 0; t < ns; ++t) {
+                const int bj = sbin[e0 + t];
+                const double* sv = &ssh[3 * (e0 + t)];
+                const double ex = xi[0] - sv[0], ey = xi[1] - sv[1], ez = xi[2] - sv[2];
+                const double ox = -sv[0] - wi[0], oy = -sv[1] - wi[1], oz = -sv[2] - wi[2];
+                for (int q = head[bj]; q < head[bj + 1]; ++q) {
+                    const int j = members[q];
+                    // neighbor at x_j + S: del = x_i - x_j - S
+                    const double dx = ex - xw[3 * j];
+                    const double dy = ey - xw[3 * j + 1];
+                    const double dz = ez - xw[3 * j + 2];
+                    const double r2 = dx * dx + dy * dy + dz * dz;
+                    // del = x_i - x_j + shift on the caller's (unwrapped) positions
+                    const double* wj = &out.wrap_shift[3 * j];
+                    cj[k] = j;
+                    cs[3 * k] = ox + wj[0];
+                    cs[3 * k + 1] = oy + wj[1];
+                    cs[3 * k + 2] = oz + wj[2];
+                    k += (r2 < rc2) & (r2 > 0.0);
+                }
+            }
+            idx.insert(idx.end(), cj.begin(), cj.begin() + k);
+            shift.insert(shift.end(), cs.begin(), cs.begin() + 3 * k);
+            cnt[i] = static_cast<int64_t>(k);
+        }
+    };
+    std::vector<int64_t> cnt(n);
+    const int nt = std::max(1, std::min(nthreads, n / 256));
+    if (nt == 1) {
+        scan(0, n, out.idx, out.shift, cnt.data());
+    } else {
+        std::vector<std::vector<int32_t>> idx(nt);
+        std::vector<std::vector<double>> sh(nt);
+        std::vector<std::thread> th;
+        auto range = [&](int t) { return std::make_pair(static_cast<int>(static_cast<int64_t>(n) * t / nt),
+                                                        static_cast<int>(static_cast<int64_t>(n) * (t + 1) / nt)); };
+        for (int t = 1; t < nt; ++t)
+            th.emplace_back([&, t]() { scan(range(t).first, range(t).second, idx[t], sh[t], cnt.data()); });
+        scan(range(0).first, range(0).second, idx[0], sh[0], cnt.data());
+        for (auto& x_ : th) x_.join();
+        size_t tot = 0;
+        for (int t = 0; t < nt; ++t) tot += idx[t].size();
+        out.idx.reserve(tot);
+        out.shift.reserve(3 * tot);
+        for (int t = 0; t < nt; ++t) {
+            out.idx.insert(out.idx.end(), idx[t].begin(), idx[t].end());
+            out.shift.insert(out.shift.end(), sh[t].begin(), sh[t].end());
+        }
+    }
+    for (int i = 0; i < n; ++i) out.off[i + 1] = out.off[i] + cnt[i];
     return true;
 }
 

@@ -15,7 +15,10 @@ struct NeighborList {
     std::vector<double> wrap_shift;
 };
 
-// cell: rows a, b, c.  Pairs with 0 < r < rc, self-images included.
-bool build_full_list(const double cell[9], int n, const double* x, double rc, NeighborList& out, std::string& err);
+// cell: rows a, b, c.  Pairs with 0 < r < rc, self-images included; each
+// atom's neighbors in a fixed order whatever nthreads (atoms are split
+// into contiguous ranges across threads for n >= 512).
+bool build_full_list(const double cell[9], int n, const double* x, double rc, NeighborList& out, std::string& err,
+                     int nthreads = 1);
 
 }  // namespace mlip

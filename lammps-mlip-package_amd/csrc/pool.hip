@@ -94,6 +94,31 @@ __global__ void k_scatter_slots(const int32_t* atom_of_slot, int nslot, int32_t*
     if (a >= 0) slot_of_atom[a] = s;
 }
 
+// Packed host->device staging: one copy of a pinned arena, then this kernel
+// moves every piece to its destination (32-bit words; destinations are
+// 4-byte aligned, pieces 16-byte aligned in the arena).
+struct ScatterDesc {
+    uint64_t src;  // byte offset in the staged arena
+    uint64_t dst;  // device address
+    uint64_t nwords;
+};
+
+__global__ void k_scatter(const ScatterDesc* d, const unsigned char* stage) {
+    const ScatterDesc q = d[blockIdx.x];
+    const unsigned* src = reinterpret_cast<const unsigned*>(stage + q.src);
+    unsigned* dst = reinterpret_cast<unsigned*>(q.dst);
+    const uint64_t stride = static_cast<uint64_t>(gridDim.y) * blockDim.x;
+    for (uint64_t w = static_cast<uint64_t>(blockIdx.y) * blockDim.x + threadIdx.x; w < q.nwords; w += stride)
+        dst[w] = src[w];
+}
+
+struct HostArena {
+    unsigned char* h = nullptr;
+    size_t cap = 0;
+    hipEvent_t ev = nullptr;
+    bool pending = false;
+};
+
 // First-fit allocator of index ranges with coalescing of freed ranges.
 class RangeAlloc {
   public:
@@ -177,7 +202,13 @@ struct Pool {
     bool in_flight = false;
     std::vector<int> batch;
     double* h_out = nullptr;  // pinned: 7 per request, then 6 kinetic per request
-    std::vector<std::vector<char>> staging;
+    // packed uploads: two pinned arenas used in turn, one device copy
+    HostArena arena[2];
+    int cur = 0;
+    size_t fill = 0;
+    std::vector<ScatterDesc> descs;
+    DBuf<unsigned char> dstage;
+    DBuf<ScatterDesc> ddesc;
 };
 
 void pool_free(mlip_ctx* c) {
@@ -193,6 +224,12 @@ void pool_free(mlip_ctx* c) {
     for (auto* b : ibs) b->release();
     p->first.release();
     p->final_.release();
+    p->dstage.release();
+    p->ddesc.release();
+    for (auto& a : p->arena) {
+        if (a.h) (void)hipHostFree(a.h);
+        if (a.ev) (void)hipEventDestroy(a.ev);
+    }
     if (p->h_out) (void)hipHostFree(p->h_out);
     delete p;
     c->pool = nullptr;
@@ -200,13 +237,74 @@ void pool_free(mlip_ctx* c) {
 
 namespace {
 
+// start a packed upload (waits until the arena's previous copy is done)
+bool stage_begin(mlip_ctx* c, Pool* p) {
+    HostArena& a = p->arena[p->cur];
+    if (a.pending && !hip_ok(c, hipEventSynchronize(a.ev), "staging arena")) return false;
+    a.pending = false;
+    p->fill = 0;
+    p->descs.clear();
+    return true;
+}
+
+bool arena_reserve(mlip_ctx* c, HostArena& a, size_t need, size_t keep) {
+    if (need <= a.cap) return true;
+    size_t cap = std::max<size_t>(need, 2 * a.cap);
+    cap = std::max<size_t>(cap, 1 << 22);
+    unsigned char* h = nullptr;
+    if (!hip_ok(c, hipHostMalloc(reinterpret_cast<void**>(&h), cap), "pinned staging")) return false;
+    if (a.h) {
+        std::memcpy(h, a.h, keep);
+        (void)hipHostFree(a.h);
+    }
+    a.h = h;
+    a.cap = cap;
+    return true;
+}
+
 template <typename T>
-bool stage_upload(mlip_ctx* c, Pool* p, T* dst, const T* src, size_t n, const char* what) {
-    if (n == 0) return true;
-    std::vector<char> buf(reinterpret_cast<const char*>(src), reinterpret_cast<const char*>(src) + n * sizeof(T));
-    const bool ok = hip_ok(c, hipMemcpyAsync(dst, buf.data(), n * sizeof(T), hipMemcpyHostToDevice, c->stream), what);
-    p->staging.push_back(std::move(buf));
-    return ok;
+bool stage_upload(mlip_ctx* c, Pool* p, T* dst, const T* src, size_t n, const char*) {
+    static_assert(sizeof(T) % 4 == 0, "staged pieces move as 32-bit words");
+    const size_t nbytes = n * sizeof(T);
+    if (nbytes == 0) return true;
+    HostArena& a = p->arena[p->cur];
+    const size_t off = (p->fill + 15) & ~static_cast<size_t>(15);
+    if (!arena_reserve(c, a, off + nbytes, p->fill)) return false;
+    std::memcpy(a.h + off, src, nbytes);
+    p->fill = off + nbytes;
+    p->descs.push_back({off, reinterpret_cast<uint64_t>(dst), (nbytes + 3) / 4});
+    return true;
+}
+
+// one device copy of the arena, then the scatter kernel
+bool stage_flush(mlip_ctx* c, Pool* p) {
+    if (p->descs.empty()) return true;
+    HostArena& a = p->arena[p->cur];
+    if (!a.ev && !hip_ok(c, hipEventCreateWithFlags(&a.ev, hipEventDisableTiming), "staging event")) return false;
+    const size_t doff = (p->fill + 15) & ~static_cast<size_t>(15);
+    const size_t dbytes = p->descs.size() * sizeof(ScatterDesc);
+    if (!arena_reserve(c, a, doff + dbytes, p->fill)) return false;
+    std::memcpy(a.h + doff, p->descs.data(), dbytes);
+    const size_t total = doff + dbytes;
+    if (total > p->dstage.cap) {
+        // growing the device stage frees the old one; wait for its readers
+        if (!hip_ok(c, hipStreamSynchronize(c->stream), "staging grow")) return false;
+        if (!p->dstage.ensure(std::max<size_t>(total, 2 * p->dstage.cap))) {
+            c->err = "device allocation failed for staging";
+            return false;
+        }
+    }
+    if (!hip_ok(c, hipMemcpyAsync(p->dstage.p, a.h, total, hipMemcpyHostToDevice, c->stream), "staged upload"))
+        return false;
+    hipLaunchKernelGGL(k_scatter, dim3(static_cast<unsigned>(p->descs.size()), 8), dim3(256), 0, c->stream,
+                       reinterpret_cast<const ScatterDesc*>(p->dstage.p + doff), p->dstage.p);
+    if (!hip_ok(c, hipGetLastError(), "scatter launch") || !hip_ok(c, hipEventRecord(a.ev, c->stream), "staging event"))
+        return false;
+    a.pending = true;
+    p->cur ^= 1;
+    p->descs.clear();
+    p->fill = 0;
+    return true;
 }
 
 // point the pipeline's system view at the pool arrays for one tick
@@ -316,16 +414,20 @@ namespace mlip {
 namespace {
 
 // Host half of an admission that needs no pool state (runs on worker
-// threads): element mapping and the neighbor list at cutoff + skin.
+// threads): element mapping, the neighbor list at cutoff + skin and the
+// request-local force-gather lists.
 struct Prepared {
     int rc = MLIP_OK;
     std::string err;
     std::vector<int32_t> elem;
     NeighborList nl;
+    std::vector<int32_t> loff, goff, gent;
+    int64_t a0 = -1, p0 = -1;        // ranges, once placed
+    size_t off[10] = {};             // arena offsets of the ten pieces
 };
 
 void prepare(const mlip_ctx* c, int natoms, const double* cell, const double* x, const int* type, double skin,
-             Prepared& out) {
+             Prepared& out, int inner_threads = 1) {
     out.elem.resize(natoms);
     for (int i = 0; i < natoms; ++i) {
         const int t = type[i];
@@ -337,79 +439,140 @@ void prepare(const mlip_ctx* c, int natoms, const double* cell, const double* x,
         out.elem[i] = c->elem_of_type[t - 1];
     }
     std::string e;
-    if (!build_full_list(cell, natoms, x, c->model.cutoff + skin, out.nl, e)) {
+    if (!build_full_list(cell, natoms, x, c->model.cutoff + skin, out.nl, e, inner_threads)) {
         out.rc = MLIP_ERR_ARG;
         out.err = "request structure: " + e;
+        return;
     }
+    out.loff.resize(natoms + 1);
+    for (int i = 0; i <= natoms; ++i) out.loff[i] = static_cast<int32_t>(out.nl.off[i]);
+    build_gather(natoms, out.loff, out.nl.idx, nullptr, out.goff, out.gent);
 }
 
-// Pool half: ranges and uploads.  MLIP_ERR_FULL leaves the pool unchanged.
-int commit(mlip_ctx* c, Pool* p, Prepared& pr, int natoms, const double* x, const double* v, int nsteps, double dt,
-           int* handle) {
-    if (static_cast<int>(p->req.size()) >= p->cap_req) {
-        c->err = "request pool: all request slots in use";
-        return MLIP_ERR_FULL;
+template <typename F>
+void parallel_for(int count, int nthreads, F&& f) {
+    const int nt = std::max(1, std::min(nthreads, count));
+    std::atomic<int> next{0};
+    auto work = [&]() {
+        for (int k = next++; k < count; k = next++) f(k);
+    };
+    std::vector<std::thread> th;
+    for (int t = 1; t < nt; ++t) th.emplace_back(work);
+    work();
+    for (auto& t : th) t.join();
+}
+
+// Admits prepared requests 0..count-1 in order until one does not fit:
+// ranges are taken sequentially (FIFO), then every admitted request's
+// pieces are written into the pinned arena by worker threads and one packed
+// copy moves them.  Returns the number admitted, or -MLIP_ERR_*.
+int commit_many(mlip_ctx* c, Pool* p, std::vector<Prepared>& pr, int count, const int* natoms, const int64_t* at,
+                const double* x, const double* v, const int* nsteps, double dt, int nthreads, int* handles) {
+    int n = 0;
+    for (; n < count; ++n) {
+        Prepared& q = pr[n];
+        if (q.rc != MLIP_OK) {
+            c->err = q.err;
+            if (n == 0) return -q.rc;
+            break;
+        }
+        if (static_cast<int>(p->req.size()) + n >= p->cap_req) break;
+        const int64_t np = static_cast<int64_t>(q.nl.idx.size());
+        q.a0 = p->atoms.take(natoms[n]);
+        if (q.a0 < 0) break;
+        q.p0 = p->pairs.take(std::max<int64_t>(np, 1));
+        if (q.p0 < 0) {
+            p->atoms.give(q.a0, natoms[n]);
+            q.a0 = -1;
+            break;
+        }
     }
-    Request r;
-    r.natoms = natoms;
-    r.nsteps = nsteps;
-    r.dt = dt;
-    r.elem = std::move(pr.elem);
-    const NeighborList& nl = pr.nl;
-    r.np = static_cast<int64_t>(nl.idx.size());
-    r.a0 = p->atoms.take(natoms);
-    if (r.a0 < 0) {
-        pr.elem = std::move(r.elem);
-        c->err = "request pool: atom budget exhausted";
-        return MLIP_ERR_FULL;
+    if (n == 0) {
+        c->err = "request pool: capacity exhausted";
+        return 0;
     }
-    r.p0 = p->pairs.take(std::max<int64_t>(r.np, 1));
-    if (r.p0 < 0) {
-        p->atoms.give(r.a0, natoms);
-        pr.elem = std::move(r.elem);
-        c->err = "request pool: pair budget exhausted";
-        return MLIP_ERR_FULL;
+    // arena layout: ten 16-byte-aligned pieces per request
+    if (!stage_begin(c, p)) return -MLIP_ERR_DEVICE;
+    HostArena& ar = p->arena[p->cur];
+    size_t fill = 0;
+    auto place = [&](size_t bytes) {
+        const size_t o = (fill + 15) & ~static_cast<size_t>(15);
+        fill = o + bytes;
+        return o;
+    };
+    for (int k = 0; k < n; ++k) {
+        Prepared& q = pr[k];
+        const size_t na = static_cast<size_t>(natoms[k]), np = q.nl.idx.size();
+        const size_t bytes[10] = {24 * na, 24 * na, 4 * na, 4 * na, 4 * na, 4 * na, 4 * na, 4 * np, 24 * np, 8 * np};
+        double* dx = p->x.p + 3 * q.a0;
+        void* dst[10] = {dx,
+                         p->v.p + 3 * q.a0,
+                         p->elem.p + q.a0,
+                         p->pbeg.p + q.a0,
+                         p->pend.p + q.a0,
+                         p->gbeg.p + q.a0,
+                         p->gend.p + q.a0,
+                         p->pnbr.p + q.p0,
+                         p->pshift.p + 3 * q.p0,
+                         p->gent.p + 2 * q.p0};
+        for (int j = 0; j < 10; ++j) {
+            q.off[j] = place(bytes[j]);
+            if (bytes[j]) p->descs.push_back({q.off[j], reinterpret_cast<uint64_t>(dst[j]), bytes[j] / 4});
+        }
     }
-    // per-centre pair ranges, neighbors and gather lists in pool indices
-    std::vector<int32_t> pbeg(natoms), pend(natoms), pnbr(r.np), gbeg(natoms), gend(natoms);
-    std::vector<int32_t> loff(natoms + 1), lnbr(nl.idx.begin(), nl.idx.end());
-    for (int i = 0; i <= natoms; ++i) loff[i] = static_cast<int32_t>(nl.off[i]);
-    for (int i = 0; i < natoms; ++i) {
-        pbeg[i] = static_cast<int32_t>(r.p0 + loff[i]);
-        pend[i] = static_cast<int32_t>(r.p0 + loff[i + 1]);
+    if (!arena_reserve(c, ar, fill, 0)) return -MLIP_ERR_DEVICE;
+    parallel_for(n, nthreads, [&](int k) {
+        Prepared& q = pr[k];
+        const int na = natoms[k];
+        const int64_t np = static_cast<int64_t>(q.nl.idx.size());
+        unsigned char* h = ar.h;
+        std::memcpy(h + q.off[0], x + 3 * at[k], 24 * static_cast<size_t>(na));
+        if (v)
+            std::memcpy(h + q.off[1], v + 3 * at[k], 24 * static_cast<size_t>(na));
+        else
+            std::memset(h + q.off[1], 0, 24 * static_cast<size_t>(na));
+        std::memcpy(h + q.off[2], q.elem.data(), 4 * static_cast<size_t>(na));
+        int32_t* pb = reinterpret_cast<int32_t*>(h + q.off[3]);
+        int32_t* pe = reinterpret_cast<int32_t*>(h + q.off[4]);
+        int32_t* gb = reinterpret_cast<int32_t*>(h + q.off[5]);
+        int32_t* ge = reinterpret_cast<int32_t*>(h + q.off[6]);
+        const int64_t g0 = 2 * q.p0;
+        for (int i = 0; i < na; ++i) {
+            pb[i] = static_cast<int32_t>(q.p0 + q.loff[i]);
+            pe[i] = static_cast<int32_t>(q.p0 + q.loff[i + 1]);
+            gb[i] = static_cast<int32_t>(g0 + q.goff[i]);
+            ge[i] = static_cast<int32_t>(g0 + q.goff[i + 1]);
+        }
+        int32_t* nb = reinterpret_cast<int32_t*>(h + q.off[7]);
+        for (int64_t e = 0; e < np; ++e) nb[e] = static_cast<int32_t>(q.a0 + q.nl.idx[e]);
+        std::memcpy(h + q.off[8], q.nl.shift.data(), 24 * static_cast<size_t>(np));
+        int32_t* gt = reinterpret_cast<int32_t*>(h + q.off[9]);
+        for (size_t e = 0; e < q.gent.size(); ++e) {
+            const int32_t en = q.gent[e];
+            gt[e] = en > 0 ? static_cast<int32_t>(en + q.p0) : static_cast<int32_t>(en - q.p0);
+        }
+    });
+    p->fill = fill;
+    if (!stage_flush(c, p)) return -MLIP_ERR_DEVICE;
+    for (int k = 0; k < n; ++k) {
+        Prepared& q = pr[k];
+        Request r;
+        r.natoms = natoms[k];
+        r.nsteps = nsteps[k];
+        r.dt = dt;
+        r.a0 = q.a0;
+        r.p0 = q.p0;
+        r.np = static_cast<int64_t>(q.nl.idx.size());
+        r.elem = std::move(q.elem);
+        r.id = p->next_id++;
+        handles[k] = r.id;
+        p->req.emplace(r.id, std::move(r));
     }
-    for (int64_t k = 0; k < r.np; ++k) pnbr[k] = static_cast<int32_t>(r.a0 + lnbr[k]);
-    std::vector<int32_t> goff, gent;
-    build_gather(natoms, loff, lnbr, nullptr, goff, gent);
-    const int64_t g0 = 2 * r.p0;
-    for (auto& en : gent) en = en > 0 ? static_cast<int32_t>(en + r.p0) : static_cast<int32_t>(en - r.p0);
-    for (int i = 0; i < natoms; ++i) {
-        gbeg[i] = static_cast<int32_t>(g0 + goff[i]);
-        gend[i] = static_cast<int32_t>(g0 + goff[i + 1]);
-    }
-    std::vector<double> vz;
-    if (!v) vz.assign(3 * static_cast<size_t>(natoms), 0.0);
-    const size_t a0 = static_cast<size_t>(r.a0), p0 = static_cast<size_t>(r.p0);
-    const bool ok = stage_upload(c, p, p->x.p + 3 * a0, x, 3 * static_cast<size_t>(natoms), "pool positions") &&
-                    stage_upload(c, p, p->v.p + 3 * a0, v ? v : vz.data(), 3 * static_cast<size_t>(natoms),
-                                 "pool velocities") &&
-                    stage_upload(c, p, p->elem.p + a0, r.elem.data(), natoms, "pool elements") &&
-                    stage_upload(c, p, p->pbeg.p + a0, pbeg.data(), natoms, "pool pair ranges") &&
-                    stage_upload(c, p, p->pend.p + a0, pend.data(), natoms, "pool pair ranges") &&
-                    stage_upload(c, p, p->gbeg.p + a0, gbeg.data(), natoms, "pool gather ranges") &&
-                    stage_upload(c, p, p->gend.p + a0, gend.data(), natoms, "pool gather ranges") &&
-                    stage_upload(c, p, p->pnbr.p + p0, pnbr.data(), pnbr.size(), "pool neighbors") &&
-                    stage_upload(c, p, p->pshift.p + 3 * p0, nl.shift.data(), nl.shift.size(), "pool shifts") &&
-                    stage_upload(c, p, p->gent.p + 2 * p0, gent.data(), gent.size(), "pool gather entries");
-    if (!ok) {
-        p->atoms.give(r.a0, natoms);
-        p->pairs.give(r.p0, std::max<int64_t>(r.np, 1));
-        return MLIP_ERR_DEVICE;
-    }
-    r.id = p->next_id++;
-    *handle = r.id;
-    p->req.emplace(r.id, std::move(r));
-    return MLIP_OK;
+    return n;
+}
+
+int default_threads() {
+    return static_cast<int>(std::min(16u, std::max(1u, std::thread::hardware_concurrency())));
 }
 
 }  // namespace
@@ -422,13 +585,12 @@ int mlip_pool_admit(mlip_ctx* c, int natoms, const double* cell, const double* x
     if (!c || !c->pool || natoms < 1 || !cell || !x || !type || !handle || nsteps < 0 || skin < 0.0 ||
         (nsteps > 0 && !(dt > 0.0)))
         return MLIP_ERR_ARG;
-    Prepared pr;
-    prepare(c, natoms, cell, x, type, skin, pr);
-    if (pr.rc != MLIP_OK) {
-        c->err = pr.err;
-        return pr.rc;
-    }
-    return commit(c, c->pool, pr, natoms, x, v, nsteps, dt, handle);
+    std::vector<Prepared> pr(1);
+    prepare(c, natoms, cell, x, type, skin, pr[0], default_threads());
+    const int64_t at[2] = {0, natoms};
+    const int k = commit_many(c, c->pool, pr, 1, &natoms, at, x, v, &nsteps, dt, 1, handle);
+    if (k < 0) return -k;
+    return k == 1 ? MLIP_OK : MLIP_ERR_FULL;
 }
 
 int mlip_pool_admit_many(mlip_ctx* c, int count, const int* natoms, const double* cells, const double* x,
@@ -437,36 +599,20 @@ int mlip_pool_admit_many(mlip_ctx* c, int count, const int* natoms, const double
     if (!c || !c->pool || count < 0 || (count > 0 && (!natoms || !cells || !x || !type || !nsteps || !skin ||
                                                       !handles)))
         return -MLIP_ERR_ARG;
+    if (count == 0) return 0;
     std::vector<int64_t> at(count + 1, 0);
     for (int k = 0; k < count; ++k) {
         if (natoms[k] < 1 || nsteps[k] < 0 || skin[k] < 0.0 || (nsteps[k] > 0 && !(dt > 0.0))) return -MLIP_ERR_ARG;
         at[k + 1] = at[k] + natoms[k];
     }
-    // neighbor lists on worker threads, then FIFO commits
+    const int nt = nthreads > 0 ? nthreads : default_threads();
     std::vector<Prepared> pr(count);
-    const int nt = std::max(1, std::min(nthreads, count));
-    std::atomic<int> next{0};
-    auto work = [&]() {
-        for (int k = next++; k < count; k = next++)
-            prepare(c, natoms[k], cells + 9 * k, x + 3 * at[k], type + at[k], skin[k], pr[k]);
-    };
-    std::vector<std::thread> th;
-    for (int t = 1; t < nt; ++t) th.emplace_back(work);
-    work();
-    for (auto& t : th) t.join();
-    int admitted = 0;
-    for (int k = 0; k < count; ++k) {
-        if (pr[k].rc != MLIP_OK) {
-            c->err = pr[k].err;
-            return admitted > 0 ? admitted : -pr[k].rc;
-        }
-        const int rc = commit(c, c->pool, pr[k], natoms[k], x + 3 * at[k], v ? v + 3 * at[k] : nullptr,
-                              nsteps[k], dt, &handles[k]);
-        if (rc == MLIP_ERR_FULL) break;
-        if (rc != MLIP_OK) return admitted > 0 ? admitted : -rc;
-        ++admitted;
-    }
-    return admitted;
+    // few requests: threads inside each neighbor build; many: one per request
+    const int inner = std::max(1, nt / count);
+    parallel_for(count, std::min(nt, count), [&](int k) {
+        prepare(c, natoms[k], cells + 9 * k, x + 3 * at[k], type + at[k], skin[k], pr[k], inner);
+    });
+    return commit_many(c, c->pool, pr, count, natoms, at.data(), x, v, nsteps, dt, nt, handles);
 }
 
 int mlip_pool_tick(mlip_ctx* c, int* nbatch) {
@@ -519,12 +665,13 @@ int mlip_pool_tick(mlip_ctx* c, int* nbatch) {
     }
     int maxn = 0;
     for (const auto& d : first) maxn = std::max(maxn, d.n);
-    bool ok = stage_upload(c, p, p->atom_of_slot.p, aos.data(), aos.size(), "pool slots") &&
+    bool ok = stage_begin(c, p) &&
+              stage_upload(c, p, p->atom_of_slot.p, aos.data(), aos.size(), "pool slots") &&
               stage_upload(c, p, p->block_type.p, btype.data(), btype.size(), "pool block types") &&
               stage_upload(c, p, p->sbeg.p, sbeg.data(), sbeg.size(), "pool request ranges") &&
               stage_upload(c, p, p->send.p, send.data(), send.size(), "pool request ranges") &&
               stage_upload(c, p, p->first.p, first.data(), first.size(), "pool nve") &&
-              stage_upload(c, p, p->final_.p, final_.data(), final_.size(), "pool nve");
+              stage_upload(c, p, p->final_.p, final_.data(), final_.size(), "pool nve") && stage_flush(c, p);
     if (!ok) return MLIP_ERR_DEVICE;
     hipStream_t st = c->stream;
     if (!first.empty())
@@ -566,7 +713,6 @@ int mlip_pool_sync(mlip_ctx* c, int* ncompleted) {
     Pool* p = c->pool;
     if (ncompleted) *ncompleted = 0;
     if (!hip_ok(c, hipStreamSynchronize(c->stream), "pool sync")) return MLIP_ERR_DEVICE;
-    p->staging.clear();
     if (!p->in_flight) return MLIP_OK;
     p->in_flight = false;
     if (c->prof) collect_profile(c);

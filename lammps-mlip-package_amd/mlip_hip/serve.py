@@ -2,10 +2,10 @@
 Poisson-arriving requests (single energy/force/stress evaluations and short
 NVE runs), admission by pool capacity, latency statistics.
 
-One loop iteration: move arrived requests into the FIFO queue, admit from
-its head while the pool has room, enqueue one batched tick of everything
-resident, admit what arrived meanwhile (host work overlaps the device), wait
-for the tick and retire finished requests.  Latency = completion time minus
+One loop iteration: enqueue one batched tick of everything resident, then
+move arrived requests into the FIFO queue and admit from its head while the
+pool has room (host work overlaps the device; they join the next tick), wait
+for the tick and retire finished requests.  An idle pool admits first.  Latency = completion time minus
 arrival time, both on the host clock."""
 import time
 from collections import deque
@@ -75,11 +75,17 @@ def serve(pool, stream, dt=0.001, skin=0.5, drain_timeout=120.0, nthreads=16, ma
     atom_evals = 0
     ticks = 0
     batch_sizes = []
+    host = {"admit_s": 0.0, "tick_s": 0.0, "sync_s": 0.0, "results_s": 0.0}
 
     def now():
         return time.perf_counter() - t0
 
     def admit():
+        ta = time.perf_counter()
+        _admit()
+        host["admit_s"] += time.perf_counter() - ta
+
+    def _admit():
         tn = now()
         while pending and pending[0].t_arrival <= tn:
             queue.append(pending.popleft())
@@ -107,19 +113,28 @@ def serve(pool, stream, dt=0.001, skin=0.5, drain_timeout=120.0, nthreads=16, ma
     while pending or queue or by_handle:
         if now() > drain_timeout:
             break
-        admit()
+        if not by_handle:
+            # idle device: admit what has arrived, or wait for the next arrival
+            admit()
+            if not by_handle:
+                if pending:
+                    wait = pending[0].t_arrival - now()
+                    if wait > 0:
+                        time.sleep(min(wait, 0.01))
+                continue
+        tt = time.perf_counter()
         n = pool.tick()
+        host["tick_s"] += time.perf_counter() - tt
         if n == 0:
-            if pending:
-                wait = pending[0].t_arrival - now()
-                if wait > 0:
-                    time.sleep(min(wait, 0.01))
             continue
         ticks += 1
         batch_sizes.append(n)
         admit()  # overlaps the tick on the device
+        ts = time.perf_counter()
         pool.sync()
         tn = now()
+        tr = time.perf_counter()
+        host["sync_s"] += tr - ts
         for h in pool.done():
             res = pool.result(h)
             r = by_handle.pop(h)
@@ -127,6 +142,7 @@ def serve(pool, stream, dt=0.001, skin=0.5, drain_timeout=120.0, nthreads=16, ma
             r.evals = res["thermo"].shape[0]
             atom_evals += r.evals * r.structure.natoms
             done.append(r)
+        host["results_s"] += time.perf_counter() - tr
     makespan = now()
     lat = np.array([r.t_done - r.t_arrival for r in done]) if done else np.zeros(0)
     return {
@@ -141,6 +157,7 @@ def serve(pool, stream, dt=0.001, skin=0.5, drain_timeout=120.0, nthreads=16, ma
         "atom_evals_per_s": atom_evals / makespan if makespan > 0 else 0.0,
         "ticks": ticks,
         "mean_batch_requests": float(np.mean(batch_sizes)) if batch_sizes else 0.0,
+        "host_time_s": host,
     }
 
 
