@@ -44,6 +44,17 @@ def lib():
         L.oracle_ylm.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, dp, dp]
         L.oracle_fn.argtypes = [ctypes.c_double, ctypes.c_int, dp, ctypes.c_double, dp, dp]
         L.oracle_num_threads.restype = ctypes.c_int
+        L.pair_oracle_create.restype = vp
+        L.pair_oracle_create.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int]
+        L.pair_oracle_destroy.argtypes = [vp]
+        L.pair_oracle_info.argtypes = [vp, ip]
+        L.pair_oracle_cutoff.restype = ctypes.c_double
+        L.pair_oracle_cutoff.argtypes = [vp]
+        L.pair_oracle_masses.argtypes = [vp, dp]
+        L.pair_oracle_combs.argtypes = [vp, ctypes.c_int, ip]
+        L.pair_oracle_set_types_by_tag.argtypes = [vp, ctypes.c_int, ip]
+        L.pair_oracle_compute.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp, lp, ctypes.c_int, ip, ip, ip, ip,
+                                          dp, dp, dp, dp]
         _lib = L
     return _lib
 
@@ -56,7 +67,75 @@ INFO_KEYS = ["n_type", "n_fn", "n_gtinv", "n_des", "n_coeff", "maxl", "n_lm_all"
              "n_comb2", "n_comb3", "n_uniq_comb", "model_type", "maxp", "n_irreps"]
 
 
-class Oracle:
+def des_type(potfile):
+    """Fourth non-empty line of a potential file: gtinv or pair."""
+    lines = [ln for ln in open(potfile).read().splitlines() if ln.strip()]
+    return lines[3].split()[0]
+
+
+def Oracle(potfile):
+    """CPU oracle for a potential file of either style."""
+    return PairOracle(potfile) if des_type(potfile) == "pair" else GtinvOracle(potfile)
+
+
+PAIR_INFO_KEYS = ["n_type", "n_fn", "n_tc", "n_des", "n_coeff", "model_type", "maxp", "n_comb2", "n_comb3"]
+
+
+class PairOracle:
+    """Restatement of pair_style mlip_pair (oracle/mlip_pair_oracle.cpp)."""
+
+    def __init__(self, potfile):
+        L = lib()
+        err = ctypes.create_string_buffer(512)
+        self.h = L.pair_oracle_create(potfile.encode(), err, 512)
+        if not self.h:
+            raise ValueError(err.value.decode())
+        info = np.zeros(9, dtype=np.int32)
+        L.pair_oracle_info(self.h, _p(info, ctypes.c_int))
+        self.info = dict(zip(PAIR_INFO_KEYS, info.tolist()))
+        self.cutoff = L.pair_oracle_cutoff(self.h)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().pair_oracle_destroy(self.h)
+            self.h = None
+
+    def masses(self):
+        out = np.zeros(8)
+        n = lib().pair_oracle_masses(self.h, _p(out, ctypes.c_double))
+        return out[:n]
+
+    def combs(self, order):
+        n = self.info["n_comb2" if order == 2 else "n_comb3"]
+        out = np.zeros((max(n, 1), order), dtype=np.int32)
+        lib().pair_oracle_combs(self.h, order, _p(out, ctypes.c_int))
+        return out[:n]
+
+    def compute(self, nl, elem_of_atom, want_features=False):
+        L = lib()
+        ea = np.ascontiguousarray(elem_of_atom, dtype=np.int32)
+        L.pair_oracle_set_types_by_tag(self.h, ea.size, _p(ea, ctypes.c_int))
+        f = np.zeros((nl.nall, 3))
+        eng = np.zeros(1)
+        vir = np.zeros(6)
+        ft = np.zeros((nl.nlocal, self.info["n_des"])) if want_features else None
+        null = ctypes.POINTER(ctypes.c_double)()
+        rc = L.pair_oracle_compute(self.h, nl.nlocal, nl.nall, _p(nl.x, ctypes.c_double),
+                                   _p(nl.tag, ctypes.c_int64), nl.inum, _p(nl.ilist, ctypes.c_int),
+                                   _p(nl.numneigh, ctypes.c_int), _p(nl.off, ctypes.c_int),
+                                   _p(nl.idx, ctypes.c_int), _p(f, ctypes.c_double), _p(eng, ctypes.c_double),
+                                   _p(vir, ctypes.c_double), _p(ft, ctypes.c_double) if want_features else null)
+        if rc != 0:
+            raise RuntimeError("pair_oracle_compute failed (%d)" % rc)
+        fo = np.zeros((nl.nlocal, 3))
+        np.add.at(fo, nl.owner, f)
+        out = {"energy": float(eng[0]), "forces_all": f, "virial": vir, "forces": fo}
+        if want_features:
+            out["features"] = ft
+        return out
+
+
+class GtinvOracle:
     def __init__(self, potfile):
         L = lib()
         err = ctypes.create_string_buffer(512)

@@ -1,5 +1,6 @@
-"""NVE on the CPU oracle against the reference's regression runs
-(test/regression/gtinv-*.in: fix nve, 100 steps from rest, metal units).
+"""NVE on the CPU oracles against the reference's regression runs
+(test/regression/gtinv-*.in and pair-*.in: fix nve, 100 steps from rest,
+metal units).
 
 Bar: thermo values within the last printed digit of log.lammps.* (8
 significant digits), final positions within 1e-8 A of dump.atom.*."""
@@ -36,7 +37,7 @@ def potential_masses(path):
     return [float(t) for t in lines[-1].split("#")[0].split()]
 
 
-@pytest.mark.parametrize("name", ["gtinv-197", "gtinv-434", "gtinv-820"])
+@pytest.mark.parametrize("name", ["gtinv-197", "gtinv-434", "gtinv-820", "pair-17", "pair-60"])
 def test_nve_100_steps_matches_log_and_dump(name):
     g = goldens()[name]
     s = W.read_lammps_data(ref(g["structure"]))
