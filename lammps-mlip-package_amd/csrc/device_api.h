@@ -29,7 +29,7 @@ struct DevModel {
     int n_type, n_fn, n_gtinv, n_lmh, n_lmh3, maxl, lmax3, n_pitc, n_p;
     int has_q34;            // order-3/4 invariants present (k_q34 runs)
     double cutoff;
-    double beta[kMaxFn], mu[kMaxFn];
+    double beta[kMaxFnPair], mu[kMaxFnPair];
     const double* coef;     // dense coefficient blocks
     const double* weight;   // coeff/scale, linear part first (n_des)
     const VisitQ12* q12;    // all types
@@ -44,6 +44,12 @@ struct DevModel {
     const int32_t* c3_off;  // cubic terms transposed by target: offsets (n_p+1)
     const int32_t* c3_bc;   // 2 per entry
     const double* c3_w;
+    // radial-only style
+    int pair_model;         // 1: features sum_j g_n(r) per type combination
+    int pair_tc[2][2];      // [centre type][neighbor type] -> type combination
+    const double* pair_w;   // linear weights, n_p (= n_des) entries
+    const int32_t* ph_off;  // per-feature derivative lists
+    const PairHEntry* ph_ent;
     int q34_red;            // doubles of LDS partial rows of split adjoint groups
     int q34_dyn_adj, q34_dyn_feat;  // visits for the general-loop order-3/4 kernel
     const YTab<double>* yt64;
@@ -102,6 +108,10 @@ void launch_force(const DevModel& m, const DevSystem& s, int precision, int thre
 void launch_struct_reduce(const DevSystem& s, const int32_t* sbeg, const int32_t* send, int nstruct, double* out,
                           hipStream_t st);
 void launch_gather(const DevModel& m, const DevSystem& s, hipStream_t st);
+// radial-only style: features into D, energy and H = dE/dd, pair forces
+void launch_pair_features(const DevModel& m, const DevSystem& s, hipStream_t st);
+void launch_pair_poly(const DevModel& m, const DevSystem& s, hipStream_t st);
+void launch_pair_force(const DevModel& m, const DevSystem& s, hipStream_t st);
 // per-slot energies; with totals, also the fixed-order whole-system sums
 void launch_energy_virial(const DevModel& m, const DevSystem& s, bool totals, hipStream_t st);
 // plain streaming copy used to calibrate hardware byte counters

@@ -51,6 +51,9 @@ bool upload_model(mlip_ctx* c) {
     dm.n_p = m.n_p;
     dm.has_q34 = m.has_order34 ? 1 : 0;
     dm.q34_red = m.q34_red;
+    dm.pair_model = m.pair_model ? 1 : 0;
+    for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < 2; ++b) dm.pair_tc[a][b] = m.pair_tc[a][b];
     dm.q34_dyn_adj = m.q34_dyn_adj ? 1 : 0;
     dm.q34_dyn_feat = m.q34_dyn_feat ? 1 : 0;
     dm.cutoff = m.cutoff;
@@ -116,6 +119,15 @@ bool upload_model(mlip_ctx* c) {
               upload(c, c->d_gadj, gadj.data(), gadj.size(), "adjoint groups") &&
               upload(c, c->d_gfeat, gfeat.data(), gfeat.size(), "feature groups");
     if (!ok) return false;
+    if (m.pair_model) {
+        if (!upload(c, c->d_pairw, m.weight.data(), static_cast<size_t>(m.n_des), "pair weights") ||
+            !upload(c, c->d_phoff, m.ph_off.data(), m.ph_off.size(), "pair derivative offsets") ||
+            !upload(c, c->d_phent, m.ph_ent.data(), m.ph_ent.size(), "pair derivative lists"))
+            return false;
+        dm.pair_w = c->d_pairw.p;
+        dm.ph_off = c->d_phoff.p;
+        dm.ph_ent = c->d_phent.p;
+    }
     {
         YTab<double> t64;
         YTab<float> t32;
@@ -311,6 +323,19 @@ bool run_pipeline(mlip_ctx* c) {
     const DevSystem& s = c->ds;
     const Model& mm = c->model;
     hipStream_t st = c->stream;
+    if (mm.pair_model) {
+        // radial-only style: features, polynomial and H, pair forces
+        { Scope sc(c, "pair_features"); launch_pair_features(m, s, st); }
+        { Scope sc(c, "pair_poly"); launch_pair_poly(m, s, st); }
+        { Scope sc(c, "force"); launch_pair_force(m, s, st); }
+        { Scope sc(c, "gather"); launch_gather(m, s, st); }
+        { Scope sc(c, "energy_virial"); launch_energy_virial(m, s, c->total_reduce, st); }
+        if (c->nstruct > 0) {
+            Scope sc(c, "struct_reduce");
+            launch_struct_reduce(s, c->sbeg, c->send, c->nstruct, c->sres, st);
+        }
+        return hip_ok(c, hipGetLastError(), "kernel launch");
+    }
     { Scope sc(c, "anlm"); launch_anlm(m, s, c->precision, st); }
     if (m.n_p > 0) {
         { Scope sc(c, "q12_feat"); launch_q12(m, s, true, st); }
@@ -594,6 +619,9 @@ void mlip_destroy(mlip_ctx* c) {
     c->d_gfeat.release();
     c->d_yt64.release();
     c->d_yt32.release();
+    c->d_pairw.release();
+    c->d_phoff.release();
+    c->d_phent.release();
     drop_graph(c);
     pool_free(c);
     c->soff.release();
@@ -993,6 +1021,17 @@ int mlip_flops_reference(const mlip_ctx* c, double* out7) {
     for (int k = 0; k < 7; ++k) out7[k] = 0.0;
     const double half = 0.5 * static_cast<double>(c->npairs);
     const double nl = m.n_lmh, nf = m.n_fn;
+    if (m.pair_model) {
+        // radial-only style: geometry and radial functions, two feature
+        // accumulations and the pair force per half pair, 2 sum|p| per atom
+        double na = 0.0;
+        for (int64_t k : c->owners_per_type) na += static_cast<double>(k);
+        out7[0] = half * (40.0 + 12.0 * nf + 2.0 * nf);
+        out7[5] = half * 16.0 * nf;
+        out7[4] = na * 2.0 * (m.n_des + 2.0 * m.comb2.size() + 3.0 * m.comb3.size());
+        for (int k = 0; k < 6; ++k) out7[6] += out7[k];
+        return MLIP_OK;
+    }
     out7[0] = half * (40.0 + 12.0 * nf + 38.0 * nl + 8.0 * nf * nl);
     out7[5] = half * 44.0 * nf * nl;
     std::vector<char> in_poly(m.n_gtinv, 0);

@@ -64,6 +64,9 @@ struct mlip_ctx {
     mlip::DBuf<mlip::GroupQ34> d_gadj, d_gfeat;
     mlip::DBuf<mlip::YTab<double>> d_yt64;
     mlip::DBuf<mlip::YTab<float>> d_yt32;
+    mlip::DBuf<double> d_pairw;
+    mlip::DBuf<int32_t> d_phoff;
+    mlip::DBuf<mlip::PairHEntry> d_phent;
     mlip::DevModel dm{};
     // structure buffers
     mlip::DBuf<double> x, pshift, A, G, D, Hq, H, Ep, Eq12, Eq34, gpair, vown, f, e_slot, red, out;

@@ -198,7 +198,9 @@ __global__ __launch_bounds__(256) void k_energy_slot(DevModel M, DevSystem S) {
     const int slot = blockIdx.x * blockDim.x + threadIdx.x;
     if (slot >= S.nblocks * kBlock) return;
     double e = 0.0;
-    if (S.atom_of_slot[slot] >= 0) {
+    if (S.atom_of_slot[slot] >= 0 && M.pair_model) {
+        e = S.Ep[slot];
+    } else if (S.atom_of_slot[slot] >= 0) {
         e = M.n_p > 0 ? S.Ep[slot] : 0.0;
         for (int n = 0; n < M.n_fn; ++n) {
             e += S.Eq12[(size_t)slot * M.n_fn + n];

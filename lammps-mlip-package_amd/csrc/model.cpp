@@ -403,6 +403,150 @@ void build_type_tables(Model& m, int t, const std::vector<std::map<std::vector<i
 
 }  // namespace
 
+// Radial-only style (des_type pair; mlip_model.h:45-101 without the gtinv
+// lines, ModelParams mlip_model_params.cpp:27-82,172-221, PolynomialPair
+// mlip_polynomial_pair.cpp:29-92): features d_{tc,n} = sum over neighbors of
+// type combination tc of g_n(r); polynomial terms as the reference orders
+// its coefficients.
+bool load_pair_model(std::istream& in, Model& m, std::string& err, bool strict_count) {
+    m.pair_model = true;
+    if (m.model_type < 1 || m.model_type > 4 || m.maxp < 1 || m.maxp > 3) {
+        err = "unsupported model_type/max_p";
+        return false;
+    }
+    if (!(m.cutoff > 0.0)) {
+        err = "cutoff must be positive";
+        return false;
+    }
+    int nreg = 0;
+    if (!first_token(in, nreg) || nreg <= 0) {
+        err = "bad number of regression coefficients";
+        return false;
+    }
+    if (!n_tokens(in, m.coeff_file, nreg) || !n_tokens(in, m.scale_file, nreg)) {
+        err = "truncated coefficient or scale line";
+        return false;
+    }
+    int npar = 0;
+    if (!first_token(in, npar) || npar <= 0 || npar > kMaxFnPair) {
+        err = "number of radial functions must be in [1,32]";
+        return false;
+    }
+    m.beta.resize(npar);
+    m.mu.resize(npar);
+    for (int i = 0; i < npar; ++i) {
+        std::vector<double> p;
+        if (!n_tokens(in, p, 2)) {
+            err = "truncated radial parameters";
+            return false;
+        }
+        m.beta[i] = p[0];
+        m.mu[i] = p[1];
+    }
+    if (!n_tokens(in, m.masses, m.n_type)) {
+        err = "truncated mass line";
+        return false;
+    }
+    if (m.n_type < 1 || m.n_type > 2) {
+        err = "only one or two elements are supported (reference ModelParams)";
+        return false;
+    }
+    m.n_fn = npar;
+    m.tcp_other = m.n_type == 1 ? std::vector<std::array<int, 2>>{{0, -1}}
+                                : std::vector<std::array<int, 2>>{{0, -1}, {1, 0}, {-1, 1}};
+    m.n_tc = static_cast<int>(m.tcp_other.size());
+    for (int t1 = 0; t1 < m.n_type; ++t1)
+        for (int t2 = 0; t2 < m.n_type; ++t2)
+            for (int tc = 0; tc < m.n_tc; ++tc)
+                if (m.tcp_other[tc][t1] == t2) {
+                    m.pair_tc[t1][t2] = tc;
+                    break;
+                }
+    m.n_des = m.n_fn * m.n_tc;
+    m.n_gtinv = 0;
+    m.comb2.clear();
+    m.comb3.clear();
+    // a type combination is seen by centre type t1 if it lists a partner
+    auto seen = [&](int tc, int t1) { return m.tcp_other[tc][t1] >= 0; };
+    if (m.model_type == 1) {
+        for (int f = 0; f < m.n_des; ++f) {
+            if (m.maxp > 1) m.comb2.push_back({f, f});
+            if (m.maxp > 2) m.comb3.push_back({f, f, f});
+        }
+        m.n_coeff = m.n_des * m.maxp;
+    } else {
+        // model_type 2 uses every feature; 3 and 4 none for this style
+        std::vector<int> pidx;
+        if (m.model_type == 2)
+            for (int f = 0; f < m.n_des; ++f) pidx.push_back(f);
+        auto tcof = [&](int f) { return f / m.n_fn; };
+        if (m.maxp > 1)
+            for (size_t i1 = 0; i1 < pidx.size(); ++i1)
+                for (size_t i2 = 0; i2 <= i1; ++i2)
+                    for (int t1 = 0; t1 < m.n_type; ++t1)
+                        if (seen(tcof(pidx[i1]), t1) && seen(tcof(pidx[i2]), t1)) {
+                            m.comb2.push_back({pidx[i2], pidx[i1]});
+                            break;
+                        }
+        if (m.maxp > 2)
+            for (size_t i1 = 0; i1 < pidx.size(); ++i1)
+                for (size_t i2 = 0; i2 <= i1; ++i2)
+                    for (size_t i3 = 0; i3 <= i2; ++i3)
+                        for (int t1 = 0; t1 < m.n_type; ++t1)
+                            if (seen(tcof(pidx[i1]), t1) && seen(tcof(pidx[i2]), t1) && seen(tcof(pidx[i3]), t1)) {
+                                m.comb3.push_back({pidx[i3], pidx[i2], pidx[i1]});
+                                break;
+                            }
+        m.n_coeff = m.n_des + static_cast<int>(m.comb2.size() + m.comb3.size());
+    }
+    if (!strict_count) return true;
+    if (m.n_coeff != nreg) {
+        char buf[200];
+        std::snprintf(buf, sizeof buf, "potential lists %d coefficients, the model defines %d", nreg, m.n_coeff);
+        err = buf;
+        return false;
+    }
+    m.weight.resize(nreg);
+    for (int i = 0; i < nreg; ++i) {
+        if (m.scale_file[i] == 0.0) {
+            err = "zero scale coefficient";
+            return false;
+        }
+        m.weight[i] = m.coeff_file[i] / m.scale_file[i];
+    }
+    // per-feature derivative lists: every position of every product term
+    std::vector<std::vector<PairHEntry>> lists(m.n_des);
+    for (size_t k = 0; k < m.comb2.size(); ++k) {
+        const double w = m.weight[m.n_des + k];
+        const auto& c = m.comb2[k];
+        lists[c[0]].push_back({c[1], -1, w, w / 2.0});
+        lists[c[1]].push_back({c[0], -1, w, w / 2.0});
+    }
+    for (size_t k = 0; k < m.comb3.size(); ++k) {
+        const double w = m.weight[m.n_des + m.comb2.size() + k];
+        const auto& c = m.comb3[k];
+        lists[c[0]].push_back({c[1], c[2], w, w / 3.0});
+        lists[c[1]].push_back({c[0], c[2], w, w / 3.0});
+        lists[c[2]].push_back({c[0], c[1], w, w / 3.0});
+    }
+    m.ph_off.assign(m.n_des + 1, 0);
+    m.ph_ent.clear();
+    for (int a = 0; a < m.n_des; ++a) {
+        m.ph_ent.insert(m.ph_ent.end(), lists[a].begin(), lists[a].end());
+        m.ph_off[a + 1] = static_cast<int32_t>(m.ph_ent.size());
+    }
+    // device layout: the feature rows reuse the polynomial-feature buffers
+    m.n_p = m.n_des;
+    m.maxl = 0;
+    m.n_lmh = 1;
+    m.n_lmh3 = 1;
+    m.lmax3 = 0;
+    m.has_order34 = false;
+    m.has_feat34 = false;
+    m.types.assign(m.n_type, TypeTables());
+    return true;
+}
+
 bool load_model(const std::string& path, const GtinvTable& table, Model& m, std::string& err, bool strict_count) {
     std::ifstream in(path);
     if (!in) {
@@ -437,8 +581,9 @@ bool load_model(const std::string& path, const GtinvTable& table, Model& m, std:
         err = "unknown pair_type " + pair_type + " (only gaussian)";
         return false;
     }
+    if (des_type == "pair") return load_pair_model(in, m, err, strict_count);
     if (des_type != "gtinv") {
-        err = "des_type must be gtinv";
+        err = "des_type must be gtinv or pair";
         return false;
     }
     if (m.model_type < 1 || m.model_type > 4 || m.maxp < 1 || m.maxp > 3) {

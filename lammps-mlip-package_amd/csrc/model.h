@@ -87,6 +87,16 @@ struct GroupQ34 {
     int32_t red;       // partial-row offset of this sub-group (split rows only)
 };
 
+// Radial-only model (des_type pair): one entry of the per-feature
+// derivative lists, H_a = w_a + sum_entries w prod(d_b, d_c) (b, c = -1 for
+// absent factors); ew = w / order is the Euler energy share of the term.
+struct PairHEntry {
+    int32_t b, c;
+    double w, ew;
+};
+
+constexpr int kMaxFnPair = 32;  // radial functions of the radial-only style
+
 struct TypeTables {
     // order 1/2: per L, range into visits_q12
     std::vector<int32_t> q12_off;      // size maxl+2
@@ -129,6 +139,11 @@ struct Model {
     std::vector<TypeTables> types;
     bool has_order34 = false;
     bool has_feat34 = false;
+    // radial-only style (des_type pair): features f = tc * n_fn + n
+    bool pair_model = false;
+    std::array<std::array<int, 2>, 2> pair_tc{};   // [centre type][neighbor type] -> tc
+    std::vector<int32_t> ph_off;                   // n_des + 1
+    std::vector<PairHEntry> ph_ent;
     int q34_red = 0;           // doubles of LDS partial rows (adjoint pass), max over types
     bool q34_dyn_adj = false;  // visits outside the unrolled signatures (general loop kernel)
     bool q34_dyn_feat = false;

@@ -1,9 +1,12 @@
-/* LAMMPS pair style `mlip_gtinv/hip`: the reference `mlip_gtinv` contract
-   (src/USER-MLIP/pair_mlip_gtinv.h) served by the HIP engine through the C ABI
-   of include/mlip_hip.h.  Built only inside a LAMMPS source tree. */
+/* LAMMPS pair styles `mlip_gtinv/hip` and `mlip_pair/hip`: the reference
+   `mlip_gtinv` and `mlip_pair` contracts (src/USER-MLIP/pair_mlip_gtinv.h,
+   pair_mlip_pair.h) served by the HIP engine through the C ABI of
+   include/mlip_hip.h; the engine reads the style from the potential file's
+   des_type line.  Built only inside a LAMMPS source tree. */
 #ifdef PAIR_CLASS
 // clang-format off
 PairStyle(mlip_gtinv/hip, PairMLIPGtinvHIP)
+PairStyle(mlip_pair/hip, PairMLIPGtinvHIP)
 // clang-format on
 #else
 

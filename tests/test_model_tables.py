@@ -54,3 +54,17 @@ def test_bad_potential_files(tmp_path):
         inspect_model(str(p), 0)
     with pytest.raises(MlipError):
         inspect_model(str(tmp_path / "missing.mlip"), 0)
+
+
+@pytest.mark.parametrize("name", ["pair-17", "pair-60"])
+def test_pair_style_tables_match_oracle(name):
+    """Radial-only style: feature count and the polynomial terms in
+    coefficient order, bit-exact against the oracle's enumeration."""
+    import oracle_lib as O
+    from mlip_hip import inspect_model
+    pot = ref(name + ".mlip")
+    o = O.Oracle(pot)
+    info = inspect_model(pot, 0)
+    assert info["n_des"] == o.info["n_des"] and info["n_coeff"] == o.info["n_coeff"]
+    for order in (2, 3):
+        assert np.array_equal(inspect_model(pot, order), o.combs(order))
