@@ -91,6 +91,24 @@ int mlip_resident_compute(mlip_ctx* ctx);
 int mlip_resident_sync(mlip_ctx* ctx);
 int mlip_resident_get(mlip_ctx* ctx, double* f, double* energy, double* virial6, double* eatom);
 
+/* Resident structure from a periodic cell (cell9 = rows a, b, c): the full
+ * neighbor list (0 < r < cutoff, self-images, image shifts), the force
+ * gather lists and the slot layout are built on the device.  The list is
+ * the one mlip_nlist_build gives, entry for entry and bit for bit, so the
+ * evaluation equals mlip_resident_set with that list.  Replaces the
+ * neighbor build LAMMPS runs before PairMLIPGtinv::compute (full list,
+ * newton on, as requested at pair_mlip_gtinv_kokkos_impl.h:229-270; binning
+ * as src/USER-MLIP/mlipkk_cell.cpp:218-307). */
+int mlip_resident_set_cell(mlip_ctx* ctx, int natoms, const double* cell9, const double* x, const int* type);
+/* Number of directed pairs of the resident structure or batch (-1 if none). */
+int64_t mlip_resident_npairs(const mlip_ctx* ctx);
+/* Copies the resident lists to host: off[natoms+1], idx[npairs],
+ * shift[3 npairs] (del = x_i - x_j + shift), and the force gather lists
+ * goff[natoms+1], gent[2 npairs] (for position k, entries -(p+1) for pairs
+ * centred on k and p+1 for pairs with neighbor k, in increasing p).  Any
+ * pointer may be NULL. */
+int mlip_resident_lists(mlip_ctx* ctx, int64_t* off, int* idx, double* shift, int* goff, int* gent);
+
 /* ---- Batched evaluation (BASELINE batched configuration; no reference
  * counterpart).  nstruct independent periodic structures: natoms[k] atoms
  * with cell cells[9k..9k+8] (rows a, b, c), positions x and 1-based types
@@ -102,6 +120,10 @@ int mlip_resident_get(mlip_ctx* ctx, double* f, double* energy, double* virial6,
 int mlip_batch_set(mlip_ctx* ctx, int nstruct, const int* natoms, const double* cells, const double* x,
                    const int* type);
 int mlip_batch_get(mlip_ctx* ctx, double* energies, double* virials, double* f);
+/* Where mlip_batch_set builds the neighbor lists: 1 on the device (the
+ * default; lists as mlip_resident_set_cell builds them), 0 on host threads
+ * with the mlip_nlist_build builder.  Both give identical lists. */
+int mlip_set_neighbor_builder(mlip_ctx* ctx, int device);
 
 /* Synthetic batched/served structures: sizes log-uniform in [nmin, nmax],
  * random triclinic cells at `density` (atoms/A^3), uniform positions with an

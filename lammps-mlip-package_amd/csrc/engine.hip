@@ -6,6 +6,8 @@
 
 #include <algorithm>
 #include <atomic>
+#include <climits>
+#include <cmath>
 #include <thread>
 #include <cstdio>
 #include <cstring>
@@ -178,13 +180,10 @@ void assign_slots(const mlip_ctx* c, int nown, const std::vector<int>& elem_of_o
     }
 }
 
-// Allocate per-structure buffers and fill the DevSystem view.
-bool setup_system(mlip_ctx* c, int nall, int nown, int64_t npairs, const std::vector<double>& x,
-                  const std::vector<int32_t>& elem, const std::vector<int32_t>& elem_own,
-                  const std::vector<int32_t>& poff,
-                  const std::vector<int32_t>& pnbr, const std::vector<int32_t>* pcen,
-                  const std::vector<double>* pshift, const std::vector<int32_t>& goff,
-                  const std::vector<int32_t>& gent) {
+// Slot layout, work buffers and the DevSystem view of a structure whose
+// positions, elements, pair lists and gather lists are already on the device.
+bool setup_layout(mlip_ctx* c, int nall, int nown, int64_t npairs, const std::vector<int32_t>& elem_own, bool has_pcen,
+                  bool has_shift) {
     const Model& m = c->model;
     drop_graph(c);
     c->nstruct = 0;
@@ -199,18 +198,10 @@ bool setup_system(mlip_ctx* c, int nall, int nown, int64_t npairs, const std::ve
     c->npairs = npairs;
     const size_t nslot = static_cast<size_t>(nblocks) * kBlock;
     const size_t asz = nslot * m.n_fn * m.n_type * m.n_lmh * 2;
-    bool ok = upload(c, c->x, x.data(), x.size(), "positions") &&
-              upload(c, c->elem, elem.data(), elem.size(), "elements") &&
-              upload(c, c->poff, poff.data(), poff.size(), "pair offsets") &&
-              upload(c, c->pnbr, pnbr.data(), pnbr.size(), "pair neighbors") &&
-              upload(c, c->atom_of_slot, atom_of_slot.data(), atom_of_slot.size(), "slots") &&
+    bool ok = upload(c, c->atom_of_slot, atom_of_slot.data(), atom_of_slot.size(), "slots") &&
               upload(c, c->slot_of_atom, slot_of_atom.data(), slot_of_atom.size(), "slots") &&
-              upload(c, c->block_type, block_type.data(), block_type.size(), "block types") &&
-              upload(c, c->goff, goff.data(), goff.size(), "gather offsets") &&
-              upload(c, c->gent, gent.data(), gent.size(), "gather entries");
+              upload(c, c->block_type, block_type.data(), block_type.size(), "block types");
     if (!ok) return false;
-    if (pcen && !upload(c, c->pcen, pcen->data(), pcen->size(), "pair centres")) return false;
-    if (pshift && !upload(c, c->pshift, pshift->data(), pshift->size(), "pair shifts")) return false;
     const size_t np = static_cast<size_t>(m.n_p);
     if (!c->A.ensure(asz) || !c->G.ensure(asz) || !c->D.ensure(nslot * np) || !c->Hq.ensure(nslot * np) ||
         !c->H.ensure(nslot * np) || !c->Ep.ensure(nslot) || !c->Eq12.ensure(nslot * m.n_fn) ||
@@ -234,8 +225,8 @@ bool setup_system(mlip_ctx* c, int nall, int nown, int64_t npairs, const std::ve
     s.poff = c->poff.p;
     s.pend = c->poff.p + 1;
     s.pnbr = c->pnbr.p;
-    s.pcen = pcen ? c->pcen.p : nullptr;
-    s.pshift = pshift ? c->pshift.p : nullptr;
+    s.pcen = has_pcen ? c->pcen.p : nullptr;
+    s.pshift = has_shift ? c->pshift.p : nullptr;
     s.atom_of_slot = c->atom_of_slot.p;
     s.slot_of_atom = c->slot_of_atom.p;
     s.block_type = c->block_type.p;
@@ -257,6 +248,66 @@ bool setup_system(mlip_ctx* c, int nall, int nown, int64_t npairs, const std::ve
     s.red = c->red.p;
     s.out = c->out.p;
     return true;
+}
+
+// Upload a structure with host-built lists and lay it out.
+bool setup_system(mlip_ctx* c, int nall, int nown, int64_t npairs, const std::vector<double>& x,
+                  const std::vector<int32_t>& elem, const std::vector<int32_t>& elem_own,
+                  const std::vector<int32_t>& poff,
+                  const std::vector<int32_t>& pnbr, const std::vector<int32_t>* pcen,
+                  const std::vector<double>* pshift, const std::vector<int32_t>& goff,
+                  const std::vector<int32_t>& gent) {
+    bool ok = upload(c, c->x, x.data(), x.size(), "positions") &&
+              upload(c, c->elem, elem.data(), elem.size(), "elements") &&
+              upload(c, c->poff, poff.data(), poff.size(), "pair offsets") &&
+              upload(c, c->pnbr, pnbr.data(), pnbr.size(), "pair neighbors") &&
+              upload(c, c->goff, goff.data(), goff.size(), "gather offsets") &&
+              upload(c, c->gent, gent.data(), gent.size(), "gather entries");
+    if (!ok) return false;
+    if (pcen && !upload(c, c->pcen, pcen->data(), pcen->size(), "pair centres")) return false;
+    if (pshift && !upload(c, c->pshift, pshift->data(), pshift->size(), "pair shifts")) return false;
+    return setup_layout(c, nall, nown, npairs, elem_own, pcen != nullptr, pshift != nullptr);
+}
+
+// Structures from periodic cells with their lists built on the device
+// (positions concatenated, natoms[s] each; elem per atom).  Returns an
+// MLIP_* code: invalid cells or positions are argument errors.
+int setup_cells_device(mlip_ctx* c, int nstruct, const int32_t* natoms, const double* cells, const double* x,
+                       const std::vector<int32_t>& elem) {
+    const size_t n = elem.size();
+    for (size_t i = 0; i < 3 * n; ++i)
+        if (!std::isfinite(x[i])) {
+            c->err = "non-finite position";
+            return MLIP_ERR_ARG;
+        }
+    for (int k = 0; k < nstruct; ++k) {
+        CellBins cb;
+        std::string e;
+        if (natoms[k] > 0 && !cell_bins(cells + 9 * static_cast<size_t>(k), c->model.cutoff, cb, e)) {
+            c->err = "structure " + std::to_string(k) + ": " + e;
+            return MLIP_ERR_ARG;
+        }
+    }
+    int64_t npairs = 0;
+    if (!upload(c, c->x, x, 3 * n, "positions") || !upload(c, c->elem, elem.data(), n, "elements") ||
+        !device_lists(c, nstruct, natoms, cells, c->model.cutoff, npairs))
+        return MLIP_ERR_DEVICE;
+    return setup_layout(c, static_cast<int>(n), static_cast<int>(n), npairs, elem, false, true) ? MLIP_OK
+                                                                                                  : MLIP_ERR_DEVICE;
+}
+
+// Per-structure atom ranges and result slots of a batch; synchronises.
+int finish_batch(mlip_ctx* c, int nstruct, const std::vector<int32_t>& soff, int n) {
+    if (!upload(c, c->soff, soff.data(), soff.size(), "structure offsets") || !c->sout.ensure(7 * nstruct))
+        return MLIP_ERR_DEVICE;
+    if (!hip_ok(c, hipStreamSynchronize(c->stream), "batch upload")) return MLIP_ERR_DEVICE;
+    c->nstruct = nstruct;
+    c->sbeg = c->soff.p;
+    c->send = c->soff.p + 1;
+    c->sres = c->sout.p;
+    c->resident = true;
+    c->resident_natoms = n;
+    return MLIP_OK;
 }
 
 }  // namespace
@@ -607,10 +658,11 @@ void mlip_destroy(mlip_ctx* c) {
     }
     DBuf<double>* dbs[] = {&c->d_coef, &c->d_weight, &c->d_wmat, &c->d_linp, &c->d_c3w, &c->x, &c->pshift,
                            &c->A, &c->G, &c->D, &c->Hq, &c->H, &c->Ep, &c->Eq12, &c->Eq34, &c->gpair,
-                           &c->vown, &c->f, &c->e_slot, &c->red, &c->out};
+                           &c->vown, &c->f, &c->e_slot, &c->red, &c->out, &c->sout};
     for (auto* b : dbs) b->release();
     DBuf<int32_t>* ibs[] = {&c->d_q12off, &c->d_c3off, &c->d_c3bc, &c->elem, &c->poff, &c->pnbr, &c->pcen,
-                            &c->atom_of_slot, &c->slot_of_atom, &c->block_type, &c->goff, &c->gent};
+                            &c->atom_of_slot, &c->slot_of_atom, &c->block_type, &c->goff, &c->gent,
+                            &c->soff};
     for (auto* b : ibs) b->release();
     c->d_q12.release();
     c->d_vadj.release();
@@ -624,6 +676,7 @@ void mlip_destroy(mlip_ctx* c) {
     c->d_phent.release();
     drop_graph(c);
     pool_free(c);
+    nb_free(c);
     c->soff.release();
     c->sout.release();
     if (c->blas) rocblas_destroy_handle(c->blas);
@@ -669,8 +722,8 @@ int mlip_compute_flat(mlip_ctx* c, int nlocal, int nghost, const double* x, cons
 int mlip_resident_set(mlip_ctx* c, int natoms, const double* x, const int* type, int64_t npairs,
                       const int64_t* nbr_off, const int* nbr_idx, const double* nbr_shift) {
     if (!c || natoms < 0 || !x || !type || !nbr_off || (npairs > 0 && !nbr_idx)) return MLIP_ERR_ARG;
-    if (npairs > INT32_MAX) {
-        c->err = "more than 2^31 pairs";
+    if (npairs > (INT32_MAX - 1) / 2) {
+        c->err = "more than 2^30 pairs";  // gather offsets reach 2 * npairs
         return MLIP_ERR_ARG;
     }
     std::vector<int32_t> elem(natoms), poff(natoms + 1), pnbr(nbr_idx, nbr_idx + npairs);
@@ -778,6 +831,11 @@ int mlip_batch_set(mlip_ctx* c, int nstruct, const int* natoms, const double* ce
         }
         elem[i] = c->elem_of_type[t - 1];
     }
+    c->nstruct = 0;
+    if (c->nb_device) {
+        const int rc = setup_cells_device(c, nstruct, natoms, cells, x, elem);
+        return rc != MLIP_OK ? rc : finish_batch(c, nstruct, soff, n);
+    }
     // per-structure neighbor lists on host threads, concatenated in order
     std::vector<NeighborList> lists(nstruct);
     std::vector<std::string> errs(nstruct);
@@ -804,26 +862,79 @@ int mlip_batch_set(mlip_ctx* c, int nstruct, const int* natoms, const double* ce
         for (int i = 0; i < natoms[k]; ++i) poff[soff[k] + i + 1] = static_cast<int32_t>(pnbr.size() + nl.off[i + 1]);
         for (int32_t j : nl.idx) pnbr.push_back(soff[k] + j);
         sh.insert(sh.end(), nl.shift.begin(), nl.shift.end());
-        if (pnbr.size() > static_cast<size_t>(INT32_MAX)) {
-            c->err = "more than 2^31 pairs in the batch";
+        if (pnbr.size() > static_cast<size_t>((INT32_MAX - 1) / 2)) {
+            c->err = "more than 2^30 pairs in the batch";
             return MLIP_ERR_ARG;
         }
     }
     std::vector<int32_t> gof, gen;
     build_gather(n, poff, pnbr, nullptr, gof, gen);
     std::vector<double> xv(x, x + 3 * static_cast<size_t>(n));
-    c->nstruct = 0;
     if (!setup_system(c, n, n, static_cast<int64_t>(pnbr.size()), xv, elem, elem, poff, pnbr, nullptr, &sh, gof, gen))
         return MLIP_ERR_DEVICE;
-    if (!upload(c, c->soff, soff.data(), soff.size(), "structure offsets") || !c->sout.ensure(7 * nstruct))
-        return MLIP_ERR_DEVICE;
-    if (!hip_ok(c, hipStreamSynchronize(c->stream), "batch upload")) return MLIP_ERR_DEVICE;
-    c->nstruct = nstruct;
-    c->sbeg = c->soff.p;
-    c->send = c->soff.p + 1;
-    c->sres = c->sout.p;
+    return finish_batch(c, nstruct, soff, n);
+}
+
+int mlip_set_neighbor_builder(mlip_ctx* c, int device) {
+    if (!c || (device != 0 && device != 1)) return MLIP_ERR_ARG;
+    c->nb_device = device == 1;
+    return MLIP_OK;
+}
+
+int mlip_resident_set_cell(mlip_ctx* c, int natoms, const double* cell9, const double* x, const int* type) {
+    if (!c || natoms < 0 || !cell9 || (natoms > 0 && (!x || !type))) return MLIP_ERR_ARG;
+    std::vector<int32_t> elem(natoms);
+    for (int i = 0; i < natoms; ++i) {
+        const int t = type[i];
+        if (t < 1 || t > static_cast<int>(c->elem_of_type.size()) || c->elem_of_type[t - 1] < 0) {
+            c->err = "atom type without an element mapping";
+            return MLIP_ERR_ARG;
+        }
+        elem[i] = c->elem_of_type[t - 1];
+    }
+    c->resident = false;
+    c->nstruct = 0;
+    const int32_t na = natoms;
+    const int rc = setup_cells_device(c, 1, &na, cell9, x, elem);
+    if (rc != MLIP_OK) return rc;
+    if (!hip_ok(c, hipStreamSynchronize(c->stream), "resident build")) return MLIP_ERR_DEVICE;
     c->resident = true;
-    c->resident_natoms = n;
+    c->resident_natoms = natoms;
+    return MLIP_OK;
+}
+
+int64_t mlip_resident_npairs(const mlip_ctx* c) { return c && c->resident ? c->npairs : -1; }
+
+int mlip_resident_lists(mlip_ctx* c, int64_t* off, int* idx, double* shift, int* goff, int* gent) {
+    if (!c) return MLIP_ERR_ARG;
+    if (!c->resident) {
+        c->err = "no resident structure";
+        return MLIP_ERR_STATE;
+    }
+    const size_t n = static_cast<size_t>(c->resident_natoms), np = static_cast<size_t>(c->npairs);
+    hipStream_t st = c->stream;
+    std::vector<int32_t> o32(n + 1);
+    bool ok = hip_ok(c, hipMemcpyAsync(o32.data(), c->poff.p, (n + 1) * sizeof(int32_t), hipMemcpyDeviceToHost, st),
+                     "pair offsets");
+    if (ok && idx && np)
+        ok = hip_ok(c, hipMemcpyAsync(idx, c->pnbr.p, np * sizeof(int32_t), hipMemcpyDeviceToHost, st), "pairs");
+    if (ok && shift && np) {
+        if (!c->ds.pshift) {
+            c->err = "the resident lists carry no image shifts";
+            return MLIP_ERR_STATE;
+        }
+        ok = hip_ok(c, hipMemcpyAsync(shift, c->pshift.p, 3 * np * sizeof(double), hipMemcpyDeviceToHost, st),
+                    "shifts");
+    }
+    if (ok && goff)
+        ok = hip_ok(c, hipMemcpyAsync(goff, c->goff.p, (n + 1) * sizeof(int32_t), hipMemcpyDeviceToHost, st),
+                    "gather offsets");
+    if (ok && gent && np)
+        ok = hip_ok(c, hipMemcpyAsync(gent, c->gent.p, 2 * np * sizeof(int32_t), hipMemcpyDeviceToHost, st),
+                    "gather entries");
+    if (!ok || !hip_ok(c, hipStreamSynchronize(st), "list readback")) return MLIP_ERR_DEVICE;
+    if (off)
+        for (size_t i = 0; i <= n; ++i) off[i] = o32[i];
     return MLIP_OK;
 }
 

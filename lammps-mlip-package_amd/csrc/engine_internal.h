@@ -44,7 +44,8 @@ struct ProfEvent {
     hipEvent_t a, b;
 };
 
-struct Pool;  // device-resident request pool (pool.hip)
+struct Pool;    // device-resident request pool (pool.hip)
+struct NbWork;  // device neighbor-build scratch (nbuild.hip)
 
 }  // namespace mlip
 
@@ -103,6 +104,10 @@ struct mlip_ctx {
     hipGraphExec_t gexec = nullptr;
     // device-resident request pool, when initialised
     mlip::Pool* pool = nullptr;
+    // neighbor lists of mlip_batch_set: built on the device (default) or on
+    // host threads; scratch of the device builder
+    bool nb_device = true;
+    mlip::NbWork* nbw = nullptr;
 };
 
 namespace mlip {
@@ -126,5 +131,12 @@ void collect_profile(mlip_ctx* c);
 void build_gather(int nall, const std::vector<int32_t>& poff, const std::vector<int32_t>& pnbr,
                   const std::vector<int32_t>* pcen, std::vector<int32_t>& goff, std::vector<int32_t>& gent);
 void pool_free(mlip_ctx* c);
+// Full periodic lists (0 < r < rc, self-images, image shifts) of nstruct
+// structures whose positions (concatenated, natoms[s] each) are already in
+// c->x, built on the device into c->poff, c->pnbr, c->pshift, with the force
+// gather lists in c->goff, c->gent; identical to build_full_list followed by
+// build_gather.  Synchronises once to size the pair arrays.
+bool device_lists(mlip_ctx* c, int nstruct, const int32_t* natoms, const double* cells, double rc, int64_t& npairs);
+void nb_free(mlip_ctx* c);
 
 }  // namespace mlip

@@ -44,25 +44,20 @@ void cross3(const double* a, const double* b, double* o) {
 
 }  // namespace
 
-bool build_full_list(const double cell[9], int n, const double* x, double rc, NeighborList& out, std::string& err,
-                     int nthreads) {
-    out.off.assign(static_cast<size_t>(n) + 1, 0);
-    out.idx.clear();
-    out.shift.clear();
-    if (n == 0) return true;
+bool cell_bins(const double cell[9], double rc, CellBins& out, std::string& err) {
     if (!(rc > 0.0)) {
         err = "cutoff must be positive";
         return false;
     }
-    double g[9];
-    inv3(cell, g);
+    inv3(cell, out.g);
     const double vol = std::fabs(cell[0] * (cell[4] * cell[8] - cell[5] * cell[7]) -
                                  cell[1] * (cell[3] * cell[8] - cell[5] * cell[6]) +
                                  cell[2] * (cell[3] * cell[7] - cell[4] * cell[6]));
-    if (!(vol > 0.0)) {
+    if (!(vol > 0.0) || !std::isfinite(vol)) {
         err = "degenerate cell";
         return false;
     }
+    out.vol = vol;
     // face-to-face widths
     double w[3], cr[3];
     cross3(cell + 3, cell + 6, cr);
@@ -71,6 +66,38 @@ bool build_full_list(const double cell[9], int n, const double* x, double rc, Ne
     w[1] = vol / norm3(cr);
     cross3(cell + 0, cell + 3, cr);
     w[2] = vol / norm3(cr);
+    // bins along each lattice direction at least rc/2 wide (fewer distance
+    // tests than rc-wide bins: a (5/2)^3 instead of 3^3 stencil volume);
+    // the stencil reach in bins covers rc even with 1-2 bins
+    int64_t nbins = 1;
+    for (int k = 0; k < 3; ++k) {
+        const double q = std::floor(2.0 * w[k] / rc);
+        out.nb[k] = q >= 1.0 ? (q < 1 << 20 ? static_cast<int>(q) : 1 << 20) : 1;
+        out.reach[k] = static_cast<int>(std::ceil(rc / (w[k] / out.nb[k])));
+        nbins *= out.nb[k];
+    }
+    if (nbins > kMaxCellBins) {
+        err = "cell too large for the bin grid (more than 2^28 bins)";
+        return false;
+    }
+    return true;
+}
+
+bool build_full_list(const double cell[9], int n, const double* x, double rc, NeighborList& out, std::string& err,
+                     int nthreads) {
+    out.off.assign(static_cast<size_t>(n) + 1, 0);
+    out.idx.clear();
+    out.shift.clear();
+    if (n == 0) return true;
+    CellBins cb;
+    if (!cell_bins(cell, rc, cb, err)) return false;
+    for (int i = 0; i < 3 * n; ++i)
+        if (!std::isfinite(x[i])) {
+            err = "non-finite position";
+            return false;
+        }
+    const double* g = cb.g;
+    const double vol = cb.vol;
     // fractional coordinates wrapped into [0,1)
     std::vector<double> f(3 * static_cast<size_t>(n));
     for (int i = 0; i < n; ++i) {
@@ -82,10 +109,7 @@ bool build_full_list(const double cell[9], int n, const double* x, double rc, Ne
             f[3 * i + k] = v;
         }
     }
-    // bins along each lattice direction at least rc/2 wide (fewer distance
-    // tests than rc-wide bins: a (5/2)^3 instead of 3^3 stencil volume)
-    int nb[3];
-    for (int k = 0; k < 3; ++k) nb[k] = std::max(1, static_cast<int>(std::floor(2.0 * w[k] / rc)));
+    const int* nb = cb.nb;
     const int nbins = nb[0] * nb[1] * nb[2];
     std::vector<int> bin_of(n), head(nbins + 1, 0), members(n);
     for (int i = 0; i < n; ++i) {
@@ -99,9 +123,7 @@ bool build_full_list(const double cell[9], int n, const double* x, double rc, Ne
         std::vector<int> fill(head.begin(), head.end() - 1);
         for (int i = 0; i < n; ++i) members[fill[bin_of[i]]++] = i;
     }
-    // stencil reach in bins per direction (covers rc even with 1-2 bins)
-    int reach[3];
-    for (int k = 0; k < 3; ++k) reach[k] = static_cast<int>(std::ceil(rc / (w[k] / nb[k])));
+    const int* reach = cb.reach;
     const double rc2 = rc * rc;
     // positions reconstructed from wrapped fractional coordinates
     std::vector<double> xw(3 * static_cast<size_t>(n));

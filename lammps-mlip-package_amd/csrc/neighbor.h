@@ -15,6 +15,19 @@ struct NeighborList {
     std::vector<double> wrap_shift;
 };
 
+// Bin grid of a periodic cell for cutoff rc: inverse cell, bins per lattice
+// direction (each at least rc/2 wide across its faces) and the stencil
+// reach in bins.  Shared by the host builder and the device builder, so both
+// bin and wrap identically.
+constexpr int64_t kMaxCellBins = int64_t(1) << 28;
+struct CellBins {
+    double g[9];     // inverse of the column matrix [a b c]
+    double vol;
+    int nb[3];
+    int reach[3];
+};
+bool cell_bins(const double cell[9], double rc, CellBins& out, std::string& err);
+
 // cell: rows a, b, c.  Pairs with 0 < r < rc, self-images included; each
 // atom's neighbors in a fixed order whatever nthreads (atoms are split
 // into contiguous ranges across threads for n >= 512).

@@ -57,6 +57,11 @@ def library():
     L.mlip_resident_compute.argtypes = [vp]
     L.mlip_resident_sync.argtypes = [vp]
     L.mlip_resident_get.argtypes = [vp, dp, dp, dp, dp]
+    L.mlip_resident_set_cell.argtypes = [vp, c_int, dp, dp, ip]
+    L.mlip_resident_npairs.argtypes = [vp]
+    L.mlip_resident_npairs.restype = ctypes.c_int64
+    L.mlip_resident_lists.argtypes = [vp, i64p, ip, dp, ip, ip]
+    L.mlip_set_neighbor_builder.argtypes = [vp, c_int]
     L.mlip_profile_enable.argtypes = [vp, c_int]
     L.mlip_profile_read.argtypes = [vp, ctypes.POINTER(c_char_p), dp, ip, c_int]
     L.mlip_flops.argtypes = [vp, dp]

@@ -164,6 +164,21 @@ class _ResidentBase:
         keys = ["anlm", "q12", "q34_feat", "q34_adj", "poly", "force", "total"]
         return dict(zip(keys, out.tolist()))
 
+    def lists(self):
+        """The resident neighbor and force-gather lists as built on the
+        device: off, idx, shift (del = x_i - x_j + shift), goff, gent."""
+        n = self.n if hasattr(self, "n") else self.natoms
+        np_ = int(self.lib.mlip_resident_npairs(self.pair.ctx))
+        off = np.zeros(n + 1, dtype=np.int64)
+        idx = np.zeros(max(np_, 1), dtype=np.int32)
+        sh = np.zeros((max(np_, 1), 3))
+        goff = np.zeros(n + 1, dtype=np.int32)
+        gent = np.zeros(max(2 * np_, 1), dtype=np.int32)
+        self.pair._check(self.lib.mlip_resident_lists(self.pair.ctx, _p(off, ctypes.c_int64), _p(idx, ctypes.c_int),
+                                                      _p(sh, ctypes.c_double), _p(goff, ctypes.c_int),
+                                                      _p(gent, ctypes.c_int)))
+        return {"off": off, "idx": idx[:np_], "shift": sh[:np_], "goff": goff, "gent": gent[:2 * np_]}
+
     def calibration_copy(self, nbytes, reps=1):
         self.pair._check(self.lib.mlip_calibration_copy(self.pair.ctx, int(nbytes), int(reps)))
 
@@ -172,9 +187,14 @@ class BatchSystem(_ResidentBase):
     """Batch of independent periodic structures evaluated together on the
     device (structures: list of workloads.Structure)."""
 
-    def __init__(self, pair, structures):
+    def __init__(self, pair, structures, builder="device"):
+        """builder: where the per-structure neighbor lists are built,
+        "device" (HIP kernels) or "host" (threads; identical lists)."""
         self.pair = pair
         self.lib = pair.lib
+        if builder not in ("device", "host"):
+            raise ValueError("builder must be 'device' or 'host'")
+        pair._check(self.lib.mlip_set_neighbor_builder(pair.ctx, 1 if builder == "device" else 0))
         self.nstruct = len(structures)
         nat = np.array([s.natoms for s in structures], dtype=np.int32)
         cells = np.ascontiguousarray(np.concatenate([s.cell.reshape(9) for s in structures]), dtype=np.float64)
@@ -199,7 +219,24 @@ class BatchSystem(_ResidentBase):
 
 class ResidentSystem(_ResidentBase):
     """Device-resident structure: positions, types and a full neighbor list
-    of owned atoms with image shifts stay in HBM; compute() only enqueues."""
+    of owned atoms with image shifts stay in HBM; compute() only enqueues.
+    ResidentSystem.from_cell builds the list on the device."""
+
+    @classmethod
+    def from_cell(cls, pair, cell, x, types):
+        """Periodic cell (rows a, b, c): neighbor list, gather lists and
+        layout built on the device (mlip_resident_set_cell)."""
+        self = cls.__new__(cls)
+        self.pair = pair
+        self.lib = pair.lib
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        self.n = int(x.shape[0])
+        cell = np.ascontiguousarray(cell, dtype=np.float64).reshape(9)
+        t = np.ascontiguousarray(types, dtype=np.int32)
+        pair._check(self.lib.mlip_resident_set_cell(pair.ctx, self.n, _p(cell, ctypes.c_double),
+                                                    _p(x, ctypes.c_double), _p(t, ctypes.c_int)))
+        self.npairs = int(self.lib.mlip_resident_npairs(pair.ctx))
+        return self
 
     def __init__(self, pair, x, types, nbr_off, nbr_idx, nbr_shift=None):
         self.pair = pair
