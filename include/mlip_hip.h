@@ -120,6 +120,19 @@ int mlip_resident_lists(mlip_ctx* ctx, int64_t* off, int* idx, double* shift, in
 int mlip_batch_set(mlip_ctx* ctx, int nstruct, const int* natoms, const double* cells, const double* x,
                    const int* type);
 int mlip_batch_get(mlip_ctx* ctx, double* energies, double* virials, double* f);
+/* Training outputs (the design matrices of a fit) of the resident structure
+ * or batch, gtinv descriptor: replaces MLIPModel::prepare_features and its
+ * getters (src/USER-MLIP/mlipkk.cpp:461-470, 1318-1370; math
+ * docs/formulation.md:282-349).  Per atom (batch order):
+ * X[natoms][n_des] structural features, P[natoms][n_coeff] polynomial
+ * features, FF[natoms][n_coeff][3] force features (FF[k][p] = -dX_p/dr_k
+ * with X_p = sum_i P[i][p]); per structure SF[nstruct][n_coeff][6] stress
+ * features in the reference's order xx, yy, zz, yz, zx, xy.  With the
+ * regression weights w (coefficient / scale): sum_p w_p X_p is the energy,
+ * sum_p w_p FF[k][p] the force on k and sum_p w_p SF[p] the virial.  n_des
+ * and n_coeff are entries 3 and 4 of mlip_model_info.  FP64 whatever the
+ * precision mode; any output may be NULL. */
+int mlip_train_features(mlip_ctx* ctx, double* X, double* P, double* FF, double* SF);
 /* Where mlip_batch_set builds the neighbor lists: 1 on the device (the
  * default; lists as mlip_resident_set_cell builds them), 0 on host threads
  * with the mlip_nlist_build builder.  Both give identical lists. */

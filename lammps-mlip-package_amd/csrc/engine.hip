@@ -677,6 +677,7 @@ void mlip_destroy(mlip_ctx* c) {
     drop_graph(c);
     pool_free(c);
     nb_free(c);
+    train_free(c);
     c->soff.release();
     c->sout.release();
     if (c->blas) rocblas_destroy_handle(c->blas);
@@ -873,6 +874,11 @@ int mlip_batch_set(mlip_ctx* c, int nstruct, const int* natoms, const double* ce
     if (!setup_system(c, n, n, static_cast<int64_t>(pnbr.size()), xv, elem, elem, poff, pnbr, nullptr, &sh, gof, gen))
         return MLIP_ERR_DEVICE;
     return finish_batch(c, nstruct, soff, n);
+}
+
+int mlip_train_features(mlip_ctx* c, double* X, double* P, double* FF, double* SF) {
+    if (!c) return MLIP_ERR_ARG;
+    return train_features(c, X, P, FF, SF);
 }
 
 int mlip_set_neighbor_builder(mlip_ctx* c, int device) {

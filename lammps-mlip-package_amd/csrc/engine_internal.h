@@ -46,6 +46,7 @@ struct ProfEvent {
 
 struct Pool;    // device-resident request pool (pool.hip)
 struct NbWork;  // device neighbor-build scratch (nbuild.hip)
+struct TrainWork;  // training-feature tables and outputs (train.hip)
 
 }  // namespace mlip
 
@@ -108,6 +109,8 @@ struct mlip_ctx {
     // host threads; scratch of the device builder
     bool nb_device = true;
     mlip::NbWork* nbw = nullptr;
+    // training features (mlip_train_features), allocated on first use
+    mlip::TrainWork* trw = nullptr;
 };
 
 namespace mlip {
@@ -138,5 +141,9 @@ void pool_free(mlip_ctx* c);
 // build_gather.  Synchronises once to size the pair arrays.
 bool device_lists(mlip_ctx* c, int nstruct, const int32_t* natoms, const double* cells, double rc, int64_t& npairs);
 void nb_free(mlip_ctx* c);
+// Training outputs of the resident structure or batch (see
+// mlip_train_features); returns an MLIP_* code.
+int train_features(mlip_ctx* c, double* X, double* P, double* FF, double* SF);
+void train_free(mlip_ctx* c);
 
 }  // namespace mlip

@@ -62,6 +62,7 @@ def library():
     L.mlip_resident_npairs.restype = ctypes.c_int64
     L.mlip_resident_lists.argtypes = [vp, i64p, ip, dp, ip, ip]
     L.mlip_set_neighbor_builder.argtypes = [vp, c_int]
+    L.mlip_train_features.argtypes = [vp, dp, dp, dp, dp]
     L.mlip_profile_enable.argtypes = [vp, c_int]
     L.mlip_profile_read.argtypes = [vp, ctypes.POINTER(c_char_p), dp, ip, c_int]
     L.mlip_flops.argtypes = [vp, dp]

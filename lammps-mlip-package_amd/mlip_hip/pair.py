@@ -164,6 +164,23 @@ class _ResidentBase:
         keys = ["anlm", "q12", "q34_feat", "q34_adj", "poly", "force", "total"]
         return dict(zip(keys, out.tolist()))
 
+    def train_features(self, want=("X", "P", "FF", "SF")):
+        """Training outputs (mlip_train_features): X [natoms, n_des],
+        P [natoms, n_coeff], FF [natoms, n_coeff, 3] (= -dX_p/dr_k),
+        SF [nstruct, n_coeff, 6] (xx, yy, zz, yz, zx, xy)."""
+        info = self.pair.model_info()
+        n = self.n if hasattr(self, "n") else self.natoms
+        ns = getattr(self, "nstruct", 1)
+        nd, nc = info["n_des"], info["n_coeff"]
+        out = {"X": np.zeros((n, nd)) if "X" in want else None,
+               "P": np.zeros((n, nc)) if "P" in want else None,
+               "FF": np.zeros((n, nc, 3)) if "FF" in want else None,
+               "SF": np.zeros((ns, nc, 6)) if "SF" in want else None}
+        q = lambda a: _p(a, ctypes.c_double) if a is not None else _null()  # noqa: E731
+        self.pair._check(self.lib.mlip_train_features(self.pair.ctx, q(out["X"]), q(out["P"]), q(out["FF"]),
+                                                      q(out["SF"])))
+        return out
+
     def lists(self):
         """The resident neighbor and force-gather lists as built on the
         device: off, idx, shift (del = x_i - x_j + shift), goff, gent."""

@@ -897,6 +897,184 @@ void run_compute(const Oracle& o, int nlocal, const double* x, const int64_t* ta
     (void)nlocal;
 }
 
+// ---------------------------------------------------------------- training
+// MLIPModel::prepare_features (src/USER-MLIP/mlipkk.cpp:461-470): structural
+// features (:609-666), polynomial features (:1159-1183), the per-pair
+// feature derivatives dirj / djri (:992-1157) and the force and stress
+// features (:1185-1281), restated over the same half list the CPU style
+// uses and in its conventions (del = x_i - x_j, a = sum g Y(del) without
+// conjugation).  For half pair (i, j) and feature f:
+//   dirj_f = d d_f^(i) / d del_ij,   djri_f = d d_f^(j) / d del_ji,
+// each a sum over CG terms and over the factors mu whose type combination is
+// the pair's, of C * prod_{mu' != mu} a * grad phi_{n l_mu m_mu}; with
+// grad phi(-del) = -(-1)^l grad phi(del).  Per coefficient p (features
+// f_1..f_q, product of structural features) the pair's force term on i is
+//   t = sum_nu [ -prod_{nu' != nu} d^(i) dirj_{f_nu} + prod_{nu' != nu} d^(j) djri_{f_nu} ],
+// added to FF[i] and subtracted from FF[j], so FF[k][p] = -dX_p/dr_k with
+// X_p = sum_i P_i(p); the stress feature adds del_ij (x) t in the order
+// xx, yy, zz, yz, zx, xy of mlipkk.cpp:1255-1266 (del = x_i - x_j here,
+// the reference's displacement is x_j - x_i with the opposite sign in front).
+void run_train(const Oracle& o, int nlocal, const double* x, const int64_t* tag, const List& L, double* Xo,
+               double* Po, double* FFo, double* SFo) {
+    const Potential& p = o.pot;
+    const int inum = L.inum;
+    const int n_fn = p.n_fn, n_lm = static_cast<int>(p.lm_info.size());
+    const int n_lm_all = 2 * n_lm - p.maxl - 1, n_type = p.n_type;
+    const int n_des = p.n_des, n_gt = p.n_gtinv;
+    auto T = [&](int i) { return o.types_by_tag[tag[i] - 1]; };
+    auto X = [&](int i, int k) { return x[3 * i + k]; };
+    // a_nlm and all structural features through the evaluation restatement
+    std::vector<double> an(size_t(inum) * n_type * n_fn * n_lm_all * 2), feat(size_t(inum) * n_des);
+    std::vector<double> ftmp;
+    {
+        // forces of the evaluation are not needed; give it a scratch array
+        // sized for every slot the list refers to
+        int nall = nlocal;
+        for (int ii = 0; ii < inum; ++ii) {
+            const int i = L.ilist[ii];
+            nall = std::max(nall, i + 1);
+            for (int jj = 0; jj < L.numneigh[i]; ++jj) nall = std::max(nall, L.idx[L.off[ii] + jj] + 1);
+        }
+        ftmp.assign(3 * size_t(nall), 0.0);
+        ComputeOut out{ftmp.data(), 0.0, {0, 0, 0, 0, 0, 0}, nullptr, an.data(), feat.data()};
+        run_compute(o, nlocal, x, tag, L, out);
+    }
+    auto A = [&](int owner, int s, int n, int lm) {
+        const size_t k = ((size_t(owner) * n_type + s) * n_fn + n) * n_lm_all + lm;
+        return dc(an[2 * k], an[2 * k + 1]);
+    };
+    // features by owner (ilist order is not assumed)
+    std::vector<double> d(size_t(nlocal) * n_des, 0.0);
+    for (int ii = 0; ii < inum; ++ii) {
+        const int own = static_cast<int>(tag[L.ilist[ii]] - 1);
+        std::copy(feat.begin() + size_t(ii) * n_des, feat.begin() + size_t(ii + 1) * n_des, d.begin() + size_t(own) * n_des);
+    }
+    // coefficient rows in coefficient order: linear features, comb2, comb3
+    // (the polynomial terms' reg_i in set_polynomial_array)
+    vec2i rows;
+    for (int f = 0; f < n_des; ++f) rows.push_back({f});
+    for (const auto& c : p.comb2) rows.push_back(c);
+    for (const auto& c : p.comb3) rows.push_back(c);
+    const int n_coeff = static_cast<int>(rows.size());
+    if (Xo) std::copy(d.begin(), d.end(), Xo);
+    if (Po)
+        for (int i = 0; i < nlocal; ++i)
+            for (int r = 0; r < n_coeff; ++r) {
+                double v = 1.0;
+                for (int f : rows[r]) v *= d[size_t(i) * n_des + f];
+                Po[size_t(i) * n_coeff + r] = v;
+            }
+    if (!FFo && !SFo) return;
+    // half pairs in list order
+    std::vector<std::array<int, 3>> pairs;  // ii, i, j
+    for (int ii = 0; ii < inum; ++ii) {
+        const int i = L.ilist[ii];
+        for (int jj = 0; jj < L.numneigh[i]; ++jj) {
+            const int j = L.idx[L.off[ii] + jj];
+            const double dx = X(i, 0) - X(j, 0), dy = X(i, 1) - X(j, 1), dz = X(i, 2) - X(j, 2);
+            if (std::sqrt(dx * dx + dy * dy + dz * dz) < p.cutoff) pairs.push_back({ii, i, j});
+        }
+    }
+    const size_t np = pairs.size();
+    std::vector<double> dirj(np * n_des * 3, 0.0), djri(np * n_des * 3, 0.0);
+#pragma omp parallel for schedule(guided)
+    for (size_t q = 0; q < np; ++q) {
+        const int i = pairs[q][1], j = pairs[q][2];
+        const int ti = T(i), tj = T(j);
+        const int oi = static_cast<int>(tag[i] - 1), oj = static_cast<int>(tag[j] - 1);
+        const double dx = X(i, 0) - X(j, 0), dy = X(i, 1) - X(j, 1), dz = X(i, 2) - X(j, 2);
+        const double dis = std::sqrt(dx * dx + dy * dy + dz * dz);
+        double th, ph;
+        cart_to_sph(dx, dy, dz, th, ph);
+        std::vector<double> fn, fnd;
+        std::vector<dc> ylm, yd;
+        get_fn_d(p, dis, fn, fnd);
+        get_ylm_d(p, th, ph, ylm, yd);
+        const double ct = std::cos(th), st = std::sin(th), cp = std::cos(ph), sp = std::sin(ph);
+        const double coeff = std::fabs(st) > 1e-15 ? 1.0 / st : 0.0;
+        // grad phi_{n,lm}(del_ij) for every full lm (conjugate symmetry for m > 0)
+        std::vector<dc> gph(size_t(n_fn) * n_lm_all * 3);
+        for (int lm = 0; lm < n_lm; ++lm) {
+            const int m = p.lm_info[lm][1], lm1 = p.lm_info[lm][2], lm2 = p.lm_info[lm][3];
+            const double cc = std::pow(-1.0, m);
+            const dc ydphi = dc(0.0, 1.0) * double(m) * ylm[lm];
+            const dc term1 = yd[lm] * ct, term2 = coeff * ydphi;
+            const dc dd[3] = {term1 * cp - term2 * sp, term1 * sp + term2 * cp, -yd[lm] * st};
+            const double del[3] = {dx, dy, dz};
+            for (int n = 0; n < n_fn; ++n)
+                for (int k = 0; k < 3; ++k) {
+                    const dc g = (fnd[n] * ylm[lm] * del[k] + fn[n] * dd[k]) / dis;
+                    gph[(size_t(n) * n_lm_all + lm1) * 3 + k] = g;
+                    gph[(size_t(n) * n_lm_all + lm2) * 3 + k] = cc * std::conj(g);
+                }
+        }
+        const int tc_ij = p.type_comb[ti][tj];
+        for (int f = 0; f < n_des; ++f) {
+            const int itc = f % n_gt, n = f / n_gt;
+            const LinearTerm& lin = p.linear[itc];
+            const vec2i& lma = p.lm_array[lin.lmindex];
+            const auto& cf = p.lm_coeffs[lin.lmindex];
+            const vec1i& lq = p.l_comb[lin.lmindex];
+            const int order = static_cast<int>(lin.tcomb_index.size());
+            double* oi_ = &dirj[(q * n_des + f) * 3];
+            double* oj_ = &djri[(q * n_des + f) * 3];
+            for (size_t t = 0; t < lma.size(); ++t)
+                for (int mu = 0; mu < order; ++mu) {
+                    if (lin.tcomb_index[mu] != tc_ij) continue;
+                    dc pi(cf[t], 0.0), pj(cf[t], 0.0);
+                    for (int mu2 = 0; mu2 < order; ++mu2) {
+                        if (mu2 == mu) continue;
+                        const int tc2 = lin.tcomb_index[mu2];
+                        const auto& si = p.type_comb_pair[tc2][ti];
+                        const auto& sj = p.type_comb_pair[tc2][tj];
+                        pi = si.empty() ? dc(0.0) : pi * A(oi, si[0], n, lma[t][mu2]);
+                        pj = sj.empty() ? dc(0.0) : pj * A(oj, sj[0], n, lma[t][mu2]);
+                    }
+                    const double par = (lq[mu] % 2) ? -1.0 : 1.0;
+                    for (int k = 0; k < 3; ++k) {
+                        const dc g = gph[(size_t(n) * n_lm_all + lma[t][mu]) * 3 + k];
+                        oi_[k] += prod_real(pi, g);
+                        oj_[k] -= par * prod_real(pj, g);
+                    }
+                }
+        }
+    }
+    std::vector<double> FF(size_t(nlocal) * n_coeff * 3, 0.0), SF(size_t(n_coeff) * 6, 0.0);
+    for (size_t q = 0; q < np; ++q) {
+        const int i = pairs[q][1], j = pairs[q][2];
+        const int oi = static_cast<int>(tag[i] - 1), oj = static_cast<int>(tag[j] - 1);
+        const double dx = X(i, 0) - X(j, 0), dy = X(i, 1) - X(j, 1), dz = X(i, 2) - X(j, 2);
+        for (int r = 0; r < n_coeff; ++r) {
+            const vec1i& row = rows[r];
+            double t[3] = {0.0, 0.0, 0.0};
+            for (size_t nu = 0; nu < row.size(); ++nu) {
+                double ci = 1.0, cj = 1.0;
+                for (size_t nu2 = 0; nu2 < row.size(); ++nu2) {
+                    if (nu2 == nu) continue;
+                    ci *= d[size_t(oi) * n_des + row[nu2]];
+                    cj *= d[size_t(oj) * n_des + row[nu2]];
+                }
+                const double* a = &dirj[(q * n_des + row[nu]) * 3];
+                const double* b = &djri[(q * n_des + row[nu]) * 3];
+                for (int k = 0; k < 3; ++k) t[k] += -ci * a[k] + cj * b[k];
+            }
+            for (int k = 0; k < 3; ++k) {
+                FF[(size_t(oi) * n_coeff + r) * 3 + k] += t[k];
+                FF[(size_t(oj) * n_coeff + r) * 3 + k] -= t[k];
+            }
+            double* sf = &SF[size_t(r) * 6];
+            sf[0] += dx * t[0];
+            sf[1] += dy * t[1];
+            sf[2] += dz * t[2];
+            sf[3] += dy * t[2];
+            sf[4] += dz * t[0];
+            sf[5] += dx * t[1];
+        }
+    }
+    if (FFo) std::copy(FF.begin(), FF.end(), FFo);
+    if (SFo) std::copy(SF.begin(), SF.end(), SFo);
+}
+
 void set_err(char* err, int len, const std::string& msg) {
     if (err && len > 0) {
         std::snprintf(err, static_cast<size_t>(len), "%s", msg.c_str());
@@ -992,6 +1170,31 @@ int oracle_compute(void* h, int nlocal, int nall, const double* x, const int64_t
     run_compute(*o, nlocal, x, tag, L, out);
     *eng = out.eng;
     for (int k = 0; k < 6; ++k) virial[k] = out.virial[k];
+    return 0;
+}
+
+// Effective regression weights w (energy = sum_p w_p X_p): the file's
+// coefficients / scale, i.e. half the stored 2/scale products
+// (mlip_model.h:90, pair_mlip_gtinv.cpp:305,513).
+int oracle_weights(void* h, double* out) {
+    const Potential& p = static_cast<Oracle*>(h)->pot;
+    for (size_t k = 0; k < p.reg_coeffs.size(); ++k) out[k] = 0.5 * p.reg_coeffs[k];
+    return static_cast<int>(p.reg_coeffs.size());
+}
+
+// Training outputs (MLIPModel::prepare_features and its getters,
+// src/USER-MLIP/mlipkk.cpp:461-470,1318-1370) for the structure given as a
+// LAMMPS list with tags: X[nlocal][n_des], P[nlocal][n_coeff],
+// FF[nlocal][n_coeff][3] (owners by tag), SF[n_coeff][6] (xx, yy, zz, yz,
+// zx, xy).  Any output may be NULL.
+int oracle_train(void* h, int nlocal, int nall, const double* x, const int64_t* tag, int inum, const int* ilist,
+                 const int* numneigh, const int* neigh_off, const int* neigh_idx, double* X, double* P, double* FF,
+                 double* SF) {
+    auto* o = static_cast<Oracle*>(h);
+    if (static_cast<int>(o->types_by_tag.size()) < nlocal) return -1;
+    (void)nall;
+    List L{inum, ilist, neigh_off, neigh_idx, numneigh};
+    run_train(*o, nlocal, x, tag, L, X, P, FF, SF);
     return 0;
 }
 

@@ -40,6 +40,9 @@ def lib():
         L.oracle_set_types_by_tag.argtypes = [vp, ctypes.c_int, ip]
         L.oracle_compute.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp, lp, ctypes.c_int, ip, ip,
                                      ip, ip, dp, dp, dp, dp, dp, dp]
+        L.oracle_train.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp, lp, ctypes.c_int, ip, ip, ip, ip, dp, dp, dp,
+                                   dp]
+        L.oracle_weights.argtypes = [vp, dp]
         L.oracle_cart_to_sph.argtypes = [ctypes.c_double] * 3 + [dp, dp]
         L.oracle_ylm.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, dp, dp]
         L.oracle_fn.argtypes = [ctypes.c_double, ctypes.c_int, dp, ctypes.c_double, dp, dp]
@@ -204,6 +207,38 @@ class GtinvOracle:
         if want_features:
             out["features"] = ft
         return out
+
+
+def _train(self, nl, elem_of_atom, want=("X", "P", "FF", "SF")):
+    """Training outputs of MLIPModel::prepare_features restated on the CPU
+    (oracle_train): X [n, n_des], P [n, n_coeff], FF [n, n_coeff, 3]
+    (owners), SF [n_coeff, 6] in the order xx, yy, zz, yz, zx, xy."""
+    L = lib()
+    ea = np.ascontiguousarray(elem_of_atom, dtype=np.int32)
+    L.oracle_set_types_by_tag(self.h, ea.size, _p(ea, ctypes.c_int))
+    n, nd, nc = nl.nlocal, self.info["n_des"], self.info["n_coeff"]
+    X = np.zeros((n, nd)) if "X" in want else None
+    P = np.zeros((n, nc)) if "P" in want else None
+    FF = np.zeros((n, nc, 3)) if "FF" in want else None
+    SF = np.zeros((nc, 6)) if "SF" in want else None
+    null = ctypes.POINTER(ctypes.c_double)()
+    q = lambda a: _p(a, ctypes.c_double) if a is not None else null  # noqa: E731
+    rc = L.oracle_train(self.h, nl.nlocal, nl.nall, _p(nl.x, ctypes.c_double), _p(nl.tag, ctypes.c_int64), nl.inum,
+                        _p(nl.ilist, ctypes.c_int), _p(nl.numneigh, ctypes.c_int), _p(nl.off, ctypes.c_int),
+                        _p(nl.idx, ctypes.c_int), q(X), q(P), q(FF), q(SF))
+    if rc != 0:
+        raise RuntimeError("oracle_train failed (%d)" % rc)
+    return {"X": X, "P": P, "FF": FF, "SF": SF}
+
+
+def _weights(self):
+    w = np.zeros(self.info["n_coeff"])
+    lib().oracle_weights(self.h, _p(w, ctypes.c_double))
+    return w
+
+
+GtinvOracle.train = _train
+GtinvOracle.weights = _weights
 
 
 def cart_to_sph(x, y, z):
