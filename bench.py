@@ -150,6 +150,54 @@ def batched_section(pot, steps, precision="mixed"):
     return out
 
 
+def train_section(nstruct, reps=3):
+    """Training outputs (structural, polynomial, force and stress features)
+    of a batch of small random cells with the fitted order-4 cubic potential
+    of the reference's regression tests (gtinv-820: 3470 features, 6695
+    coefficients), copied back to host; the CPU oracle on a few of the same
+    structures for scale."""
+    pot = os.path.join(ROOT, "tests", "golden", "reference", "gtinv-820.mlip")
+    p = make_pair(pot, "fp64")
+    structs = W.random_structures(nstruct, 8, 48, seed=13)
+    natoms = sum(s.natoms for s in structs)
+    b = BatchSystem(p, structs)
+    b.train_features()
+    best = None
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        g = b.train_features()
+        dt = time.perf_counter() - t0
+        best = dt if best is None else min(best, dt)
+    if not np.all(np.isfinite(g["FF"])):
+        raise SystemExit("non-finite training features")
+    out_bytes = sum(v.nbytes for v in g.values() if v is not None)
+    del g, b
+    cpu = None
+    try:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_lib
+        o = oracle_lib.Oracle(pot)
+        t0 = time.perf_counter()
+        k = 0
+        while k < min(8, nstruct) and time.perf_counter() - t0 < 10.0:
+            s = structs[k]
+            o.train(W.build_neighbor_list(s, o.cutoff), s.types - 1)
+            k += 1
+        cpu = {"structures_per_s": k / (time.perf_counter() - t0), "sample": "first %d structures" % k,
+               "cores": oracle_lib.lib().oracle_num_threads()}
+    except Exception as e:  # the oracle is test infrastructure; report its absence
+        cpu = {"error": str(e)}
+    out = {"structures_per_s": nstruct / best, "atoms_per_s": natoms / best, "ms_per_batch": 1e3 * best,
+           "nstruct": nstruct, "natoms": natoms, "output_gb": out_bytes / 1e9, "includes_copy_to_host": True,
+           "workload": "%d random triclinic cells, 8-48 atoms log-uniform (seed 13); gtinv-820 potential "
+                       "(order 4, model_type 3, max_p 3)" % nstruct,
+           "cpu_baseline": cpu}
+    log("train: %d structures (%d atoms) in %.1f ms -> %.4g structures/s (%.2f GB of features to host); "
+        "CPU oracle %s" % (nstruct, natoms, 1e3 * best, out["structures_per_s"], out["output_gb"],
+                           cpu.get("structures_per_s") if cpu else None))
+    return out
+
+
 def served_section(pot, rates, windows, budget_gb):
     """Poisson request streams (seed 7) through the continuous-batching pool
     in mixed precision, and the same streams one request at a time in FP64."""
@@ -187,6 +235,8 @@ def main():
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision timing")
     ap.add_argument("--no-batched", action="store_true", help="skip the 4096-structure batched evaluation")
     ap.add_argument("--no-served", action="store_true", help="skip the served request streams")
+    ap.add_argument("--no-train", action="store_true", help="skip the training-feature batch")
+    ap.add_argument("--train-structures", type=int, default=256)
     ap.add_argument("--serve-windows", type=str, default="1.0,0.5,0.25",
                     help="arrival windows (s) for the 200/1000/5000 req/s streams")
     ap.add_argument("--pool-gb", type=float, default=16.0, help="HBM budget of the request pool")
@@ -205,12 +255,28 @@ def main():
     pair.coeff("*", "*", pot, "Ti", "Al")
     pair.set_precision(args.precision)
     log("model %s" % pair.model_info())
-    t0 = time.time()
+    t0 = time.perf_counter()
     off, idx, sh = periodic_full_list(s.cell, s.positions, pair.init_one())
-    log("neighbor list: %d atoms, %d pairs (%.1f per atom) in %.1fs"
-        % (s.natoms, idx.size, idx.size / s.natoms, time.time() - t0))
-    rs = ResidentSystem(pair, s.positions, s.types, off, idx, sh)
+    t_host = time.perf_counter() - t0
+    host_pairs = idx.size
     del off, idx, sh
+    # the resident structure's lists come from the device builder (identical
+    # to the host builder's); the second build reuses the allocations, as a
+    # re-neighboring step of a run would
+    t0 = time.perf_counter()
+    rs = ResidentSystem.from_cell(pair, s.cell, s.positions, s.types)
+    t_dev_first = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    rs = ResidentSystem.from_cell(pair, s.cell, s.positions, s.types)
+    t_dev = time.perf_counter() - t0
+    if rs.npairs != host_pairs:
+        raise SystemExit("device and host neighbor lists differ in size")
+    neighbor = {"pairs": rs.npairs, "pairs_per_atom": rs.npairs / s.natoms, "device_build_s": t_dev,
+                "device_first_build_s": t_dev_first, "host_build_s": t_host, "host_threads": 1,
+                "note": "device time: positions upload, bin sort, count/scan/fill, gather-list sort and merge, "
+                        "slot layout; identical lists to the host builder (tests/test_gpu_neighbor.py)"}
+    log("neighbor list: %d atoms, %d pairs (%.1f per atom): device %.3fs (first %.2fs), host %.2fs"
+        % (s.natoms, rs.npairs, rs.npairs / s.natoms, t_dev, t_dev_first, t_host))
     dt, prof, res, nt = timed_steps(rs, args.warmup, args.steps, args.precision)
     if not np.isfinite(res["energy"]) or not np.all(np.isfinite(res["forces"])):
         raise SystemExit("non-finite result")
@@ -237,6 +303,7 @@ def main():
     npairs = rs.npairs
     del rs
     batched = None if args.no_batched else batched_section(pot, max(args.steps, 3))
+    train = None if args.no_train else train_section(args.train_structures)
     served = None
     if not args.no_served:
         served = served_section(pot, [200, 1000, 5000], [float(w) for w in args.serve_windows.split(",")],
@@ -299,6 +366,8 @@ def main():
         "mixed": mixed,
         "batched": batched,
         "served": served,
+        "neighbor_build": neighbor,
+        "train": train,
         "kernels_ms": {k: round(v["avg_ms"], 4) for k, v in kernels.items()},
         "flops_per_step": flops,
         "flops_reference_per_step": flops_ref,

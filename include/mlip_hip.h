@@ -122,7 +122,7 @@ int mlip_batch_set(mlip_ctx* ctx, int nstruct, const int* natoms, const double* 
 int mlip_batch_get(mlip_ctx* ctx, double* energies, double* virials, double* f);
 /* Training outputs (the design matrices of a fit) of the resident structure
  * or batch, gtinv descriptor: replaces MLIPModel::prepare_features and its
- * getters (src/USER-MLIP/mlipkk.cpp:461-470, 1318-1370; math
+ * getters (src/USER-MLIP/mlipkk.cpp:461-470, 1322-1381; math
  * docs/formulation.md:282-349).  Per atom (batch order):
  * X[natoms][n_des] structural features, P[natoms][n_coeff] polynomial
  * features, FF[natoms][n_coeff][3] force features (FF[k][p] = -dX_p/dr_k

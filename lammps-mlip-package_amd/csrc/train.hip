@@ -1,11 +1,11 @@
 // Training outputs of the gtinv model on the device: the design matrices a
 // fit needs (MLIPModel::prepare_features, src/USER-MLIP/mlipkk.cpp:461-470,
-// getters :1318-1370).  For every atom the structural features
+// getters :1322-1381).  For every atom the structural features
 // X[i][f] = d_f^(i) (all n_des, f = n * n_gtinv + itc) and the polynomial
 // features P[i][p] = prod_{f in p} X[i][f]; the force features
 // FF[k][p] = -dX_p/dr_k with X_p = sum_i P[i][p]; per structure the stress
 // features SF[p] = sum over pairs del (x) (force term) in the order
-// xx, yy, zz, yz, zx, xy (mlipkk.cpp:1255-1266).  With the regression
+// xx, yy, zz, yz, zx, xy (mlipkk.cpp:1260-1271).  With the regression
 // weights w: sum_p w_p X_p = E, sum_p w_p FF[k][p] = F_k and sum_p w_p SF[p]
 // is the virial.
 //

@@ -912,7 +912,7 @@ void run_compute(const Oracle& o, int nlocal, const double* x, const int64_t* ta
 //   t = sum_nu [ -prod_{nu' != nu} d^(i) dirj_{f_nu} + prod_{nu' != nu} d^(j) djri_{f_nu} ],
 // added to FF[i] and subtracted from FF[j], so FF[k][p] = -dX_p/dr_k with
 // X_p = sum_i P_i(p); the stress feature adds del_ij (x) t in the order
-// xx, yy, zz, yz, zx, xy of mlipkk.cpp:1255-1266 (del = x_i - x_j here,
+// xx, yy, zz, yz, zx, xy of mlipkk.cpp:1260-1271 (del = x_i - x_j here,
 // the reference's displacement is x_j - x_i with the opposite sign in front).
 void run_train(const Oracle& o, int nlocal, const double* x, const int64_t* tag, const List& L, double* Xo,
                double* Po, double* FFo, double* SFo) {
@@ -1183,7 +1183,7 @@ int oracle_weights(void* h, double* out) {
 }
 
 // Training outputs (MLIPModel::prepare_features and its getters,
-// src/USER-MLIP/mlipkk.cpp:461-470,1318-1370) for the structure given as a
+// src/USER-MLIP/mlipkk.cpp:461-470,1322-1381) for the structure given as a
 // LAMMPS list with tags: X[nlocal][n_des], P[nlocal][n_coeff],
 // FF[nlocal][n_coeff][3] (owners by tag), SF[n_coeff][6] (xx, yy, zz, yz,
 // zx, xy).  Any output may be NULL.
