@@ -50,25 +50,74 @@ def make_workload(ncell, tmpdir):
 
 
 def cpu_baseline(pot, budget_s):
-    """Oracle on a 4x4x4-cell (256-atom) sub-cell of the same generator."""
+    """Oracle on a 4x4x4-cell (256-atom) sub-cell of the same generator, on
+    all host threads and on one thread (the reference's CPU style scales
+    poorly with OpenMP)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib
     o = oracle_lib.Oracle(pot)
     s = W.fcc(4, 4.0, 0.08, 4, type_fraction=0.5, type_seed=4)
     nl = W.build_neighbor_list(s, o.cutoff, full=False)
-    n_eval, t_tot = 0, 0.0
-    while t_tot < budget_s or n_eval == 0:
-        t0 = time.perf_counter()
-        o.compute(nl, s.types - 1)
-        t_tot += time.perf_counter() - t0
-        n_eval += 1
-        if n_eval >= 50:
-            break
-    threads = oracle_lib.lib().oracle_num_threads()
+
+    def timed(budget):
+        n_eval, t_tot = 0, 0.0
+        while t_tot < budget or n_eval == 0:
+            t0 = time.perf_counter()
+            o.compute(nl, s.types - 1)
+            t_tot += time.perf_counter() - t0
+            n_eval += 1
+            if n_eval >= 50:
+                break
+        return n_eval, t_tot
+
+    L = oracle_lib.lib()
+    threads = L.oracle_num_threads()
+    n_eval, t_tot = timed(budget_s)
+    L.oracle_set_num_threads(1)
+    n1, t1 = timed(budget_s / 2)
+    L.oracle_set_num_threads(threads)
     return {"value": s.natoms * n_eval / t_tot, "unit": "atom-evals/s", "cores": int(threads),
             "kind": "port",
             "sample": "%d evaluation(s) of a 256-atom (4x4x4-cell) sub-cell of the same generator and model, "
-                      "%.1f s of CPU work; per-atom cost is size-independent" % (n_eval, t_tot)}
+                      "%.1f s of CPU work; per-atom cost is size-independent" % (n_eval, t_tot),
+            "single_thread": {"value": s.natoms * n1 / t1, "cores": 1,
+                              "sample": "%d evaluation(s), %.1f s" % (n1, t1)}}
+
+
+def nve_drift(pot, precision, nsteps=200, ncell=10):
+    """Total-energy drift of a device-resident NVE run (4000-atom binary fcc
+    of the benchmark generator, 300 K Maxwell start, 1 fs) in the request
+    pool; returns the TotEng series per atom."""
+    p = make_pair(pot, precision)
+    s = W.fcc(ncell, 4.0, 0.08, 4, type_fraction=0.5, type_seed=4)
+    v = SV.maxwell(s, [47.88, 26.98], 300.0, np.random.default_rng(8))
+    pool = SV.RequestPool(p, s.natoms + 64, (s.natoms + 64) * 260, 2)
+    h = pool.admit(s, nsteps=nsteps, dt=0.001, skin=1.0, velocities=v)
+    t0 = time.perf_counter()
+    while pool.tick():
+        pool.sync()
+    dt = time.perf_counter() - t0
+    th = pool.result(h, max_evals=nsteps + 8)["thermo"]
+    etot = (th[:, 0] + 0.5 * (th[:, 7] + th[:, 8] + th[:, 9])) / s.natoms
+    return etot, th[:, 0] / s.natoms, dt
+
+
+def drift_section(pot, nsteps=200):
+    e64, p64, t64 = nve_drift(pot, "fp64", nsteps)
+    emx, pmx, tmx = nve_drift(pot, "mixed", nsteps)
+    out = {"workload": "4000-atom binary fcc (10^3 cells, seed 4), same model, Maxwell 300 K (seed 8), "
+                       "velocity Verlet 1 fs, %d steps in the request pool" % nsteps,
+           "fp64_totE_drift_per_atom": float(e64[-1] - e64[0]),
+           "fp64_totE_maxdev_per_atom": float(np.abs(e64 - e64[0]).max()),
+           "mixed_totE_drift_per_atom": float(emx[-1] - emx[0]),
+           "mixed_totE_maxdev_per_atom": float(np.abs(emx - emx[0]).max()),
+           "mixed_vs_fp64_epair_maxdev_per_atom": float(np.abs(pmx - p64).max()),
+           "unit": "eV/atom", "run_s": {"fp64": t64, "mixed": tmx}}
+    log("NVE drift over %d steps: FP64 %.2e, mixed %.2e eV/atom (max dev %.2e / %.2e); mixed vs FP64 E_pair "
+        "max %.2e eV/atom" % (nsteps, out["fp64_totE_drift_per_atom"], out["mixed_totE_drift_per_atom"],
+                               out["fp64_totE_maxdev_per_atom"], out["mixed_totE_maxdev_per_atom"],
+                               out["mixed_vs_fp64_epair_maxdev_per_atom"]))
+    return out
 
 
 def timed_steps(rs, warmup, steps, tag):
@@ -297,6 +346,7 @@ def main():
         log("mixed: %.1f ms/step (%.2fx FP64), dE/atom %.2e, F max %.2e MAE %.2e eV/A"
             % (mixed["ms_per_step"], mixed["speedup_vs_fp64"], mixed["energy_err_per_atom"],
                mixed["force_err_max"], mixed["force_mae"]))
+        mixed["nve_drift"] = drift_section(pot)
     if args.calibrate_copy_mib > 0:
         rs.calibration_copy(args.calibrate_copy_mib << 20, reps=3)
 

@@ -47,6 +47,7 @@ def lib():
         L.oracle_ylm.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, dp, dp]
         L.oracle_fn.argtypes = [ctypes.c_double, ctypes.c_int, dp, ctypes.c_double, dp, dp]
         L.oracle_num_threads.restype = ctypes.c_int
+        L.oracle_set_num_threads.argtypes = [ctypes.c_int]
         L.pair_oracle_create.restype = vp
         L.pair_oracle_create.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int]
         L.pair_oracle_destroy.argtypes = [vp]
