@@ -1105,9 +1105,17 @@ int mlip_flops(const mlip_ctx* c, double* out7) {
                                     if (std::abs(k - ma - mb) <= v.lc) ++cnt;
                         }
                     }
-                    const double per = (v.order == 3 ? 11.0 : 17.0) + ((adj && (v.flags & 1)) ? 5.0 : 0.0);
-                    f += per * cnt;
+                    // per term: complex product 6, then 4 (adjoint: two FMAs
+                    // into G) or 5 (features: Re of the product with the
+                    // output factor, one FMA); order 4 adds the a x b
+                    // product (6) per (m_a, m_b), the adjoint H x a (2) per m_a
+                    const double na_ = 2.0 * v.la + 1.0, nb_ = 2.0 * v.lb + 1.0;
+                    f += (adj ? 10.0 : 11.0) * cnt + (v.order == 4 ? 6.0 * na_ * nb_ : 0.0) + (adj ? 2.0 * na_ : 0.0);
                 }
+            // adjoint: the energy of linear families from the G rows (Euler),
+            // at most two class boundaries per group
+            if (adj)
+                for (const GroupQ34& g : gs) f += 2.0 * 5.0 * (g.L + 1);
             return f;
         };
         out7[2] += na * terms(tt.groups_feat, tt.visits_feat, false);

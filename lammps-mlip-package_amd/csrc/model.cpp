@@ -251,11 +251,12 @@ void build_type_tables(Model& m, int t, const std::vector<std::map<std::vector<i
             v.sc = q == 4 ? sp[oth[2]] : 0;
             v.pslot = pslot;
             const bool is_poly = pslot >= 0;
-            const bool with_d = (p == 0) && !is_poly;
             int lmax_v = std::max(L, std::max(v.la, v.lb));
             if (q == 4) lmax_v = std::max(lmax_v, v.lc);
             const bool fast = (q == 3 && lmax_v <= kUnrollCap3) || (q == 4 && lmax_v <= kUnrollCap4);
-            v.flags = (with_d ? 1 : 0) | 2 | (fast ? 4 : 0);
+            // adjoint visits accumulate G only; the energy of linear-only
+            // families follows from G by Euler's theorem in the kernel
+            v.flags = 2 | (fast ? 4 : 0);
             std::vector<int> mt(q);
             long cost = 0;
             auto emit = [&](bool dense_d, bool g_only_sum) {
@@ -300,10 +301,6 @@ void build_type_tables(Model& m, int t, const std::vector<std::map<std::vector<i
             v.coef = static_cast<int>(m.coef.size());
             emit(false, true);
             v.dcoef = -1;
-            if (with_d) {
-                v.dcoef = static_cast<int>(m.coef.size());
-                emit(true, false);
-            }
             adj.push_back({sp[p], L, v, cost});
             if (p == 0 && is_poly) {
                 VisitQ34 f = v;
@@ -324,13 +321,17 @@ void build_type_tables(Model& m, int t, const std::vector<std::map<std::vector<i
     // group by (S, L), balance groups over the waves of a workgroup
     auto make_groups = [&](std::vector<Pending>& pend, std::vector<GroupQ34>& groups,
                            std::vector<VisitQ34>& visits, bool adjoint, int& red_total, bool& dyn) {
-        // group by (S, L); inside a group, visits with the same code path
-        // (signature and flags) are adjacent for instruction-cache locality
-        std::stable_sort(pend.begin(), pend.end(), [](const Pending& a, const Pending& b) {
+        // group by (S, L); inside a group, visits by energy class (linear
+        // order 3, linear order 4, polynomial: the adjoint kernel takes each
+        // linear class's energy from the accumulated G row at the class
+        // boundary), then with the same code path (signature) adjacent for
+        // instruction-cache locality
+        auto cls = [](const VisitQ34& v) { return v.pslot >= 0 ? 2 : (v.order == 3 ? 0 : 1); };
+        std::stable_sort(pend.begin(), pend.end(), [&](const Pending& a, const Pending& b) {
             if (a.S != b.S) return a.S < b.S;
             if (a.L != b.L) return a.L < b.L;
-            const auto ka = std::make_tuple(a.v.flags & 4, a.v.order, a.v.la, a.v.lb, a.v.lc, a.v.flags & 1);
-            const auto kb = std::make_tuple(b.v.flags & 4, b.v.order, b.v.la, b.v.lb, b.v.lc, b.v.flags & 1);
+            const auto ka = std::make_tuple(cls(a.v), a.v.flags & 4, a.v.order, a.v.la, a.v.lb, a.v.lc);
+            const auto kb = std::make_tuple(cls(b.v), b.v.flags & 4, b.v.order, b.v.la, b.v.lb, b.v.lc);
             return ka < kb;
         });
         long total = 0;

@@ -61,6 +61,10 @@ __device__ __forceinline__ void visit3(const Tile& T, const DevModel& M, const V
         const int ma = ia - la;
         double xr, xi;
         T.get(sa, la, ma, xr, xi);
+        if (DOG) {
+            xr *= H;
+            xi *= H;
+        }
         cptr<double> cgk = cg + ia * (L + 1);
         cptr<double> cdk = cd + ia * (L + 1);
 #pragma unroll
@@ -72,9 +76,8 @@ __device__ __forceinline__ void visit3(const Tile& T, const DevModel& M, const V
             const double pr = xr * yr - xi * yi;
             const double pi = xr * yi + xi * yr;
             if (DOG) {
-                const double hc = H * cgk[k];
-                gr[k] = fma(hc, pr, gr[k]);
-                gi[k] = fma(hc, pi, gi[k]);
+                gr[k] = fma(cgk[k], pr, gr[k]);
+                gi[k] = fma(cgk[k], pi, gi[k]);
             }
             if (DOD) d = fma(cdk[k], lr[k] * pr - li[k] * pi, d);
         }
@@ -95,6 +98,10 @@ __device__ __forceinline__ void visit4(const Tile& T, const DevModel& M, const V
         const int ma = ia - la;
         double xr, xi;
         T.get(sa, la, ma, xr, xi);
+        if (DOG) {
+            xr *= H;
+            xi *= H;
+        }
         for (int ib = 0; ib < nb; ++ib) {
             const int mb = ib - lb;
             double yr, yi;
@@ -112,9 +119,8 @@ __device__ __forceinline__ void visit4(const Tile& T, const DevModel& M, const V
                 const double pr = qr * zr - qi * zi;
                 const double pi = qr * zi + qi * zr;
                 if (DOG) {
-                    const double hc = H * cgk[k];
-                    gr[k] = fma(hc, pr, gr[k]);
-                    gi[k] = fma(hc, pi, gi[k]);
+                    gr[k] = fma(cgk[k], pr, gr[k]);
+                    gi[k] = fma(cgk[k], pi, gi[k]);
                 }
                 if (DOD) d = fma(cdk[k], lr[k] * pr - li[k] * pi, d);
             }
@@ -168,6 +174,10 @@ __device__ __forceinline__ void visit3u(const Tile& T, int S, int sa, int sb, cp
         // next a factor requested one m_a ahead
         double nxr = 0.0, nxi = 0.0;
         if (ia < 2 * LA) T.get(sa, LA, ma + 1, nxr, nxi);
+        if (DOG) {
+            xr *= H;
+            xi *= H;
+        }
         // d-forming visits read the output factor a_{S,L,-k} term by term; the
         // barrier keeps the compiler from hoisting those reads into registers
         if (DOD) asm volatile("" ::: "memory");
@@ -180,9 +190,8 @@ __device__ __forceinline__ void visit3u(const Tile& T, int S, int sa, int sb, cp
             const double pr = xr * yr - xi * yi;
             const double pi = xr * yi + xi * yr;
             if (DOG) {
-                const double hc = H * cg[t];
-                gr[k] = fma(hc, pr, gr[k]);
-                gi[k] = fma(hc, pi, gi[k]);
+                gr[k] = fma(cg[t], pr, gr[k]);
+                gi[k] = fma(cg[t], pi, gi[k]);
             }
             if (DOD) {
                 double lr, li;
@@ -212,6 +221,10 @@ __device__ __forceinline__ void visit4u(const Tile& T, int S, int sa, int sb, in
         const int ma = ia - LA;
         double xr, xi;
         T.get(sa, LA, ma, xr, xi);
+        if (DOG) {
+            xr *= H;
+            xi *= H;
+        }
 #pragma unroll
         for (int ib = 0; ib <= 2 * LB; ++ib) {
             const int mb = ib - LB;
@@ -229,9 +242,8 @@ __device__ __forceinline__ void visit4u(const Tile& T, int S, int sa, int sb, in
                 const double pr = qr * zr - qi * zi;
                 const double pi = qr * zi + qi * zr;
                 if (DOG) {
-                    const double hc = H * cg[t];
-                    gr[k] = fma(hc, pr, gr[k]);
-                    gi[k] = fma(hc, pi, gi[k]);
+                    gr[k] = fma(cg[t], pr, gr[k]);
+                    gi[k] = fma(cg[t], pi, gi[k]);
                 }
                 if (DOD) {
                     double lr, li;
@@ -288,6 +300,23 @@ struct Unrolled {
     }
 };
 
+// sum_{m <= 0} w_m Re(G_m a_{S,L,m}) over the accumulated row (w_0 = 1,
+// w_{m<0} = 2): the derivative of the row's share of the energy along
+// a -> (1 + eps) a.  For a homogeneous family of order q summed over all its
+// factor positions this is q times its energy (Euler).
+template <int L>
+__device__ __forceinline__ double row_dot(const Tile& T, int S, const double (&gr)[L + 1],
+                                          const double (&gi)[L + 1]) {
+    double sum = 0.0;
+#pragma unroll
+    for (int k = 0; k <= L; ++k) {
+        double ar, ai;
+        T.get(S, L, -k, ar, ai);
+        sum = fma(k == 0 ? 1.0 : 2.0, gr[k] * ar - gi[k] * ai, sum);
+    }
+    return sum;
+}
+
 template <int L, bool FEAT, bool DYN>
 __device__ __forceinline__ void q34_group(const Tile& T, const DevModel& M, const DevSystem& S, const GroupQ34& g,
                                           int blk, int n, double* red, double& eacc) {
@@ -298,6 +327,17 @@ __device__ __forceinline__ void q34_group(const Tile& T, const DevModel& M, cons
     const int slot = blk * kBlock + T.lane;
     cptr<VisitQ34> vis = as_const(FEAT ? M.vfeat : M.vadj);
     const int vb = uniform(g.vbeg), ve = uniform(g.vend);
+    // energy classes of the visits (sorted in this order within a row):
+    // 0 linear order 3, 1 linear order 4, 2 polynomial features
+    int cls_cur = -1;
+    double dprev = 0.0;
+    auto close_class = [&](int c) {
+        if (c == 0 || c == 1) {
+            const double D = row_dot<L>(T, Sx, gr, gi);
+            eacc = fma(c == 0 ? (1.0 / 3.0) : 0.25, D - dprev, eacc);
+            dprev = D;
+        }
+    };
     for (int vi = vb; vi < ve; ++vi) {
         const VisitQ34 v = vis[vi];
         const int order = uniform(v.order), flags = uniform(v.flags), pslot = uniform(v.pslot);
@@ -305,6 +345,13 @@ __device__ __forceinline__ void q34_group(const Tile& T, const DevModel& M, cons
         if (fast == DYN) continue;  // the other kernel's visit
         const int itc = uniform(v.itc);
         double d = 0.0;
+        if (!FEAT) {
+            const int c = pslot >= 0 ? 2 : (order == 3 ? 0 : 1);
+            if (c != cls_cur) {
+                close_class(cls_cur);
+                cls_cur = c;
+            }
+        }
         const int la = uniform(v.la), lb = uniform(v.lb), lc = uniform(v.lc);
         cptr<double> cg = as_const(M.coef) + uniform(v.coef);
         cptr<double> cd = as_const(M.coef) + uniform(v.dcoef);
@@ -322,28 +369,17 @@ __device__ __forceinline__ void q34_group(const Tile& T, const DevModel& M, cons
         }
         const double H =
             pslot >= 0 ? S.H[(size_t)slot * M.n_p + n * M.n_pitc + pslot] : as_const(M.weight)[n * M.n_gtinv + itc];
-        if (flags & 1) {
-            if constexpr (DYN) {
-                if (order == 3)
-                    visit3<L, true, true>(T, M, v, Sx, H, gr, gi, d);
-                else
-                    visit4<L, true, true>(T, M, v, Sx, H, gr, gi, d);
-            } else {
-                Unrolled<L, true, true>::run(T, v, Sx, order, la, lb, lc, cg, cd, H, gr, gi, d);
-            }
-            eacc = fma(as_const(M.weight)[n * M.n_gtinv + itc], d, eacc);
+        if constexpr (DYN) {
+            if (order == 3)
+                visit3<L, true, false>(T, M, v, Sx, H, gr, gi, d);
+            else
+                visit4<L, true, false>(T, M, v, Sx, H, gr, gi, d);
         } else {
-            if constexpr (DYN) {
-                if (order == 3)
-                    visit3<L, true, false>(T, M, v, Sx, H, gr, gi, d);
-                else
-                    visit4<L, true, false>(T, M, v, Sx, H, gr, gi, d);
-            } else {
-                Unrolled<L, true, false>::run(T, v, Sx, order, la, lb, lc, cg, cd, H, gr, gi, d);
-            }
+            Unrolled<L, true, false>::run(T, v, Sx, order, la, lb, lc, cg, cd, H, gr, gi, d);
         }
     }
     if (!FEAT) {
+        close_class(cls_cur);
         if (uniform(g.nparts) == 1) {
 #pragma unroll
             for (int k = 0; k <= L; ++k) {
