@@ -3,6 +3,7 @@
 #include "model.h"
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -202,6 +203,7 @@ void build_type_tables(Model& m, int t, const std::vector<std::map<std::vector<i
         int S, L;
         VisitQ34 v;
         long cost;
+        int clen;  // length of the visit's coefficient block
     };
     std::vector<Pending> adj, feat;
     auto push_coef = [&](double v) { m.coef.push_back(v); };
@@ -301,14 +303,14 @@ void build_type_tables(Model& m, int t, const std::vector<std::map<std::vector<i
             v.coef = static_cast<int>(m.coef.size());
             emit(false, true);
             v.dcoef = -1;
-            adj.push_back({sp[p], L, v, cost});
+            adj.push_back({sp[p], L, v, cost, static_cast<int>(m.coef.size()) - v.coef});
             if (p == 0 && is_poly) {
                 VisitQ34 f = v;
                 f.flags = 1 | (fast ? 4 : 0);
                 f.dcoef = static_cast<int>(m.coef.size());
                 f.coef = f.dcoef;
                 emit(true, false);
-                feat.push_back({sp[p], L, f, cost});
+                feat.push_back({sp[p], L, f, cost, static_cast<int>(m.coef.size()) - f.coef});
             }
         }
     }
@@ -396,6 +398,27 @@ void build_type_tables(Model& m, int t, const std::vector<std::map<std::vector<i
             load[w] += gcost[k] + 16;
         }
     };
+    // Factor positions of one family that carry the same (species, l) and
+    // see the same other factors in the same order add the same products
+    // into the same G row, term for term: merge them into one visit whose
+    // coefficient block is the sum of theirs.
+    {
+        std::map<std::array<int, 11>, size_t> first;
+        std::vector<Pending> merged;
+        for (const Pending& q : adj) {
+            const VisitQ34& v = q.v;
+            const std::array<int, 11> key{q.S, q.L, v.itc, v.order, v.la, v.sa, v.lb, v.sb, v.lc, v.sc, v.flags};
+            auto it = first.find(key);
+            if (it == first.end()) {
+                first.emplace(key, merged.size());
+                merged.push_back(q);
+                continue;
+            }
+            Pending& f = merged[it->second];
+            for (int k = 0; k < q.clen; ++k) m.coef[f.v.coef + k] += m.coef[v.coef + k];
+        }
+        adj.swap(merged);
+    }
     int feat_red = 0;
     make_groups(adj, tt.groups_adj, tt.visits_adj, true, m.q34_red, m.q34_dyn_adj);
     make_groups(feat, tt.groups_feat, tt.visits_feat, false, feat_red, m.q34_dyn_feat);
