@@ -40,6 +40,10 @@ struct DevModel {
     const VisitQ34* vfeat;
     const GroupQ34* gfeat;
     int32_t gfeat_beg[2], gfeat_end[2];
+    // polynomial-feature slots (pslot) no feature visit of a species writes:
+    // zlist[zoff[t] .. zoff[t+1]) are set to zero by the order-1/2 feature pass
+    const int32_t* zlist;
+    int32_t zoff[3];
     const double* lin_p;    // linear weights of polynomial features (n_p)
     const int32_t* c3_off;  // cubic terms transposed by target: offsets (n_p+1)
     const int32_t* c3_bc;   // 2 per entry

@@ -87,6 +87,20 @@ bool upload_model(mlip_ctx* c) {
         append(tt.groups_adj, tt.visits_adj, gadj, vadj, dm.gadj_beg[t], dm.gadj_end[t]);
         append(tt.groups_feat, tt.visits_feat, gfeat, vfeat, dm.gfeat_beg[t], dm.gfeat_end[t]);
     }
+    // feature slots no visit of a species writes (zeroed by k_q12<true>)
+    std::vector<int32_t> zlist;
+    for (int t = 0; t < m.n_type && t < 2; ++t) {
+        std::vector<char> written(static_cast<size_t>(m.n_pitc), 0);
+        for (const VisitQ12& v : m.types[t].visits_q12)
+            if (v.pslot >= 0) written[v.pslot] = 1;
+        for (const VisitQ34& v : m.types[t].visits_feat)
+            if (v.pslot >= 0) written[v.pslot] = 1;
+        dm.zoff[t] = static_cast<int32_t>(zlist.size());
+        for (int r = 0; r < m.n_pitc; ++r)
+            if (!written[r]) zlist.push_back(r);
+    }
+    for (int t = m.n_type; t < 2; ++t) dm.zoff[t] = static_cast<int32_t>(zlist.size());
+    dm.zoff[2] = static_cast<int32_t>(zlist.size());
     // cubic terms transposed by target feature
     std::vector<std::vector<std::pair<std::array<int, 2>, double>>> c3(m.n_p);
     for (size_t k = 0; k < m.c3_w.size(); ++k) {
@@ -119,8 +133,10 @@ bool upload_model(mlip_ctx* c) {
               upload(c, c->d_vadj, vadj.data(), vadj.size(), "adjoint visits") &&
               upload(c, c->d_vfeat, vfeat.data(), vfeat.size(), "feature visits") &&
               upload(c, c->d_gadj, gadj.data(), gadj.size(), "adjoint groups") &&
-              upload(c, c->d_gfeat, gfeat.data(), gfeat.size(), "feature groups");
+              upload(c, c->d_gfeat, gfeat.data(), gfeat.size(), "feature groups") &&
+              upload(c, c->d_zlist, zlist.data(), zlist.size(), "feature zero list");
     if (!ok) return false;
+    dm.zlist = c->d_zlist.p;
     if (m.pair_model) {
         if (!upload(c, c->d_pairw, m.weight.data(), static_cast<size_t>(m.n_des), "pair weights") ||
             !upload(c, c->d_phoff, m.ph_off.data(), m.ph_off.size(), "pair derivative offsets") ||
@@ -662,7 +678,7 @@ void mlip_destroy(mlip_ctx* c) {
     for (auto* b : dbs) b->release();
     DBuf<int32_t>* ibs[] = {&c->d_q12off, &c->d_c3off, &c->d_c3bc, &c->elem, &c->poff, &c->pnbr, &c->pcen,
                             &c->atom_of_slot, &c->slot_of_atom, &c->block_type, &c->goff, &c->gent,
-                            &c->soff};
+                            &c->soff, &c->d_zlist};
     for (auto* b : ibs) b->release();
     c->d_q12.release();
     c->d_vadj.release();

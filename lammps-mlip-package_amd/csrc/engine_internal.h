@@ -67,7 +67,7 @@ struct mlip_ctx {
     mlip::DBuf<mlip::YTab<double>> d_yt64;
     mlip::DBuf<mlip::YTab<float>> d_yt32;
     mlip::DBuf<double> d_pairw;
-    mlip::DBuf<int32_t> d_phoff;
+    mlip::DBuf<int32_t> d_phoff, d_zlist;
     mlip::DBuf<mlip::PairHEntry> d_phent;
     mlip::DevModel dm{};
     // structure buffers

@@ -136,9 +136,10 @@ __global__ __launch_bounds__(64) void k_q12(DevModel M, DevSystem S) {
     if (FEAT) {
         // features of families the centre's species does not see are zero
         // (the reference never accumulates them); the visits below and the
-        // order-3/4 feature kernel overwrite the active ones
+        // order-3/4 feature kernel write every other slot of the row
         double* row = S.D + (size_t)(blk * kBlock + lane) * M.n_p + n * M.n_pitc;
-        for (int r = 0; r < M.n_pitc; ++r) row[r] = 0.0;
+        const int z0 = M.zoff[t], z1 = M.zoff[t + 1];
+        for (int r = z0; r < z1; ++r) row[M.zlist[r]] = 0.0;
     }
     Q12Levels<0, kMaxL, FEAT>::run(M, S, blk, n, lane, t, e);
     if (!FEAT) S.Eq12[(size_t)(blk * kBlock + lane) * M.n_fn + n] = e;
