@@ -201,8 +201,10 @@ __device__ __forceinline__ void visit3u(const Tile& T, int S, int sa, int sb, cp
             ++t;
         }
         // one m_a per scheduling region: interleaving all of them costs more
-        // registers (and occupancy) than it hides latency
-        __builtin_amdgcn_sched_barrier(0);
+        // registers (and occupancy) than it hides latency; scalar
+        // instructions (the coefficient loads) may still cross, so the next
+        // region's loads issue early and merge into wider ones
+        __builtin_amdgcn_sched_barrier(4);
         xr = nxr;
         xi = nxi;
     }
@@ -252,7 +254,7 @@ __device__ __forceinline__ void visit4u(const Tile& T, int S, int sa, int sb, in
                 }
                 ++t;
             }
-            __builtin_amdgcn_sched_barrier(0);
+            __builtin_amdgcn_sched_barrier(4);
         }
     }
 }
