@@ -156,7 +156,7 @@ int mlip_graph_mode(mlip_ctx* ctx, int on);
 int mlip_autotune(mlip_ctx* ctx, int reps, int* chosen);
 
 /* Sets the force kernel's launch shape directly (64, 128, 192 or 256
- * threads per workgroup; 128 by default).  Results do not depend on it
+ * threads per workgroup; 192 by default).  Results do not depend on it
  * beyond the summation order of the per-atom virial. */
 int mlip_set_force_threads(mlip_ctx* ctx, int threads);
 

@@ -88,7 +88,7 @@ struct mlip_ctx {
     std::vector<int64_t> owners_per_type;
     int64_t npairs = 0;
     // launch shape chosen by mlip_autotune
-    int force_threads = 128;
+    int force_threads = 192;
     // batch of independent structures (0 = single structure): per-structure
     // atom ranges [sbeg, send) and 7 results each
     int nstruct = 0;

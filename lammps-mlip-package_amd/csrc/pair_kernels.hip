@@ -257,8 +257,12 @@ struct ForceLds {
     T grad[kL1][kL1];
 };
 
+// At most 168 VGPRs (3 waves per SIMD): the FP64 instantiation would take
+// 214 and run 2 waves per SIMD; the few spills this costs are cheaper than
+// the lost latency hiding (28.0 -> 19.6 ms at 256k atoms, 192 threads, where
+// four workgroups of G tiles fit the LDS).
 template <int NF, int NT, typename T>
-__global__ __launch_bounds__(NT) void k_force(DevModel M, DevSystem S) {
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void k_force(DevModel M, DevSystem S) {
     extern __shared__ unsigned char force_dyn[];
     T* Gs = reinterpret_cast<T*>(force_dyn);  // [ns][n_lmh][NF][2]
     __shared__ ForceLds<T> C;
