@@ -257,12 +257,13 @@ struct ForceLds {
     T grad[kL1][kL1];
 };
 
-// At most 168 VGPRs (3 waves per SIMD): the FP64 instantiation would take
-// 214 and run 2 waves per SIMD; the few spills this costs are cheaper than
-// the lost latency hiding (28.0 -> 19.6 ms at 256k atoms, 192 threads, where
-// four workgroups of G tiles fit the LDS).
+// Register budget for occupancy: FP64 at most 168 VGPRs (3 waves per SIMD;
+// unconstrained it takes 214 and runs 2 waves), FP32 at most 128 (4 waves).
+// The spills this costs are cheaper than the lost latency hiding: 28.0 ->
+// 19.6 ms (FP64) and 14.2 -> 12.9 ms (FP32) at 256k atoms with 192 threads,
+// where the workgroups' G tiles still fit the LDS.
 template <int NF, int NT, typename T>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void k_force(DevModel M, DevSystem S) {
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 3))) void k_force(DevModel M, DevSystem S) {
     extern __shared__ unsigned char force_dyn[];
     T* Gs = reinterpret_cast<T*>(force_dyn);  // [ns][n_lmh][NF][2]
     __shared__ ForceLds<T> C;
