@@ -34,10 +34,10 @@ struct DevModel {
     const double* weight;   // coeff/scale, linear part first (n_des)
     const VisitQ12* q12;    // all types
     const int32_t* q12_off; // [type][maxl+2]
-    const VisitQ34* vadj;
+    const VisitPack* vadj;
     const GroupQ34* gadj;
     int32_t gadj_beg[2], gadj_end[2];
-    const VisitQ34* vfeat;
+    const VisitPack* vfeat;
     const GroupQ34* gfeat;
     int32_t gfeat_beg[2], gfeat_end[2];
     // polynomial-feature slots (pslot) no feature visit of a species writes:

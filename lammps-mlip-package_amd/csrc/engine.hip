@@ -87,6 +87,24 @@ bool upload_model(mlip_ctx* c) {
         append(tt.groups_adj, tt.visits_adj, gadj, vadj, dm.gadj_beg[t], dm.gadj_end[t]);
         append(tt.groups_feat, tt.visits_feat, gfeat, vfeat, dm.gfeat_beg[t], dm.gfeat_end[t]);
     }
+    // device form of the order-3/4 visits
+    auto pack = [&](const std::vector<VisitQ34>& vs, std::vector<VisitPack>& out) {
+        for (const VisitQ34& v : vs) {
+            const int f[8] = {v.order, v.la, v.lb, v.lc, v.sa, v.sb, v.sc, v.flags};
+            uint32_t sig = 0;
+            for (int k = 0; k < 8; ++k) {
+                if (f[k] < 0 || f[k] > 15) return false;
+                sig |= static_cast<uint32_t>(f[k]) << (4 * k);
+            }
+            out.push_back({static_cast<int32_t>(sig), v.coef, v.pslot, v.itc});
+        }
+        return true;
+    };
+    std::vector<VisitPack> vadjp, vfeatp;
+    if (!pack(vadj, vadjp) || !pack(vfeat, vfeatp)) {
+        c->err = "order-3/4 visit outside the packed range";
+        return false;
+    }
     // feature slots no visit of a species writes (zeroed by k_q12<true>)
     std::vector<int32_t> zlist;
     for (int t = 0; t < m.n_type && t < 2; ++t) {
@@ -130,8 +148,8 @@ bool upload_model(mlip_ctx* c) {
               upload(c, c->d_c3off, c3off.data(), c3off.size(), "cubic offsets") &&
               upload(c, c->d_q12, q12.data(), q12.size(), "q12 visits") &&
               upload(c, c->d_q12off, q12off.data(), q12off.size(), "q12 offsets") &&
-              upload(c, c->d_vadj, vadj.data(), vadj.size(), "adjoint visits") &&
-              upload(c, c->d_vfeat, vfeat.data(), vfeat.size(), "feature visits") &&
+              upload(c, c->d_vadj, vadjp.data(), vadjp.size(), "adjoint visits") &&
+              upload(c, c->d_vfeat, vfeatp.data(), vfeatp.size(), "feature visits") &&
               upload(c, c->d_gadj, gadj.data(), gadj.size(), "adjoint groups") &&
               upload(c, c->d_gfeat, gfeat.data(), gfeat.size(), "feature groups") &&
               upload(c, c->d_zlist, zlist.data(), zlist.size(), "feature zero list");

@@ -62,7 +62,7 @@ struct mlip_ctx {
     mlip::DBuf<double> d_coef, d_weight, d_wmat, d_linp, d_c3w;
     mlip::DBuf<mlip::VisitQ12> d_q12;
     mlip::DBuf<int32_t> d_q12off, d_c3off, d_c3bc;
-    mlip::DBuf<mlip::VisitQ34> d_vadj, d_vfeat;
+    mlip::DBuf<mlip::VisitPack> d_vadj, d_vfeat;
     mlip::DBuf<mlip::GroupQ34> d_gadj, d_gfeat;
     mlip::DBuf<mlip::YTab<double>> d_yt64;
     mlip::DBuf<mlip::YTab<float>> d_yt32;

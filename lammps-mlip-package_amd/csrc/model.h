@@ -72,6 +72,16 @@ struct VisitQ34 {
     int32_t dcoef;     // dense pair-summed block offset (flags bit0)
 };
 
+// Device form of a VisitQ34 (16 bytes, one scalar load): sig packs order,
+// l_a, l_b, l_c, s_a, s_b, s_c and flags in 4 bits each (low to high); the
+// feature-pass visits read their coefficients at coef (dcoef == coef there).
+struct VisitPack {
+    int32_t sig;
+    int32_t coef;
+    int32_t pslot;
+    int32_t itc;
+};
+
 // Output group: all visits of one (species S, angular momentum L) handled by
 // one wave of the adjoint kernel with a statically unrolled G_L accumulator.
 // Output-stationary group of the order-3/4 kernel: visits that all add into

@@ -43,6 +43,29 @@ struct Tile {
     }
 };
 
+// A visit decoded from its packed device form (wave-uniform fields).
+struct VDec {
+    int order, itc, la, sa, lb, sb, lc, sc, pslot, flags, coef, dcoef;
+};
+
+__device__ __forceinline__ VDec decode(const VisitPack& p) {
+    const int s = uniform(p.sig);
+    VDec v;
+    v.order = s & 15;
+    v.la = (s >> 4) & 15;
+    v.lb = (s >> 8) & 15;
+    v.lc = (s >> 12) & 15;
+    v.sa = (s >> 16) & 15;
+    v.sb = (s >> 20) & 15;
+    v.sc = (s >> 24) & 15;
+    v.flags = (s >> 28) & 15;
+    v.coef = uniform(p.coef);
+    v.dcoef = v.coef;
+    v.pslot = uniform(p.pslot);
+    v.itc = uniform(p.itc);
+    return v;
+}
+
 template <int L>
 __device__ __forceinline__ void load_out_row(const Tile& T, int S, double (&lr)[L + 1], double (&li)[L + 1]) {
 #pragma unroll
@@ -50,7 +73,7 @@ __device__ __forceinline__ void load_out_row(const Tile& T, int S, double (&lr)[
 }
 
 template <int L, bool DOG, bool DOD>
-__device__ __forceinline__ void visit3(const Tile& T, const DevModel& M, const VisitQ34& v, int S, double H,
+__device__ __forceinline__ void visit3(const Tile& T, const DevModel& M, const VDec& v, int S, double H,
                                        double (&gr)[L + 1], double (&gi)[L + 1], double& d) {
     double lr[L + 1], li[L + 1];
     if (DOD) load_out_row<L>(T, S, lr, li);
@@ -85,7 +108,7 @@ __device__ __forceinline__ void visit3(const Tile& T, const DevModel& M, const V
 }
 
 template <int L, bool DOG, bool DOD>
-__device__ __forceinline__ void visit4(const Tile& T, const DevModel& M, const VisitQ34& v, int S, double H,
+__device__ __forceinline__ void visit4(const Tile& T, const DevModel& M, const VDec& v, int S, double H,
                                        double (&gr)[L + 1], double (&gi)[L + 1], double& d) {
     double lr[L + 1], li[L + 1];
     if (DOD) load_out_row<L>(T, S, lr, li);
@@ -275,7 +298,7 @@ struct Unrolled;
     template <bool DOG, bool DOD>                                                                             \
     struct Unrolled<LL, DOG, DOD> {                                                                           \
         static constexpr int LV = LL;                                                                         \
-        __device__ static bool run(const Tile& T, const VisitQ34& v, int S, int order, int la, int lb, int lc, \
+        __device__ static bool run(const Tile& T, const VDec& v, int S, int order, int la, int lb, int lc,     \
                                    cptr<double> cg, cptr<double> cd, double H, double (&gr)[LL + 1],          \
                                    double (&gi)[LL + 1], double& d) {                                         \
             if (order == 3) {                                                                                 \
@@ -296,7 +319,7 @@ MLIP_UNROLLED(7, MLIP_SIG3_L7(MLIP_CASE3), MLIP_NO_CASES)
 MLIP_UNROLLED(8, MLIP_SIG3_L8(MLIP_CASE3), MLIP_NO_CASES)
 template <int L, bool DOG, bool DOD>
 struct Unrolled {
-    __device__ static bool run(const Tile&, const VisitQ34&, int, int, int, int, int, cptr<double>, cptr<double>,
+    __device__ static bool run(const Tile&, const VDec&, int, int, int, int, int, cptr<double>, cptr<double>,
                                double, double (&)[L + 1], double (&)[L + 1], double&) {
         return false;
     }
@@ -327,7 +350,7 @@ __device__ __forceinline__ void q34_group(const Tile& T, const DevModel& M, cons
 #pragma unroll
     for (int k = 0; k <= L; ++k) gr[k] = gi[k] = 0.0;
     const int slot = blk * kBlock + T.lane;
-    cptr<VisitQ34> vis = as_const(FEAT ? M.vfeat : M.vadj);
+    cptr<VisitPack> vis = as_const(FEAT ? M.vfeat : M.vadj);
     const int vb = uniform(g.vbeg), ve = uniform(g.vend);
     // energy classes of the visits (sorted in this order within a row):
     // 0 linear order 3, 1 linear order 4, 2 polynomial features
@@ -340,9 +363,14 @@ __device__ __forceinline__ void q34_group(const Tile& T, const DevModel& M, cons
             dprev = D;
         }
     };
+    // descriptors one visit ahead: the next visit's load overlaps this one
+    VisitPack nxt{};
+    if (vb < ve) nxt = vis[vb];
     for (int vi = vb; vi < ve; ++vi) {
-        const VisitQ34 v = vis[vi];
-        const int order = uniform(v.order), flags = uniform(v.flags), pslot = uniform(v.pslot);
+        const VisitPack cur = nxt;
+        if (vi + 1 < ve) nxt = vis[vi + 1];
+        const VDec v = decode(cur);
+        const int order = v.order, flags = v.flags, pslot = v.pslot;
         const bool fast = (flags & 4) != 0;
         if (fast == DYN) continue;  // the other kernel's visit
         const int itc = uniform(v.itc);
