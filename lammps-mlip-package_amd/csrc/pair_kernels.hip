@@ -331,21 +331,29 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(T) ==
                 // exponentials costs more registers than it hides latency
                 __builtin_amdgcn_sched_barrier(0);
             }
-            // sums over lm in FP64 in both modes (terms in T)
+            // The gradient terms are linear in q = Qb_l^m and c Q' with
+            // coefficients that depend on m only, so each m column keeps three
+            // complex sums over l (in T)
+            //   Sp = sum_l q Gp,  Sq = sum_l q Gq,  Sd = sum_l c Q' Gq
+            // and folds them into er, t (FP64 in both modes) once per column:
+            //   er += w_m (-1)^m Re(Sp conj(u)^m)
+            //   m = 0: t += Re(Sd) (z x, z y, -(1-z^2))
+            //   m > 0: SA = m z Sq + (1-z^2) Sd, SB = m Sq - z Sd,
+            //          SE = m (1+z^2) Sq + z (1-z^2) Sd,
+            //          t += w_m (-1)^m (Re(-SB u+ + SE u-) / 2, Im(SB u+ + SE u-) / 2, -Re(SA u0))
+            //   with u0, u+, u- = conj(u)^m, conj(u)^(m+1), conj(u)^(m-1).
             double er = 0.0;                        // sum w Re(Gp Y)
             double tx = 0.0, ty = 0.0, tz = 0.0;    // sum w Re(Gq grad Y) * r
-            const T zz = T(1) - z * z;
+            const double xd = static_cast<double>(x), yd = static_cast<double>(y), zd = static_cast<double>(z);
+            const double zzd = 1.0 - zd * zd;
             T cr = T(1), ci = T(0);  // conj(u)^m
             T mr = T(0), mi = T(0);  // conj(u)^(m-1)
 #pragma unroll 1
             for (int m = 0; m <= maxl; ++m) {
-                const T sg = (m & 1) ? T(-1) : T(1);
-                const T w = (m == 0) ? T(1) : T(2);
-                const T ws = w * sg;
                 const T pr = cr * x + ci * y, pi = ci * x - cr * y;  // conj(u)^(m+1)
-                const T fm = static_cast<T>(m);
                 T a1 = T(0), a2 = T(0);  // Qb_{l-1}^m, Qb_{l-2}^m
                 T b1 = T(0), b2 = T(0);  // Qb_{l-1}^{m+1}, Qb_{l-2}^{m+1}
+                T spr = T(0), spi = T(0), sqr = T(0), sqi = T(0), sdr = T(0), sdi = T(0);
                 int lm = m * (m + 1) / 2;  // lmh(m, -m)
 #pragma unroll 1
                 for (int l = m; l <= maxl; ++l) {
@@ -370,34 +378,40 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(T) ==
                         qp = C.alm[l][m + 1] * (z * b1 - C.ainv[l][m + 1] * b2);
                     b2 = b1;
                     b1 = qp;
-                    // Y_{l,-m} = sg q conj(u)^m
-                    er += static_cast<double>(w * sg * q * (gpr * cr - gpi * ci));
-                    T gxr, gxi, gyr, gyi, gzr, gzi;
-                    if (m == 0) {
-                        const T c = C.grad[l][0] * qp;
-                        gxr = c * z * x;
-                        gxi = T(0);
-                        gyr = c * z * y;
-                        gyi = T(0);
-                        gzr = -c * zz;
-                        gzi = T(0);
-                    } else {
-                        const T c = C.grad[l][m];
-                        const T A = fm * z * q + zz * c * qp;
-                        const T B = fm * q - z * c * qp;
-                        const T E = fm * (T(1) + z * z) * q + z * zz * c * qp;
-                        gxr = T(0.5) * (-B * pr + E * mr);
-                        gxi = T(0.5) * (-B * pi + E * mi);
-                        const T zr = B * pr + E * mr, zi = B * pi + E * mi;
-                        gyr = T(0.5) * zi;
-                        gyi = T(-0.5) * zr;
-                        gzr = -A * cr;
-                        gzi = -A * ci;
-                    }
-                    tx += static_cast<double>(ws * (gqr * gxr - gqi * gxi));
-                    ty += static_cast<double>(ws * (gqr * gyr - gqi * gyi));
-                    tz += static_cast<double>(ws * (gqr * gzr - gqi * gzi));
+                    const T cq = C.grad[l][m] * qp;
+                    spr = fma(q, gpr, spr);
+                    spi = fma(q, gpi, spi);
+                    sqr = fma(q, gqr, sqr);
+                    sqi = fma(q, gqi, sqi);
+                    sdr = fma(cq, gqr, sdr);
+                    sdi = fma(cq, gqi, sdi);
                     lm += l + 2;  // lmh(l+1, -m) = lmh(l, -m) + l + 2
+                }
+                const double ws = (m == 0) ? 1.0 : ((m & 1) ? -2.0 : 2.0);  // w_m (-1)^m
+                const double c0r = static_cast<double>(cr), c0i = static_cast<double>(ci);
+                const double Spr = static_cast<double>(spr), Spi = static_cast<double>(spi);
+                const double Sdr = static_cast<double>(sdr), Sdi = static_cast<double>(sdi);
+                er += ws * (Spr * c0r - Spi * c0i);
+                if (m == 0) {
+                    tx += Sdr * zd * xd;
+                    ty += Sdr * zd * yd;
+                    tz -= Sdr * zzd;
+                } else {
+                    const double Sqr = static_cast<double>(sqr), Sqi = static_cast<double>(sqi);
+                    const double fm = static_cast<double>(m);
+                    const double kaq = fm * zd, kbq = fm, keq = fm * (1.0 + zd * zd);
+                    const double ked = zd * zzd;
+                    const double sar = kaq * Sqr + zzd * Sdr, sai = kaq * Sqi + zzd * Sdi;
+                    const double sbr = kbq * Sqr - zd * Sdr, sbi = kbq * Sqi - zd * Sdi;
+                    const double ser = keq * Sqr + ked * Sdr, sei = keq * Sqi + ked * Sdi;
+                    const double upr = static_cast<double>(pr), upi = static_cast<double>(pi);
+                    const double umr = static_cast<double>(mr), umi = static_cast<double>(mi);
+                    // SB u+ and SE u- (complex products)
+                    const double bpr = sbr * upr - sbi * upi, bpi = sbr * upi + sbi * upr;
+                    const double emr = ser * umr - sei * umi, emi = ser * umi + sei * umr;
+                    tx += 0.5 * ws * (emr - bpr);
+                    ty += 0.5 * ws * (bpi + emi);
+                    tz -= ws * (sar * c0r - sai * c0i);
                 }
                 mr = cr;
                 mi = ci;
