@@ -264,7 +264,7 @@ struct ForceLds {
 // where the workgroups' G tiles still fit the LDS.
 template <int NF, int NT, typename T>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 3))) void k_force(DevModel M, DevSystem S) {
-    extern __shared__ unsigned char force_dyn[];
+    extern __shared__ __attribute__((aligned(16))) unsigned char force_dyn[];
     T* Gs = reinterpret_cast<T*>(force_dyn);  // [ns][n_lmh][NF][2]
     __shared__ ForceLds<T> C;
     __shared__ double vred[6][NT / 64];
@@ -359,10 +359,31 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(T) ==
                 for (int l = m; l <= maxl; ++l) {
                     // contraction over n first: independent of the recursion
                     T gpr = T(0), gpi = T(0), gqr = T(0), gqi = T(0);
+                    // 16-byte LDS reads (one n in FP64, two in FP32)
                     const T* gl = Gsp + (size_t)lm * NF * 2;
+                    T grv[2 * NF];
+                    if constexpr (sizeof(T) == 8) {
+                        const double2* g2 = reinterpret_cast<const double2*>(gl);
+#pragma unroll
+                        for (int n = 0; n < NF; ++n) {
+                            const double2 v = g2[n];
+                            grv[2 * n] = v.x;
+                            grv[2 * n + 1] = v.y;
+                        }
+                    } else {
+                        const float4* g4 = reinterpret_cast<const float4*>(gl);
+#pragma unroll
+                        for (int h = 0; h < NF / 2; ++h) {
+                            const float4 v = g4[h];
+                            grv[4 * h] = v.x;
+                            grv[4 * h + 1] = v.y;
+                            grv[4 * h + 2] = v.z;
+                            grv[4 * h + 3] = v.w;
+                        }
+                    }
 #pragma unroll
                     for (int n = 0; n < NF; ++n) {
-                        const T grr = gl[2 * n], gii = gl[2 * n + 1];
+                        const T grr = grv[2 * n], gii = grv[2 * n + 1];
                         gpr = fma(gd[n], grr, gpr);
                         gpi = fma(gd[n], gii, gpi);
                         gqr = fma(gn[n], grr, gqr);
