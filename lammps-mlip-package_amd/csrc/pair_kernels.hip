@@ -99,7 +99,7 @@ struct AnlmLds {
     T ux[kAnlmThreads], uy[kAnlmThreads], uz[kAnlmThreads];
     double rr[kAnlmThreads], fc[kAnlmThreads];
     int spec[kAnlmThreads];
-    double g[kAnlmJ][kMaxFn];
+    alignas(16) double g[kAnlmJ][kMaxFn];
 };
 
 template <int NPT, typename T>
@@ -201,7 +201,15 @@ __global__ __launch_bounds__(kAnlmThreads) void k_anlm(DevModel M, DevSystem S, 
                     if (spec < 0) continue;
                     const T2 yv = ys[j * nlmh + lm2];
                     const double yr = static_cast<double>(yv.x), yi = static_cast<double>(yv.y);
-                    const double* gj = &L.g[j][nbase];
+                    // g_n by 16-byte LDS reads (NPT is even, nbase a multiple of NPT)
+                    const double2* gj2 = reinterpret_cast<const double2*>(&L.g[j][nbase]);
+                    double gj[NPT];
+#pragma unroll
+                    for (int h = 0; h < NPT / 2; ++h) {
+                        const double2 v = gj2[h];
+                        gj[2 * h] = v.x;
+                        gj[2 * h + 1] = v.y;
+                    }
                     if (spec == 0) {
 #pragma unroll
                         for (int k = 0; k < NPT; ++k) {

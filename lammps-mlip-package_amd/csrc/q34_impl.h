@@ -3,7 +3,7 @@
 //
 // Workgroup = kWavesPerWg waves on one (block of 64 same-type centres,
 // radial index n).  The a_nlm tile (l <= lmax3, both species) sits in LDS as
-// [s][lm][re,im][lane]; each wave owns (S, L) output groups (or balanced
+// [s][lm][lane](re, im); each wave owns (S, L) output groups (or balanced
 // parts of heavy ones) and keeps G_{S,L,m<=0} in registers.  Visits with
 // fully unrolled signatures run in one kernel, the rest in a second
 // general-loop kernel (only launched for models that have such visits), so
@@ -27,14 +27,15 @@ using namespace dev;
 constexpr int kQ34Threads = 64 * kWavesPerWg;
 
 struct Tile {
-    const double* As;
+    const double2* As;  // [s][lm][lane] (re, im): one 16-byte LDS read per entry
     int nlmh3;
     int lane;
     __device__ __forceinline__ void get(int s, int l, int m, double& re, double& im) const {
         const int am = m < 0 ? -m : m;
         const int idx = s * nlmh3 + l * (l + 1) / 2 + l - am;
-        re = As[(2 * idx) * kBlock + lane];
-        im = As[(2 * idx + 1) * kBlock + lane];
+        const double2 v = As[idx * kBlock + lane];
+        re = v.x;
+        im = v.y;
         if (m > 0) {
             const double sg = (m & 1) ? -1.0 : 1.0;
             re *= sg;
@@ -452,18 +453,23 @@ constexpr int kMaxLUnrolled = MLIP_CAP3;
 
 template <bool FEAT, bool DYN>
 __global__ __launch_bounds__(kQ34Threads) void k_q34(DevModel M, DevSystem S) {
-    extern __shared__ double As[];
+    extern __shared__ __attribute__((aligned(16))) double As[];
     const int blk = blockIdx.x, n = blockIdx.y;
     const int t = S.block_type[blk];
-    const int per_s = M.n_lmh3 * 2 * kBlock;
+    const int per_s = M.n_lmh3 * kBlock;  // entries (re, im) per species
+    double2* As2 = reinterpret_cast<double2*>(As);
     for (int s = 0; s < M.n_type; ++s) {
+        // global [lm][re,im][lane] -> LDS [lm][lane](re, im)
         const double* src = S.A + a_index(M, blk, n, s, 0, 0, 0);
-        for (int k = threadIdx.x; k < per_s; k += kQ34Threads) As[s * per_s + k] = src[k];
+        for (int k = threadIdx.x; k < per_s; k += kQ34Threads) {
+            const int lm = k / kBlock, ln = k % kBlock;
+            As2[s * per_s + k] = double2{src[2 * lm * kBlock + ln], src[(2 * lm + 1) * kBlock + ln]};
+        }
     }
-    double* red = As + M.n_type * per_s;  // partial rows of split groups
+    double* red = As + 2 * M.n_type * per_s;  // partial rows of split groups
     __syncthreads();
     const int wave = uniform(threadIdx.x >> 6);
-    Tile T{As, M.n_lmh3, static_cast<int>(threadIdx.x & 63)};
+    Tile T{As2, M.n_lmh3, static_cast<int>(threadIdx.x & 63)};
     cptr<GroupQ34> groups = as_const(FEAT ? M.gfeat : M.gadj);
     const int gb = FEAT ? M.gfeat_beg[t] : M.gadj_beg[t];
     const int ge = FEAT ? M.gfeat_end[t] : M.gadj_end[t];
