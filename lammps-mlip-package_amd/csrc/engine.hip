@@ -1100,10 +1100,12 @@ int mlip_profile_read(mlip_ctx* c, const char** names, double* ms, int* count, i
 // structure, per kernel (adds and multiplies count 1, fused multiply-adds 2):
 //   [0] a_nlm       per pair 4 n_fn n_lmh (accumulate) + 8 n_lmh (Y_lm) + 8 n_fn
 //   [1] order 1/2   per centre and n: 4 per order-1 family, 20 (L+1) per order-2 family
-//   [2] order 3/4 polynomial features  per valid term 11 (order 3) / 17 (order 4)
-//   [3] order 3/4 adjoint              per valid term 11 / 17, +5 where d is formed
+//   [2] order 3/4 polynomial features  per valid term 11, + 6 per (m_a, m_b) at order 4
+//   [3] order 3/4 adjoint              per valid term 10, + 6 per (m_a, m_b) at order 4,
+//                                      + 2 per m_a, + 10 (L+1) per group (energy by Euler)
 //   [4] polynomial  2 n_p^2 (quadratic form) + 6 n_p + 4 per cubic entry, per centre
-//   [5] forces      per pair 8 n_fn n_lmh (contraction) + 40 n_lmh + 14 n_fn
+//   [5] forces      per pair 8 n_fn n_lmh (contraction) + 21 n_lmh (recursion and
+//                   the three column sums) + 60 (maxl + 1) (per-column fold) + 14 n_fn
 //   [6] total
 int mlip_flops(const mlip_ctx* c, double* out7) {
     if (!c || !out7) return MLIP_ERR_ARG;
@@ -1111,7 +1113,7 @@ int mlip_flops(const mlip_ctx* c, double* out7) {
     for (int k = 0; k < 7; ++k) out7[k] = 0.0;
     const double pairs = static_cast<double>(c->npairs);
     out7[0] = pairs * (4.0 * m.n_fn * m.n_lmh + 8.0 * m.n_lmh + 8.0 * m.n_fn);
-    out7[5] = pairs * (8.0 * m.n_fn * m.n_lmh + 40.0 * m.n_lmh + 14.0 * m.n_fn);
+    out7[5] = pairs * (8.0 * m.n_fn * m.n_lmh + 21.0 * m.n_lmh + 60.0 * (m.maxl + 1) + 14.0 * m.n_fn);
     for (int t = 0; t < m.n_type && t < static_cast<int>(c->owners_per_type.size()); ++t) {
         const double na = static_cast<double>(c->owners_per_type[t]) * m.n_fn;
         const TypeTables& tt = m.types[t];
