@@ -381,10 +381,11 @@ def main():
                 "executed_tflops": exe_fl / (dom_ms * 1e-3) / 1e12 if fl_key else None,
                 "traffic_source": traffic["source"] if traffic else None,
                 "note": "FP64 VALU work (the MI355X FP64 peak is the same for vector and matrix units); the "
-                        "kernel pairs m with -m, shares partial products and takes the energy of linear "
-                        "families from the adjoint by Euler's theorem, so it executes 3.5x fewer operations "
-                        "than the reference formulation counts: frac above 1 by the rule, executed_tflops / "
-                        "peak for how busy the FP64 pipes are"}
+                        "kernel pairs m with -m, shares partial products, merges factor positions with identical "
+                        "signatures and takes the energy of linear families from the adjoint by Euler's theorem, "
+                        "so it executes %.1fx fewer operations than the reference formulation counts: frac above "
+                        "1 by the rule, executed_tflops / peak for how busy the FP64 pipes are"
+                        % ((ref_fl / exe_fl) if (fl_key and exe_fl) else float("nan"))}
     total_rate = flops["total"] * args.steps / dt / 1e12
     log("steps %d in %.3fs -> %.1f ms/step, %.4g atom-evals/s; whole-step FP64 rate %.2f TFLOP/s"
         % (args.steps, dt, 1e3 * dt / args.steps, value, total_rate))
