@@ -89,6 +89,7 @@ __device__ __forceinline__ PairDel pair_del(const DevModel& M, const DevSystem& 
 constexpr int kAnlmThreads = 192;
 constexpr int kAnlmJ = 16;
 constexpr int kAnlmParts = kAnlmThreads / kAnlmJ;  // 12
+static_assert(kAnlmJ <= 64, "a chunk's neighbor species fit one per lane of a wave");
 
 template <typename T>
 struct AnlmLds {
@@ -195,9 +196,14 @@ __global__ __launch_bounds__(kAnlmThreads) void k_anlm(DevModel M, DevSystem S, 
             }
             __syncthreads();
             const int nc = min(kAnlmJ, nb - c0);
+            // species of the chunk's neighbors, one per lane: each neighbor's
+            // branch reads a register (readlane), not an LDS round trip
+            // (nc <= kAnlmJ <= 64, so lanes 0..nc-1 of every wave hold them)
+            const int wl = tid & 63;
+            const int specv = wl < nc ? L.spec[c0 + wl] : -1;
             if (act2) {
                 for (int j = 0; j < nc; ++j) {
-                    const int spec = __builtin_amdgcn_readfirstlane(L.spec[c0 + j]);
+                    const int spec = __builtin_amdgcn_readlane(specv, j);
                     if (spec < 0) continue;
                     const T2 yv = ys[j * nlmh + lm2];
                     const double yr = static_cast<double>(yv.x), yi = static_cast<double>(yv.y);
