@@ -44,9 +44,16 @@ struct DevModel {
     // zlist[zoff[t] .. zoff[t+1]) are set to zero by the order-1/2 feature pass
     const int32_t* zlist;
     int32_t zoff[3];
-    const double* lin_p;    // linear weights of polynomial features (n_p)
-    const int32_t* c3_off;  // cubic terms transposed by target: offsets (n_p+1)
-    const int32_t* c3_bc;   // 2 per entry
+    // Polynomial features on the device are per species and compact: a
+    // centre of species t keeps the ranks [lo, hi) = prange[t] it sees, as
+    // the row [n][rank - lo] of n_fn * (hi - lo) doubles in a slot row of
+    // n_pr (the largest over species); lin_p ([t][n_pr]) and c3_off
+    // ([t][n_pr + 1]) use the same per-species columns.
+    int32_t prange[2][2];
+    int32_t n_pr;
+    const double* lin_p;    // linear weights of polynomial features ([species][n_pr]; n_p for mlip_pair)
+    const int32_t* c3_off;  // cubic terms transposed by target: offsets ([species][n_pr + 1])
+    const int32_t* c3_bc;   // 2 per entry (columns of the same species)
     const double* c3_w;
     // radial-only style
     int pair_model;         // 1: features sum_j g_n(r) per type combination

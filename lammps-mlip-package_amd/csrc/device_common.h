@@ -49,6 +49,12 @@ __device__ __forceinline__ size_t a_index(const DevModel& M, int blk, int n, int
 
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
+// Column of polynomial feature (n, rank) in the compact row of a centre of
+// species t (the rank lies in the species' range prange[t]).
+__device__ __forceinline__ int pcol(const DevModel& M, int t, int n, int rank) {
+    return n * (M.prange[t][1] - M.prange[t][0]) + rank - M.prange[t][0];
+}
+
 
 }  // namespace dev
 }  // namespace mlip

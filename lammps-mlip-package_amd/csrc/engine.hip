@@ -51,6 +51,8 @@ bool upload_model(mlip_ctx* c) {
     dm.lmax3 = m.lmax3;
     dm.n_pitc = m.n_pitc;
     dm.n_p = m.n_p;
+    for (int t = 0; t < 2; ++t)
+        for (int k = 0; k < 2; ++k) dm.prange[t][k] = m.prange[t][k];
     dm.has_q34 = m.has_order34 ? 1 : 0;
     dm.q34_red = m.q34_red;
     dm.pair_model = m.pair_model ? 1 : 0;
@@ -105,7 +107,16 @@ bool upload_model(mlip_ctx* c) {
         c->err = "order-3/4 visit outside the packed range";
         return false;
     }
-    // feature slots no visit of a species writes (zeroed by k_q12<true>)
+    // Polynomial features per species, compact: species t keeps the ranks
+    // [lo, hi) = prange[t] it sees (the ranks are grouped so that this is
+    // one range), column ci(n, r) = n (hi - lo) + r - lo, in a slot row of
+    // n_pr doubles; the model's tables use the feature n * n_pitc + r.
+    int n_pr = 0;
+    for (int t = 0; t < m.n_type && t < 2; ++t)
+        n_pr = std::max(n_pr, m.n_fn * (m.prange[t][1] - m.prange[t][0]));
+    dm.n_pr = n_pr;
+    // feature ranks of the range no visit of a species writes (zeroed by
+    // k_q12<true>), as offsets r - lo
     std::vector<int32_t> zlist;
     for (int t = 0; t < m.n_type && t < 2; ++t) {
         std::vector<char> written(static_cast<size_t>(m.n_pitc), 0);
@@ -114,35 +125,61 @@ bool upload_model(mlip_ctx* c) {
         for (const VisitQ34& v : m.types[t].visits_feat)
             if (v.pslot >= 0) written[v.pslot] = 1;
         dm.zoff[t] = static_cast<int32_t>(zlist.size());
-        for (int r = 0; r < m.n_pitc; ++r)
-            if (!written[r]) zlist.push_back(r);
+        for (int r = 0; r < m.n_pitc; ++r) {
+            const bool in = r >= m.prange[t][0] && r < m.prange[t][1];
+            if (written[r] && !in) {
+                c->err = "internal: polynomial feature outside its species' rank range";
+                return false;
+            }
+            if (in && !written[r]) zlist.push_back(r - m.prange[t][0]);
+        }
     }
     for (int t = m.n_type; t < 2; ++t) dm.zoff[t] = static_cast<int32_t>(zlist.size());
     dm.zoff[2] = static_cast<int32_t>(zlist.size());
-    // cubic terms transposed by target feature
-    std::vector<std::vector<std::pair<std::array<int, 2>, double>>> c3(m.n_p);
-    for (size_t k = 0; k < m.c3_w.size(); ++k) {
-        const int a = m.c3_idx[3 * k], b = m.c3_idx[3 * k + 1], cc = m.c3_idx[3 * k + 2];
-        const double w = m.c3_w[k];
-        c3[a].push_back({{b, cc}, w});
-        c3[b].push_back({{a, cc}, w});
-        c3[cc].push_back({{a, b}, w});
-    }
-    std::vector<int32_t> c3off(m.n_p + 1, 0), c3bc;
-    std::vector<double> c3w;
-    for (int a = 0; a < m.n_p; ++a) {
-        c3off[a] = static_cast<int>(c3w.size());
-        for (const auto& e : c3[a]) {
-            c3bc.push_back(e.first[0]);
-            c3bc.push_back(e.first[1]);
-            c3w.push_back(e.second);
+    // per species: quadratic form on its columns (dense, for one GEMM per
+    // species), linear weights, cubic terms transposed by target column
+    // (terms with a factor the species does not see vanish and are dropped)
+    std::vector<double> wc, linc(static_cast<size_t>(2) * n_pr, 0.0), c3w;
+    std::vector<int32_t> c3off(static_cast<size_t>(2) * (n_pr + 1), 0), c3bc;
+    for (int t = 0; t < 2; ++t) {
+        const bool on = t < m.n_type && m.n_p > 0;
+        const int lo = m.prange[t][0], wr = on ? m.prange[t][1] - lo : 0, wf = m.n_fn * wr;
+        c->wofs[t] = wc.size();
+        c->wdim[t] = wf;
+        std::vector<int> gof(static_cast<size_t>(wf));  // compact column -> model feature
+        std::vector<int> cof(static_cast<size_t>(m.n_p), -1);  // model feature -> compact column
+        for (int ci = 0; ci < wf; ++ci) {
+            gof[ci] = (ci / wr) * m.n_pitc + lo + ci % wr;
+            cof[gof[ci]] = ci;
+        }
+        for (int ci = 0; ci < wf; ++ci)
+            for (int cj = 0; cj < wf; ++cj)
+                wc.push_back(m.wmat[static_cast<size_t>(gof[ci]) * m.n_p + gof[cj]]);
+        for (int ci = 0; ci < wf; ++ci) linc[static_cast<size_t>(t) * n_pr + ci] = m.lin_p[gof[ci]];
+        std::vector<std::vector<std::pair<std::array<int, 2>, double>>> c3(static_cast<size_t>(wf));
+        for (size_t k = 0; on && k < m.c3_w.size(); ++k) {
+            const int a = cof[m.c3_idx[3 * k]], b = cof[m.c3_idx[3 * k + 1]], cc = cof[m.c3_idx[3 * k + 2]];
+            if (a < 0 || b < 0 || cc < 0) continue;
+            const double w = m.c3_w[k];
+            c3[a].push_back({{b, cc}, w});
+            c3[b].push_back({{a, cc}, w});
+            c3[cc].push_back({{a, b}, w});
+        }
+        int32_t* off = c3off.data() + static_cast<size_t>(t) * (n_pr + 1);
+        for (int ci = 0; ci <= n_pr; ++ci) {
+            off[ci] = static_cast<int32_t>(c3w.size());
+            if (ci < wf)
+                for (const auto& e : c3[ci]) {
+                    c3bc.push_back(e.first[0]);
+                    c3bc.push_back(e.first[1]);
+                    c3w.push_back(e.second);
+                }
         }
     }
-    if (m.n_p > 0) c3off[m.n_p] = static_cast<int>(c3w.size());
     bool ok = upload(c, c->d_coef, m.coef.data(), m.coef.size(), "coefficients") &&
               upload(c, c->d_weight, m.weight.data(), m.weight.size(), "weights") &&
-              upload(c, c->d_wmat, m.wmat.data(), m.wmat.size(), "quadratic form") &&
-              upload(c, c->d_linp, m.lin_p.data(), m.lin_p.size(), "linear poly weights") &&
+              upload(c, c->d_wmat, wc.data(), wc.size(), "quadratic form") &&
+              upload(c, c->d_linp, linc.data(), linc.size(), "linear poly weights") &&
               upload(c, c->d_c3w, c3w.data(), c3w.size(), "cubic weights") &&
               upload(c, c->d_c3bc, c3bc.data(), c3bc.size(), "cubic index") &&
               upload(c, c->d_c3off, c3off.data(), c3off.size(), "cubic offsets") &&
@@ -226,6 +263,8 @@ bool setup_layout(mlip_ctx* c, int nall, int nown, int64_t npairs, const std::ve
     std::vector<int32_t> atom_of_slot, slot_of_atom, block_type;
     assign_slots(c, nown, elem_own, atom_of_slot, slot_of_atom, block_type);
     const int nblocks = static_cast<int>(block_type.size());
+    c->nblk_type[0] = c->nblk_type[1] = 0;
+    for (int32_t t : block_type) ++c->nblk_type[t];
     c->owners_per_type.assign(m.n_type, 0);
     for (int e : elem_own)
         if (e >= 0 && e < m.n_type) ++c->owners_per_type[e];
@@ -427,12 +466,25 @@ bool run_pipeline(mlip_ctx* c) {
         if (mm.has_feat34) { Scope sc(c, "q34_feat"); launch_q34_feat(m, s, st); }
         {
             Scope sc(c, "poly_gemm");
+            // Hq = D W per species: slot blocks are grouped by species and a
+            // species' row holds only the features it sees, so each species
+            // is one dense GEMM on its rows and its own quadratic form
             const double one = 1.0, zero = 0.0;
-            const int nslot = s.nblocks * kBlock;
-            if (nslot > 0 &&
-                rocblas_dgemm(c->blas, rocblas_operation_none, rocblas_operation_none, m.n_p, nslot, m.n_p, &one,
-                              c->d_wmat.p, m.n_p, s.D, m.n_p, &zero, s.Hq, m.n_p) != rocblas_status_success) {
-                c->err = "rocblas_dgemm failed";
+            int b0 = 0;
+            for (int t = 0; t < mm.n_type && t < 2; ++t) {
+                const int nslot_t = c->nblk_type[t] * kBlock, w = c->wdim[t];
+                const size_t r0 = static_cast<size_t>(b0) * kBlock * m.n_pr;
+                b0 += c->nblk_type[t];
+                if (nslot_t == 0 || w == 0) continue;
+                if (rocblas_dgemm(c->blas, rocblas_operation_none, rocblas_operation_none, w, nslot_t, w, &one,
+                                  c->d_wmat.p + c->wofs[t], w, s.D + r0, m.n_pr, &zero, s.Hq + r0, m.n_pr) !=
+                    rocblas_status_success) {
+                    c->err = "rocblas_dgemm failed";
+                    return false;
+                }
+            }
+            if (b0 != s.nblocks) {
+                c->err = "internal: slot blocks per species do not add up";
                 return false;
             }
         }
@@ -1103,7 +1155,8 @@ int mlip_profile_read(mlip_ctx* c, const char** names, double* ms, int* count, i
 //   [2] order 3/4 polynomial features  per valid term 11, + 6 per (m_a, m_b) at order 4
 //   [3] order 3/4 adjoint              per valid term 10, + 6 per (m_a, m_b) at order 4,
 //                                      + 2 per m_a, + 10 (L+1) per group (energy by Euler)
-//   [4] polynomial  2 n_p^2 (quadratic form) + 6 n_p + 4 per cubic entry, per centre
+//   [4] polynomial  2 w^2 (quadratic form) + 6 w + 4 per cubic entry, per centre, w = the
+//                   feature columns its species sees
 //   [5] forces      per pair 8 n_fn n_lmh (contraction) + 21 n_lmh (recursion and
 //                   the three column sums) + 60 (maxl + 1) (per-column fold) + 14 n_fn
 //   [6] total
@@ -1156,8 +1209,8 @@ int mlip_flops(const mlip_ctx* c, double* out7) {
         };
         out7[2] += na * terms(tt.groups_feat, tt.visits_feat, false);
         out7[3] += na * terms(tt.groups_adj, tt.visits_adj, true);
-        const double np = m.n_p;
-        out7[4] += static_cast<double>(c->owners_per_type[t]) * (np > 0 ? 2.0 * np * np + 6.0 * np + 12.0 * m.c3_w.size() / np : 0.0);
+        const double np = m.n_p, w = static_cast<double>(m.prange[t][1] - m.prange[t][0]) * m.n_fn;
+        out7[4] += static_cast<double>(c->owners_per_type[t]) * (np > 0 ? 2.0 * w * w + 6.0 * w + 12.0 * m.c3_w.size() / np : 0.0);
     }
     for (int k = 0; k < 6; ++k) out7[6] += out7[k];
     return MLIP_OK;

@@ -89,6 +89,9 @@ struct mlip_ctx {
     int64_t npairs = 0;
     // launch shape chosen by mlip_autotune
     int force_threads = 192;
+    int nblk_type[2] = {0, 0};  // slot blocks of each species (blocks are grouped by species)
+    size_t wofs[2] = {0, 0};    // per-species quadratic form in d_wmat: offset and dimension
+    int wdim[2] = {0, 0};
     // batch of independent structures (0 = single structure): per-structure
     // atom ranges [sbeg, send) and 7 results each
     int nstruct = 0;

@@ -60,7 +60,7 @@ __device__ void q12_level(const DevModel& M, const DevSystem& S, int blk, int n,
         const int s0 = uniform(v.s0), s1 = uniform(v.s1);
         const double* cp = M.coef + uniform(v.coef);
         double H = 0.0;
-        if (!FEAT) H = pslot >= 0 ? S.H[(size_t)slot * M.n_p + n * M.n_pitc + pslot] : M.weight[n * M.n_gtinv + v.itc];
+        if (!FEAT) H = pslot >= 0 ? S.H[(size_t)slot * M.n_pr + pcol(M, t, n, pslot)] : M.weight[n * M.n_gtinv + v.itc];
         const double f0 = s0 == 0 ? 1.0 : 0.0, f1 = 1.0 - f0;
         const double e0 = s1 == 0 ? 1.0 : 0.0, e1 = 1.0 - e0;
         double d = 0.0;
@@ -98,7 +98,7 @@ __device__ void q12_level(const DevModel& M, const DevSystem& S, int blk, int n,
             }
         }
         if (FEAT) {
-            S.D[(size_t)slot * M.n_p + n * M.n_pitc + pslot] = d;
+            S.D[(size_t)slot * M.n_pr + pcol(M, t, n, pslot)] = d;
         } else if (pslot < 0) {
             eacc = fma(M.weight[n * M.n_gtinv + v.itc], d, eacc);
         }
@@ -137,7 +137,7 @@ __global__ __launch_bounds__(64) void k_q12(DevModel M, DevSystem S) {
         // features of families the centre's species does not see are zero
         // (the reference never accumulates them); the visits below and the
         // order-3/4 feature kernel write every other slot of the row
-        double* row = S.D + (size_t)(blk * kBlock + lane) * M.n_p + n * M.n_pitc;
+        double* row = S.D + (size_t)(blk * kBlock + lane) * M.n_pr + pcol(M, t, n, M.prange[t][0]);
         const int z0 = M.zoff[t], z1 = M.zoff[t + 1];
         for (int r = z0; r < z1; ++r) row[M.zlist[r]] = 0.0;
     }
@@ -155,14 +155,20 @@ __global__ __launch_bounds__(256) void k_poly(DevModel M, DevSystem S) {
     __shared__ double red[4][64];
     double e = 0.0;
     if (slot < S.nblocks * kBlock) {
-        const double* d = S.D + (size_t)slot * M.n_p;
-        const double* hq = S.Hq + (size_t)slot * M.n_p;
-        double* h = S.H + (size_t)slot * M.n_p;
-        for (int a = lane; a < M.n_p; a += 64) {
+        // the compact row of the slot's species (features it does not see
+        // are zero and have no column)
+        const int t = S.block_type[slot / kBlock];
+        const double* d = S.D + (size_t)slot * M.n_pr;
+        const double* hq = S.Hq + (size_t)slot * M.n_pr;
+        double* h = S.H + (size_t)slot * M.n_pr;
+        const int w = M.n_fn * (M.prange[t][1] - M.prange[t][0]);
+        const int32_t* c3o = M.c3_off + t * (M.n_pr + 1);
+        const double* lin = M.lin_p + (size_t)t * M.n_pr;
+        for (int a = lane; a < w; a += 64) {
             double h3 = 0.0;
-            for (int k = M.c3_off[a]; k < M.c3_off[a + 1]; ++k)
+            for (int k = c3o[a]; k < c3o[a + 1]; ++k)
                 h3 = fma(M.c3_w[k], d[M.c3_bc[2 * k]] * d[M.c3_bc[2 * k + 1]], h3);
-            const double la = M.lin_p[a];
+            const double la = lin[a];
             h[a] = la + hq[a] + h3;
             e = fma(d[a], la + 0.5 * hq[a] + h3 / 3.0, e);
         }

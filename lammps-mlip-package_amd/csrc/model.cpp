@@ -816,10 +816,27 @@ bool load_model(const std::string& path, const GtinvTable& table, Model& m, std:
     {
         std::vector<char> isp(m.n_gtinv, 0);
         for (int f : m.poly_features) isp[f % m.n_gtinv] = 1;
-        int r = 0;
-        for (int i = 0; i < m.n_gtinv; ++i)
-            if (isp[i]) m.prank[i] = r++;
+        // ranks grouped by the centre species that see the family: species 0
+        // only, both, species 1 only, neither (coefficient order inside a
+        // group), so each species' features are one contiguous rank range
+        // and the quadratic form of a species' centres is a dense sub-block
+        auto sees = [&](int i, int t) { return t < m.n_type && ((m.itcs[i].type_mask >> t) & 1u) != 0; };
+        auto group = [&](int i) {
+            const bool a = sees(i, 0), b = sees(i, 1);
+            return a && !b ? 0 : (a && b ? 1 : (b ? 2 : 3));
+        };
+        int r = 0, cnt[4] = {0, 0, 0, 0};
+        for (int gp = 0; gp < 4; ++gp)
+            for (int i = 0; i < m.n_gtinv; ++i)
+                if (isp[i] && group(i) == gp) {
+                    m.prank[i] = r++;
+                    ++cnt[gp];
+                }
         m.n_pitc = r;
+        m.prange[0][0] = 0;
+        m.prange[0][1] = cnt[0] + cnt[1];
+        m.prange[1][0] = cnt[0];
+        m.prange[1][1] = cnt[0] + cnt[1] + cnt[2];
         m.n_p = m.n_fn * m.n_pitc;
         // every feature of a polynomial itc must be in the list for all n
         if (static_cast<int>(m.poly_features.size()) != m.n_p) {

@@ -140,7 +140,8 @@ struct Model {
     int n_lmh3 = 0;            // same for lmax3
     int n_pitc = 0;            // itcs whose features enter polynomial products
     std::vector<int> prank;    // itc -> rank among polynomial itcs or -1
-    int n_p = 0;               // n_fn * n_pitc
+    int n_p = 0;               // n_fn * n_pitc; feature (n, rank) = n * n_pitc + rank
+    int prange[2][2] = {{0, 0}, {0, 0}};  // ranks a centre species sees: [lo, hi)
     std::vector<double> wmat;  // n_p x n_p symmetric quadratic form
     std::vector<int32_t> c3_idx;  // comb3 in polynomial-feature space, 3 per term
     std::vector<double> c3_w;

@@ -679,6 +679,8 @@ int mlip_pool_tick(mlip_ctx* c, int* nbatch) {
                            st, p->first.p, p->elem.p, p->mass.p, p->f.p, p->v.p, p->x.p);
     hipLaunchKernelGGL(k_scatter_slots, dim3((nslot + 255) / 256), dim3(256), 0, st, p->atom_of_slot.p, nslot,
                        p->slot_of_atom.p);
+    c->nblk_type[0] = c->nblk_type[1] = 0;
+    for (int32_t t : btype) ++c->nblk_type[t];
     bind_pool(c, p, static_cast<int>(btype.size()));
     c->nstruct = nb;
     c->sbeg = p->sbeg.p;

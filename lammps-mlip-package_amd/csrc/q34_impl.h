@@ -351,6 +351,10 @@ __device__ __forceinline__ void q34_group(const Tile& T, const DevModel& M, cons
 #pragma unroll
     for (int k = 0; k <= L; ++k) gr[k] = gi[k] = 0.0;
     const int slot = blk * kBlock + T.lane;
+    // polynomial features and H of this slot: compact row of its species,
+    // column pcol(n, rank) = prow + rank
+    const int tsp = uniform(S.block_type[blk]);
+    const int64_t prow = (int64_t)slot * M.n_pr + pcol(M, tsp, n, 0);
     cptr<VisitPack> vis = as_const(FEAT ? M.vfeat : M.vadj);
     const int vb = uniform(g.vbeg), ve = uniform(g.vend);
     // energy classes of the visits (sorted in this order within a row):
@@ -395,11 +399,11 @@ __device__ __forceinline__ void q34_group(const Tile& T, const DevModel& M, cons
             } else {
                 Unrolled<L, false, true>::run(T, v, Sx, order, la, lb, lc, cg, cd, 0.0, gr, gi, d);
             }
-            S.D[(size_t)slot * M.n_p + n * M.n_pitc + pslot] = d;
+            S.D[prow + pslot] = d;
             continue;
         }
         const double H =
-            pslot >= 0 ? S.H[(size_t)slot * M.n_p + n * M.n_pitc + pslot] : as_const(M.weight)[n * M.n_gtinv + itc];
+            pslot >= 0 ? S.H[prow + pslot] : as_const(M.weight)[n * M.n_gtinv + itc];
         if constexpr (DYN) {
             if (order == 3)
                 visit3<L, true, false>(T, M, v, Sx, H, gr, gi, d);
