@@ -114,6 +114,8 @@ bool upload_model(mlip_ctx* c) {
     int n_pr = 0;
     for (int t = 0; t < m.n_type && t < 2; ++t)
         n_pr = std::max(n_pr, m.n_fn * (m.prange[t][1] - m.prange[t][0]));
+    n_pr = (n_pr + 15) / 16 * 16;  // rows start on 128-byte lines
+    if (n_pr > m.n_p) n_pr = m.n_p;  // the row buffers hold n_p per slot
     dm.n_pr = n_pr;
     // feature ranks of the range no visit of a species writes (zeroed by
     // k_q12<true>), as offsets r - lo
