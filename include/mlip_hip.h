@@ -184,8 +184,9 @@ int mlip_flops_reference(const mlip_ctx* ctx, double* out7);
 int mlip_calibration_copy(mlip_ctx* ctx, int64_t nbytes, int reps);
 
 /* Diagnostics for parity tests: copy device intermediates of the last
- * evaluation to host.  what = 0: a_nlm [slot order], 1: polynomial features,
- * 2: per-centre energies; returns the number of doubles written. */
+ * evaluation to host.  what = 0: a_nlm [slot order], 1: polynomial features
+ * (per slot a compact row of the features its species sees, in the engine's
+ * column order), 2: per-centre energies; returns the number of doubles written. */
 int64_t mlip_debug_fetch(mlip_ctx* ctx, int what, double* out, int64_t cap);
 
 /* Model description: n_type, n_fn, n_gtinv, n_des, n_coeff, maxl, lmax3,
