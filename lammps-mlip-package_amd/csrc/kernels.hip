@@ -18,6 +18,7 @@
 // m > 0 entries follow from a_{l,m} = (-1)^m conj(a_{l,-m}).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 
@@ -32,11 +33,13 @@ using namespace dev;
 
 
 // ---------------------------------------------------------------- order 1/2
+constexpr int kQ12Pitch = kBlock + 1;           // LDS tile pitch (doubles) of the staged feature pass
+constexpr size_t kQ12MaxStage = 32 * 1024;      // largest staged tile (bytes)
 // Wave = 64 slots of one block at one radial index n; a_{s,L,m} for the
 // current L is held in registers, G_{s,L,m} accumulated in registers.
 template <int L, bool FEAT>
 __device__ void q12_level(const DevModel& M, const DevSystem& S, int blk, int n, int lane, int t,
-                          double& eacc) {
+                          double& eacc, double* tile) {
     double ar[2][L + 1], ai[2][L + 1], gr[2][L + 1], gi[2][L + 1];
     const int ns = M.n_type;
 #pragma unroll
@@ -98,7 +101,10 @@ __device__ void q12_level(const DevModel& M, const DevSystem& S, int blk, int n,
             }
         }
         if (FEAT) {
-            S.D[(size_t)slot * M.n_pr + pcol(M, t, n, pslot)] = d;
+            if (tile)
+                tile[(pslot - M.prange[t][0]) * kQ12Pitch + lane] = d;
+            else
+                S.D[(size_t)slot * M.n_pr + pcol(M, t, n, pslot)] = d;
         } else if (pslot < 0) {
             eacc = fma(M.weight[n * M.n_gtinv + v.itc], d, eacc);
         }
@@ -118,31 +124,57 @@ __device__ void q12_level(const DevModel& M, const DevSystem& S, int blk, int n,
 template <int L, int LMAX, bool FEAT>
 struct Q12Levels {
     __device__ static void run(const DevModel& M, const DevSystem& S, int blk, int n, int lane, int t,
-                               double& e) {
-        if (L <= M.maxl) q12_level<L, FEAT>(M, S, blk, n, lane, t, e);
-        Q12Levels<L + 1, LMAX, FEAT>::run(M, S, blk, n, lane, t, e);
+                               double& e, double* tile) {
+        if (L <= M.maxl) q12_level<L, FEAT>(M, S, blk, n, lane, t, e, tile);
+        Q12Levels<L + 1, LMAX, FEAT>::run(M, S, blk, n, lane, t, e, tile);
     }
 };
 template <int LMAX, bool FEAT>
 struct Q12Levels<LMAX + 1, LMAX, FEAT> {
-    __device__ static void run(const DevModel&, const DevSystem&, int, int, int, int, double&) {}
+    __device__ static void run(const DevModel&, const DevSystem&, int, int, int, int, double&, double*) {}
 };
 
+// The feature pass (stage = 1) collects the block's row segments of radial
+// index n in LDS, [rank - lo][lane] with a pitch of kQ12Pitch, zeroed first
+// (features no visit writes stay zero), and writes them out row by row in
+// whole lines; without staging each lane writes its own row directly.
 template <bool FEAT>
-__global__ __launch_bounds__(64) void k_q12(DevModel M, DevSystem S) {
+__global__ __launch_bounds__(64) void k_q12(DevModel M, DevSystem S, int stage) {
+    extern __shared__ double q12_tile[];
     const int blk = blockIdx.x, n = blockIdx.y, lane = threadIdx.x;
     const int t = S.block_type[blk];
+    const int w = M.prange[t][1] - M.prange[t][0];
+    double* tile = (FEAT && stage) ? q12_tile : nullptr;
     double e = 0.0;
     if (FEAT) {
-        // features of families the centre's species does not see are zero
-        // (the reference never accumulates them); the visits below and the
-        // order-3/4 feature kernel write every other slot of the row
-        double* row = S.D + (size_t)(blk * kBlock + lane) * M.n_pr + pcol(M, t, n, M.prange[t][0]);
-        const int z0 = M.zoff[t], z1 = M.zoff[t + 1];
-        for (int r = z0; r < z1; ++r) row[M.zlist[r]] = 0.0;
+        if (tile) {
+            for (int k = lane; k < w * kQ12Pitch; k += 64) tile[k] = 0.0;
+            __syncthreads();
+        } else {
+            // features of families the centre's species does not see are zero
+            // (the reference never accumulates them); the visits below and the
+            // order-3/4 feature kernel write every other slot of the row
+            double* row = S.D + (size_t)(blk * kBlock + lane) * M.n_pr + pcol(M, t, n, M.prange[t][0]);
+            const int z0 = M.zoff[t], z1 = M.zoff[t + 1];
+            for (int r = z0; r < z1; ++r) row[M.zlist[r]] = 0.0;
+        }
     }
-    Q12Levels<0, kMaxL, FEAT>::run(M, S, blk, n, lane, t, e);
+    Q12Levels<0, kMaxL, FEAT>::run(M, S, blk, n, lane, t, e, tile);
     if (!FEAT) S.Eq12[(size_t)(blk * kBlock + lane) * M.n_fn + n] = e;
+    if (FEAT && tile && w > 0) {
+        __syncthreads();
+        // segment of slot j: w doubles from column n * w of its row
+        double* base = S.D + (size_t)blk * kBlock * M.n_pr + (size_t)n * w;
+        int j = lane / w, r = lane - j * w;
+        for (int k = lane; k < kBlock * w; k += 64) {
+            base[(size_t)j * M.n_pr + r] = tile[r * kQ12Pitch + j];
+            r += 64;
+            while (r >= w) {
+                r -= w;
+                ++j;
+            }
+        }
+    }
 }
 
 // ---------------------------------------------------------------- polynomial
@@ -295,10 +327,16 @@ void check(hipError_t e, const char* what) {
 
 void launch_q12(const DevModel& m, const DevSystem& s, bool feat, hipStream_t st) {
     if (s.nblocks == 0) return;
-    if (feat)
-        hipLaunchKernelGGL(k_q12<true>, dim3(s.nblocks, m.n_fn), dim3(64), 0, st, m, s);
-    else
-        hipLaunchKernelGGL(k_q12<false>, dim3(s.nblocks, m.n_fn), dim3(64), 0, st, m, s);
+    if (feat) {
+        // staged feature writes when the widest species' tile is small
+        int wmax = 0;
+        for (int t = 0; t < m.n_type && t < 2; ++t) wmax = std::max(wmax, m.prange[t][1] - m.prange[t][0]);
+        const size_t lds = sizeof(double) * kQ12Pitch * wmax;
+        const int stage = (wmax > 0 && lds <= kQ12MaxStage) ? 1 : 0;
+        hipLaunchKernelGGL(k_q12<true>, dim3(s.nblocks, m.n_fn), dim3(64), stage ? lds : 0, st, m, s, stage);
+    } else {
+        hipLaunchKernelGGL(k_q12<false>, dim3(s.nblocks, m.n_fn), dim3(64), 0, st, m, s, 0);
+    }
     check(hipGetLastError(), "k_q12 launch");
 }
 
