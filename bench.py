@@ -1,7 +1,7 @@
 #!/usr/bin/env python3
 """Benchmark of the gtinv energy/force path on one MI355X.
 
-Workload (BASELINE metric, 256k-atom configuration): binary random
+Workload (the 256k-atom configuration of the headline metric): binary random
 substitutional fcc, 40x40x40 conventional cells = 256,000 atoms, a = 4.0 A,
 Gaussian displacements sigma = 0.08 A (seed 4), species 50/50 (seed 4);
 potential r_c = 8 A, 12 Gaussians (beta 1, centres 0..7 A), gtinv order 4
@@ -293,8 +293,8 @@ def main():
                     help="also launch a plain copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build evaluates on one GPU; --gpus %d is not supported "
-              "(the multi-GPU decomposition is described in DESIGN.md)" % args.gpus, file=sys.stderr)
+        print("bench.py: evaluation runs on one GPU; --gpus %d is not supported "
+              "(a multi-GPU run would shard the structure by spatial domain)" % args.gpus, file=sys.stderr)
         sys.exit(2)
 
     tmp = tempfile.mkdtemp(prefix="mlip_bench_")

@@ -109,8 +109,8 @@ int64_t mlip_resident_npairs(const mlip_ctx* ctx);
  * pointer may be NULL. */
 int mlip_resident_lists(mlip_ctx* ctx, int64_t* off, int* idx, double* shift, int* goff, int* gent);
 
-/* ---- Batched evaluation (BASELINE batched configuration; no reference
- * counterpart).  nstruct independent periodic structures: natoms[k] atoms
+/* ---- Batched evaluation (no reference counterpart: thousands of small
+ * structures per evaluation).  nstruct independent periodic structures: natoms[k] atoms
  * with cell cells[9k..9k+8] (rows a, b, c), positions x and 1-based types
  * concatenated in structure order.  Each structure gets its own periodic
  * neighbor list; one evaluation covers the whole batch and reduces energy
