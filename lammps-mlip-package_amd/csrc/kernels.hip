@@ -219,7 +219,29 @@ __global__ __launch_bounds__(256) void k_gather(DevSystem S) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= S.nall) return;
     double fx = 0.0, fy = 0.0, fz = 0.0;
-    for (int e = S.goff[k]; e < S.gend[k]; ++e) {
+    const int e1 = S.gend[k];
+    int e = S.goff[k];
+    // four entries per step: their pair forces are loaded together, then
+    // added in list order (the same sums as one entry at a time)
+    for (; e + 4 <= e1; e += 4) {
+        double v[4][3], sg[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int code = S.gent[e + u];
+            const int64_t p = code > 0 ? code - 1 : -code - 1;
+            sg[u] = code > 0 ? 1.0 : -1.0;
+            v[u][0] = S.gpair[3 * p];
+            v[u][1] = S.gpair[3 * p + 1];
+            v[u][2] = S.gpair[3 * p + 2];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            fx = fma(sg[u], v[u][0], fx);
+            fy = fma(sg[u], v[u][1], fy);
+            fz = fma(sg[u], v[u][2], fz);
+        }
+    }
+    for (; e < e1; ++e) {
         const int code = S.gent[e];
         const int64_t p = code > 0 ? code - 1 : -code - 1;
         const double sg = code > 0 ? 1.0 : -1.0;
