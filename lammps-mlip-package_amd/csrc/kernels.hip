@@ -221,12 +221,12 @@ __global__ __launch_bounds__(256) void k_gather(DevSystem S) {
     double fx = 0.0, fy = 0.0, fz = 0.0;
     const int e1 = S.gend[k];
     int e = S.goff[k];
-    // four entries per step: their pair forces are loaded together, then
+    // eight entries per step: their pair forces are loaded together, then
     // added in list order (the same sums as one entry at a time)
-    for (; e + 4 <= e1; e += 4) {
-        double v[4][3], sg[4];
+    for (; e + 8 <= e1; e += 8) {
+        double v[8][3], sg[8];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < 8; ++u) {
             const int code = S.gent[e + u];
             const int64_t p = code > 0 ? code - 1 : -code - 1;
             sg[u] = code > 0 ? 1.0 : -1.0;
@@ -235,7 +235,7 @@ __global__ __launch_bounds__(256) void k_gather(DevSystem S) {
             v[u][2] = S.gpair[3 * p + 2];
         }
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < 8; ++u) {
             fx = fma(sg[u], v[u][0], fx);
             fy = fma(sg[u], v[u][1], fy);
             fz = fma(sg[u], v[u][2], fz);
