@@ -339,11 +339,12 @@ void build_type_tables(Model& m, int t, const std::vector<std::map<std::vector<i
         for (const Pending& q : pend)
             if (!(q.v.flags & 4)) dyn = true;
         // Work model of a visit: its terms plus the per-m_a (per (m_a, m_b))
-        // region and per-visit overheads; of a part: plus loading and storing
-        // its G row.
+        // region and per-visit overheads, in units of terms (3 and 24 were the
+        // best of a sweep at the 256k-atom benchmark model: 40.3 -> 38.8 ms);
+        // of a part: plus loading and storing its G row.
         auto vcost = [](const Pending& q) {
             const long regions = (2L * q.v.la + 1) * (q.v.order == 4 ? 2L * q.v.lb + 1 : 1);
-            return q.cost + 2 * regions + 8;
+            return q.cost + 3 * regions + 24;
         };
         struct Row {
             size_t b, e;
