@@ -49,17 +49,29 @@ def make_workload(ncell, tmpdir):
     return pot, s
 
 
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(pot, budget_s):
-    """Oracle on a 4x4x4-cell (256-atom) sub-cell of the same generator, on
-    all host threads and on one thread (the reference's CPU style scales
-    poorly with OpenMP)."""
+    """Oracle (FP64 restatement of the reference CPU pair style) on sub-cells
+    of the same generator: a 10x10x10-cell (4,000-atom) sub-cell on all host
+    threads, and a 4x4x4-cell (256-atom) sub-cell on one thread (the
+    reference's CPU style scales poorly with OpenMP).  The per-atom cost does
+    not depend on the cell size (a fixed neighbor count per atom)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib
     o = oracle_lib.Oracle(pot)
-    s = W.fcc(4, 4.0, 0.08, 4, type_fraction=0.5, type_seed=4)
-    nl = W.build_neighbor_list(s, o.cutoff, full=False)
 
-    def timed(budget):
+    def timed(ncell, budget):
+        s = W.fcc(ncell, 4.0, 0.08, 4, type_fraction=0.5, type_seed=4)
+        nl = W.build_neighbor_list(s, o.cutoff, full=False)
         n_eval, t_tot = 0, 0.0
         while t_tot < budget or n_eval == 0:
             t0 = time.perf_counter()
@@ -68,20 +80,22 @@ def cpu_baseline(pot, budget_s):
             n_eval += 1
             if n_eval >= 50:
                 break
-        return n_eval, t_tot
+        return s.natoms, n_eval, t_tot
 
     L = oracle_lib.lib()
     threads = L.oracle_num_threads()
-    n_eval, t_tot = timed(budget_s)
+    na, n_eval, t_tot = timed(10, budget_s)
     L.oracle_set_num_threads(1)
-    n1, t1 = timed(budget_s / 2)
+    n1a, n1, t1 = timed(4, budget_s / 2)
     L.oracle_set_num_threads(threads)
-    return {"value": s.natoms * n_eval / t_tot, "unit": "atom-evals/s", "cores": int(threads),
-            "kind": "port",
-            "sample": "%d evaluation(s) of a 256-atom (4x4x4-cell) sub-cell of the same generator and model, "
-                      "%.1f s of CPU work; per-atom cost is size-independent" % (n_eval, t_tot),
-            "single_thread": {"value": s.natoms * n1 / t1, "cores": 1,
-                              "sample": "%d evaluation(s), %.1f s" % (n1, t1)}}
+    return {"value": na * n_eval / t_tot, "unit": "atom-evals/s", "cores": int(threads),
+            "kind": "port", "cpu_model": cpu_model(), "host_cpus": os.cpu_count(),
+            "sample": "%d evaluation(s) of a %d-atom (10x10x10-cell) sub-cell of the same generator and model, "
+                      "%.1f s of CPU work on %d threads; the per-atom cost is size-independent"
+                      % (n_eval, na, t_tot, threads),
+            "single_thread": {"value": n1a * n1 / t1, "cores": 1,
+                              "sample": "%d evaluation(s) of a %d-atom (4x4x4-cell) sub-cell, %.1f s"
+                                        % (n1, n1a, t1)}}
 
 
 def nve_drift(pot, precision, nsteps=200, ncell=10):
@@ -155,6 +169,19 @@ def measured_traffic(kernel):
     return {"bytes_per_launch": k["bytes"], "source": "profiles/hbm_traffic_256k_fp64.json"}
 
 
+def measured_flops(kernel):
+    """Executed FP64 FLOPs per launch of `kernel` from the SQ_INSTS_VALU_
+    {FMA,MUL,ADD,TRANS}_F64 counter pass summarised in profiles/ (64 lanes per
+    instruction, 2 per FMA), or None if absent."""
+    path = os.path.join(ROOT, "profiles", "valu_flops_256k_fp64.json")
+    if not os.path.exists(path):
+        return None
+    k = json.load(open(path)).get("kernels", {}).get(kernel)
+    if not k:
+        return None
+    return {"flops_per_launch": k["flops"], "source": "profiles/valu_flops_256k_fp64.json"}
+
+
 def make_pair(pot, precision):
     p = PairMLIPGtinv()
     p.settings()
@@ -163,39 +190,140 @@ def make_pair(pot, precision):
     return p
 
 
-def batched_section(pot, steps, precision="mixed"):
+def batched_section(pot, steps):
     """4096 independent structures (log-uniform 8-512 atoms, seed 6) in one
     batched evaluation, replayed as a captured HIP graph; force launch shape
-    from the autotuner."""
-    p = make_pair(pot, precision)
+    from the autotuner.  Mixed precision, and the same workload in FP64 as
+    its baseline; per-structure error of the mixed result against FP64."""
     structs = W.random_structures(4096, 8, 512, seed=6)
     natoms = sum(s.natoms for s in structs)
-    t0 = time.time()
-    b = BatchSystem(p, structs)
-    t_setup = time.time() - t0
-    b.compute()
-    b.sync()
-    nt = b.autotune(reps=2)
-    b.graph(True)
-    for _ in range(2):  # eager after the change, then the capture
+    nat = np.array([s.natoms for s in structs])
+    runs = {}
+    for precision in ("mixed", "fp64"):
+        p = make_pair(pot, precision)
+        t0 = time.time()
+        b = BatchSystem(p, structs)
+        t_setup = time.time() - t0
         b.compute()
-    b.sync()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        b.compute()
-    b.sync()
-    dt = time.perf_counter() - t0
-    r = b.get(forces=False)
-    if not np.all(np.isfinite(r["energies"])):
-        raise SystemExit("non-finite batched result")
-    out = {"value": natoms * steps / dt, "unit": "atom-evals/s", "ms_per_eval": 1e3 * dt / steps,
-           "nstruct": len(structs), "natoms": natoms, "precision": precision, "graph": True,
-           "force_threads": nt, "host_setup_s": t_setup,
+        b.sync()
+        nt = b.autotune(reps=2)
+        b.graph(True)
+        for _ in range(2):  # eager after the change, then the capture
+            b.compute()
+        b.sync()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            b.compute()
+        b.sync()
+        dt = time.perf_counter() - t0
+        r = b.get(forces=True)
+        if not np.all(np.isfinite(r["energies"])):
+            raise SystemExit("non-finite batched result")
+        runs[precision] = {"dt": dt, "nt": nt, "setup": t_setup, "res": r}
+        log("batched %s: %d structures, %d atoms, %.2f ms/eval, %.4g atom-evals/s (setup %.1fs)"
+            % (precision, len(structs), natoms, 1e3 * dt / steps, natoms * steps / dt, t_setup))
+        del b, p
+    rm, r64 = runs["mixed"]["res"], runs["fp64"]["res"]
+    de = np.abs(rm["energies"] - r64["energies"]) / nat
+    df = np.array([np.abs(a - c).max() if a.size else 0.0 for a, c in zip(rm["forces"], r64["forces"])])
+    dv = np.abs(rm["virials"] - r64["virials"]).max(axis=1)
+    dm = runs["mixed"]["dt"]
+    out = {"value": natoms * steps / dm, "unit": "atom-evals/s", "ms_per_eval": 1e3 * dm / steps,
+           "nstruct": len(structs), "natoms": natoms, "precision": "mixed", "graph": True,
+           "force_threads": runs["mixed"]["nt"], "host_setup_s": runs["mixed"]["setup"],
+           "fp64": {"value": natoms * steps / runs["fp64"]["dt"], "ms_per_eval": 1e3 * runs["fp64"]["dt"] / steps,
+                    "force_threads": runs["fp64"]["nt"]},
+           "speedup_mixed_vs_fp64": runs["fp64"]["dt"] / dm,
+           "mixed_vs_fp64_error": {
+               "energy_per_atom_max": float(de.max()), "energy_per_atom_median": float(np.median(de)),
+               "force_max": float(df.max()), "force_median_of_structure_max": float(np.median(df)),
+               "virial_max": float(dv.max()), "structures": len(structs), "unit": "eV/atom, eV/A, eV",
+               "worst_structure": int(np.argmax(df))},
            "workload": "4096 random triclinic cells, 8-512 atoms log-uniform (seed 6), 0.06 atoms/A^3, "
                        "1.8 A rejection, two species with random fraction; same model"}
-    log("batched %s: %d structures, %d atoms, %.2f ms/eval, %.4g atom-evals/s (setup %.1fs)"
-        % (precision, len(structs), natoms, out["ms_per_eval"], out["value"], t_setup))
-    del b
+    log("batched mixed vs FP64: %.2fx; per structure max dE/atom %.2e, max dF %.2e eV/A"
+        % (out["speedup_mixed_vs_fp64"], de.max(), df.max()))
+    return out
+
+
+def trajectory_section(pot, s, nsteps=200, dt=0.001, skin=1.0, temperature=300.0):
+    """The 256k-atom structure run 200 NVE steps (velocity Verlet, 1 fs,
+    Maxwell start at 300 K, seed 3) twice: driven by FP64 forces, with the
+    mixed-precision evaluation of every step's positions compared against
+    FP64 (energy per atom, force MAE and max), and driven by mixed-precision
+    forces; total-energy drift per atom of both.  Positions stay resident
+    (the list is built once at r_c + skin and rebuilt if an atom moves more
+    than skin / 2)."""
+    p = make_pair(pot, "fp64")
+    mass_t = p.masses()
+    m = mass_t[s.types - 1][:, None]
+    ftm2v = 1.0 / 1.0364269e-4
+    v0 = SV.maxwell(s, mass_t, temperature, np.random.default_rng(3))
+
+    def build(x):
+        off, idx, sh = periodic_full_list(s.cell, x, p.init_one() + skin)
+        return ResidentSystem(p, x, s.types, off, idx, sh)
+
+    def run(drive, compare):
+        x = s.positions.copy()
+        v = v0.copy()
+        rs = build(x)
+        xref = x.copy()
+        rebuilds = 0
+
+        def evaluate(prec):
+            p.set_precision(prec)
+            rs.update_positions(x)
+            rs.compute()
+            rs.sync()
+            return rs.get()
+
+        r = evaluate(drive)
+        rows, etot = [], []
+        for step in range(nsteps + 1):
+            if compare:
+                rc = evaluate("mixed")
+                df = np.abs(rc["forces"] - r["forces"])
+                rows.append((abs(rc["energy"] - r["energy"]) / s.natoms, float(df.mean()), float(df.max())))
+            ke = 0.5 * float((m * v * v).sum()) / ftm2v
+            etot.append((r["energy"] + ke) / s.natoms)
+            if step == nsteps:
+                break
+            v += 0.5 * dt * ftm2v * r["forces"] / m
+            x += dt * v
+            if np.sqrt(((x - xref) ** 2).sum(axis=1).max()) > 0.5 * skin:
+                rs = build(x)
+                xref = x.copy()
+                rebuilds += 1
+            r = evaluate(drive)
+            v += 0.5 * dt * ftm2v * r["forces"] / m
+        p.set_precision("fp64")
+        return np.array(rows), np.array(etot), rebuilds
+
+    t0 = time.perf_counter()
+    rows, e64, rb64 = run("fp64", True)
+    t1 = time.perf_counter()
+    _, emx, rbmx = run("mixed", False)
+    t2 = time.perf_counter()
+    out = {"workload": "the 256,000-atom structure, velocity Verlet %d steps at %.0f fs from a %.0f K Maxwell draw "
+                       "(seed 3), neighbor list at r_c + %.1f A" % (nsteps, dt * 1e3, temperature, skin),
+           "mixed_vs_fp64_per_step": {"energy_per_atom_max": float(rows[:, 0].max()),
+                                      "force_mae_max": float(rows[:, 1].max()),
+                                      "force_max": float(rows[:, 2].max()),
+                                      "energy_per_atom": [float("%.3g" % v) for v in rows[:, 0]],
+                                      "force_mae": [float("%.3g" % v) for v in rows[:, 1]],
+                                      "force_max_by_step": [float("%.3g" % v) for v in rows[:, 2]],
+                                      "unit": "eV/atom, eV/A"},
+           "fp64_totE_drift_per_atom": float(e64[-1] - e64[0]),
+           "fp64_totE_maxdev_per_atom": float(np.abs(e64 - e64[0]).max()),
+           "mixed_totE_drift_per_atom": float(emx[-1] - emx[0]),
+           "mixed_totE_maxdev_per_atom": float(np.abs(emx - emx[0]).max()),
+           "list_rebuilds": {"fp64": rb64, "mixed": rbmx}, "run_s": {"fp64_with_mixed_checks": t1 - t0,
+                                                                  "mixed": t2 - t1}}
+    log("256k trajectory: mixed vs FP64 over %d steps: dE/atom max %.2e, F MAE max %.2e, F max %.2e eV/A; "
+        "drift FP64 %.2e, mixed %.2e eV/atom" % (nsteps, rows[:, 0].max(), rows[:, 1].max(), rows[:, 2].max(),
+                                                  out["fp64_totE_drift_per_atom"], out["mixed_totE_drift_per_atom"]))
+    del p
     return out
 
 
@@ -261,9 +389,11 @@ def served_section(pot, rates, windows, budget_gb):
     for rate, win in zip(rates, windows):
         stream = SV.make_stream(rate, win, 7, masses)
         r = SV.serve(pool, stream)
-        seq = SV.serve_sequential(seq_pool, SV.make_stream(rate, win, 7, masses))
+        seq_stream = SV.make_stream(rate, win, 7, masses)
+        seq = SV.serve_sequential(seq_pool, seq_stream)
         r["arrival_window_s"] = win
         r["fp64_sequential"] = seq
+        r["mixed_vs_fp64_error"] = SV.stream_errors(stream, seq_stream)
         out["rates"][str(rate)] = r
         log("served %d req/s (%d requests): p50 %.1f ms p99 %.1f ms, %.4g atom-evals/s; "
             "FP64 one-at-a-time: p50 %.1f ms p99 %.1f ms, %.4g atom-evals/s"
@@ -285,6 +415,8 @@ def main():
     ap.add_argument("--no-batched", action="store_true", help="skip the 4096-structure batched evaluation")
     ap.add_argument("--no-served", action="store_true", help="skip the served request streams")
     ap.add_argument("--no-train", action="store_true", help="skip the training-feature batch")
+    ap.add_argument("--no-trajectory", action="store_true", help="skip the 256k-atom NVE error trajectory")
+    ap.add_argument("--trajectory-steps", type=int, default=200)
     ap.add_argument("--train-structures", type=int, default=256)
     ap.add_argument("--serve-windows", type=str, default="1.0,0.5,0.25",
                     help="arrival windows (s) for the 200/1000/5000 req/s streams")
@@ -352,6 +484,8 @@ def main():
 
     npairs = rs.npairs
     del rs
+    if mixed is not None and not args.no_trajectory:
+        mixed["trajectory_256k"] = trajectory_section(pot, s, nsteps=args.trajectory_steps)
     batched = None if args.no_batched else batched_section(pot, max(args.steps, 3))
     train = None if args.no_train else train_section(args.train_structures)
     served = None
@@ -368,25 +502,31 @@ def main():
     dom_ms = kernels[dom]["avg_ms"]
     ref_fl = flops_ref[fl_key] if fl_key else None
     exe_fl = flops[fl_key] if fl_key else None
-    achieved = ref_fl / (dom_ms * 1e-3) / 1e12 if fl_key else None
+    achieved = exe_fl / (dom_ms * 1e-3) / 1e12 if fl_key else None
     traffic = measured_traffic(dom)
-    roofline = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+    counted = measured_flops(dom)
+    total_rate = flops["total"] * args.steps / dt / 1e12
+    roofline = {"bound": "fp64_valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": (achieved / FP64_PEAK_TFLOPS) if achieved else None,
                 "traffic": traffic["bytes_per_launch"] if traffic else None,
                 "kernel": dom, "kernel_avg_ms": dom_ms,
-                "algorithmic_flops_per_launch": ref_fl,
-                "rule": "reference formulation: per half pair 40 + 12 n_fn + 38 n_lmh + 52 n_fn n_lmh; per atom "
-                        "n_fn sum T(6(q-1)+3) invariants + n_fn sum T q(6(q-1)+4) adjoint + 2 sum|p| polynomial",
                 "executed_flops_per_launch": exe_fl,
-                "executed_tflops": exe_fl / (dom_ms * 1e-3) / 1e12 if fl_key else None,
+                "counter_flops_per_launch": counted["flops_per_launch"] if counted else None,
+                "counter_vs_model": (counted["flops_per_launch"] / exe_fl) if (counted and exe_fl) else None,
+                "flops_source": counted["source"] if counted else None,
+                "reference_rule_flops_per_launch": ref_fl,
+                "reference_rule_tflops": ref_fl / (dom_ms * 1e-3) / 1e12 if fl_key else None,
+                "step_executed_tflops": total_rate, "step_frac": total_rate / FP64_PEAK_TFLOPS,
                 "traffic_source": traffic["source"] if traffic else None,
-                "note": "FP64 VALU work (the MI355X FP64 peak is the same for vector and matrix units); the "
-                        "kernel pairs m with -m, shares partial products, merges factor positions with identical "
-                        "signatures and takes the energy of linear families from the adjoint by Euler's theorem, "
-                        "so it executes %.1fx fewer operations than the reference formulation counts: frac above "
-                        "1 by the rule, executed_tflops / peak for how busy the FP64 pipes are"
-                        % ((ref_fl / exe_fl) if (fl_key and exe_fl) else float("nan"))}
-    total_rate = flops["total"] * args.steps / dt / 1e12
+                "rule": "achieved = FP64 operations the kernel executes (engine count mlip_flops: add/mul 1, "
+                        "FMA 2; checked against the SQ_INSTS_VALU_*_F64 counters of the same launch) / its "
+                        "average HIP-event duration; the reference-formulation count (per half pair 40 + "
+                        "12 n_fn + 38 n_lmh + 52 n_fn n_lmh; per atom n_fn sum T(6(q-1)+3) + n_fn sum "
+                        "T q(6(q-1)+4) + 2 sum|p|) is %.1fx larger because the kernel pairs m with -m, shares "
+                        "partial products, merges factor positions with identical signatures and takes the "
+                        "energy of linear families from the adjoint by Euler's theorem"
+                        % ((ref_fl / exe_fl) if (fl_key and exe_fl) else float("nan")),
+                "note": "FP64 VALU kernel (the MI355X FP64 peak is 78.6 TFLOP/s for vector and matrix units)"}
     log("steps %d in %.3fs -> %.1f ms/step, %.4g atom-evals/s; whole-step FP64 rate %.2f TFLOP/s"
         % (args.steps, dt, 1e3 * dt / args.steps, value, total_rate))
     for k, v in sorted(kernels.items(), key=lambda kv: -kv[1]["avg_ms"]):

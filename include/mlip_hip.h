@@ -25,7 +25,7 @@ enum {
     MLIP_ERR_ARG = 1,     /* bad argument (e.g. unknown element, newton off, n_type > 2) */
     MLIP_ERR_FILE = 2,    /* potential file missing / malformed */
     MLIP_ERR_MODEL = 3,   /* model outside what the HIP path evaluates */
-    MLIP_ERR_DEVICE = 4,  /* HIP or rocBLAS failure */
+    MLIP_ERR_DEVICE = 4,  /* HIP failure (no device, allocation, launch, execution) */
     MLIP_ERR_STATE = 5,   /* call order (e.g. resident compute before resident set) */
     MLIP_ERR_FULL = 6     /* request pool: no room for the request now (retry after completions) */
 };
@@ -58,15 +58,26 @@ int mlip_set_precision(mlip_ctx* ctx, int mode);
 
 const char* mlip_last_error(const mlip_ctx* ctx);
 
+/* Per-atom energy and virial of mlip_compute / mlip_compute_flat:
+ * MLIP_ATOM_PAIR_SPLIT (default) as the reference CPU style fills eatom and
+ * vatom through ev_tally_xyz (pair_mlip_gtinv.cpp:153,240-253;
+ * pair_mlip_pair.cpp:226): each pair's energy (every term of dE/da weighted
+ * by 1 / its degree) and its virial del (x) f go half to each of its two
+ * positions, ghosts included (their sums are LAMMPS reverse communication's);
+ * MLIP_ATOM_SITE: site energy E_i of each centre and the virial of the pairs
+ * it owns (no ghost entries).  Both sum to eng_vdwl and virial6. */
+enum { MLIP_ATOM_PAIR_SPLIT = 0, MLIP_ATOM_SITE = 1 };
+int mlip_set_atom_mode(mlip_ctx* ctx, int mode);
+
 /* Replaces PairMLIPGtinv::compute(eflag, vflag) (pair_mlip_gtinv.cpp:63-125).
  * Reads x[nall][3] (nall = nlocal + nghost), 1-based type[nall], tag[nall]
  * and the LAMMPS neighbor list (inum, ilist, numneigh, firstneigh).
  * neigh_full = 0: half list with newton on, as the CPU style requests;
  * neigh_full = 1: full list, as the Kokkos style requests.
  * Accumulates into f[nall][3] (ghost entries are left for LAMMPS reverse
- * communication); writes *eng_vdwl and virial6 (xx, yy, zz, xy, xz, yz).
- * eatom (nullable) receives per-centre site energies; vatom6 (nullable)
- * the per-centre virial of the pairs it owns. */
+ * communication); adds to *eng_vdwl and virial6 (xx, yy, zz, xy, xz, yz).
+ * eatom[nall] and vatom6[nall][6] (nullable) are accumulated as
+ * mlip_set_atom_mode selects. */
 int mlip_compute(mlip_ctx* ctx, int nlocal, int nghost, const double* x, const int* type,
                  const int64_t* tag, int inum, const int* ilist, const int* numneigh,
                  const int* const* firstneigh, int neigh_full, int eflag, int vflag, double* f,
@@ -186,7 +197,11 @@ int mlip_calibration_copy(mlip_ctx* ctx, int64_t nbytes, int reps);
 /* Diagnostics for parity tests: copy device intermediates of the last
  * evaluation to host.  what = 0: a_nlm [slot order], 1: polynomial features
  * (per slot a compact row of the features its species sees, in the engine's
- * column order), 2: per-centre energies; returns the number of doubles written. */
+ * column order), 2: per-centre energies, 3: the adjoint G (layout of a_nlm),
+ * 4: the quadratic-form products Hq (layout of 1), 5: the per-species
+ * quadratic forms, 6: seven numbers (row pitch; per species the offset and
+ * dimension of its form and its slot blocks); returns the number of doubles
+ * written. */
 int64_t mlip_debug_fetch(mlip_ctx* ctx, int what, double* out, int64_t cap);
 
 /* Model description: n_type, n_fn, n_gtinv, n_des, n_coeff, maxl, lmax3,
@@ -257,6 +272,13 @@ int mlip_pool_result(mlip_ctx* ctx, int handle, double* energy, double* virial, 
                      double* thermo, int thermo_cap);
 /* used atoms, used pairs, atom capacity, pair capacity, requests held, requests running */
 int mlip_pool_stats(const mlip_ctx* ctx, int64_t* out6);
+/* Neighbor-list rebuilds of NVE requests so far (-1 without a pool).  An NVE
+ * request needs skin > 0 (MLIP_ERR_ARG otherwise); its list is rebuilt from
+ * the current positions whenever an atom would otherwise move more than
+ * skin / 2 from where the list was built.  A request whose rebuilt list no
+ * longer fits the pair capacity ends with an error (mlip_pool_result
+ * returns -MLIP_ERR_FULL with the message in mlip_last_error). */
+int64_t mlip_pool_rebuilds(const mlip_ctx* ctx);
 
 #ifdef __cplusplus
 }

@@ -55,6 +55,14 @@ struct DevModel {
     const int32_t* c3_off;  // cubic terms transposed by target: offsets ([species][n_pr + 1])
     const int32_t* c3_bc;   // 2 per entry (columns of the same species)
     const double* c3_w;
+    // per-atom energy split of the reference CPU style (each polynomial term
+    // of dE/da weighted by 1 / its degree in a): linear weights divided by
+    // the family's order (n_des), the degree (family order) of each compact
+    // polynomial column ([species][n_pr]) and the per-species quadratic forms
+    const double* weight_e;
+    const int32_t* pdeg;
+    const double* wmat;
+    int32_t wofs[2], wdim[2];
     // radial-only style
     int pair_model;         // 1: features sum_j g_n(r) per type combination
     int pair_tc[2][2];      // [centre type][neighbor type] -> type combination
@@ -65,6 +73,13 @@ struct DevModel {
     int q34_dyn_adj, q34_dyn_feat;  // visits for the general-loop order-3/4 kernel
     const YTab<double>* yt64;
     const YTab<float>* yt32;
+    // orders 5/6: visits of every species (adjoint, then features), their
+    // term codes and coefficients; tile of a_nlm up to l = lmaxh
+    const VisitHi* hvis;
+    const int32_t* hterm;
+    const double* hcoef;
+    int32_t hadj_beg[2], hadj_end[2], hfeat_beg[2], hfeat_end[2];
+    int lmaxh, n_lmhh;
 };
 
 struct DevSystem {
@@ -105,27 +120,58 @@ struct DevSystem {
 // Host-side construction of the recursion tables (uploaded per context).
 void build_ylm_tables(YTab<double>& t64, YTab<float>& t32);
 
-void launch_anlm(const DevModel& m, const DevSystem& s, int precision, hipStream_t st);
-void launch_q12(const DevModel& m, const DevSystem& s, bool feat, hipStream_t st);
-void launch_q34_adj(const DevModel& m, const DevSystem& s, hipStream_t st);
-void launch_q34_feat(const DevModel& m, const DevSystem& s, hipStream_t st);
+// Launch wrappers return the first HIP error of their launches (a refused
+// launch configuration, an LDS request beyond the device's limit, an invalid
+// kernel argument), hipSuccess otherwise; the engine turns any error into a
+// failed call.  They enqueue work only: execution errors surface at the next
+// synchronisation.
+inline hipError_t first_error(hipError_t a, hipError_t b) { return a != hipSuccess ? a : b; }
+// Largest dynamic + static LDS one workgroup may use on the current device (bytes).
+size_t lds_limit();
+// hipGetLastError after a launch, with a dynamic LDS request checked first.
+hipError_t launched(size_t lds_bytes = 0);
+
+hipError_t launch_anlm(const DevModel& m, const DevSystem& s, int precision, hipStream_t st);
+hipError_t launch_q12(const DevModel& m, const DevSystem& s, bool feat, hipStream_t st);
+hipError_t launch_q34_adj(const DevModel& m, const DevSystem& s, hipStream_t st);
+hipError_t launch_q34_feat(const DevModel& m, const DevSystem& s, hipStream_t st);
 // general-loop passes for visits outside the unrolled signatures (launched
 // by the two functions above when the model has such visits)
-void launch_q34_dyn(const DevModel& m, const DevSystem& s, bool feat, hipStream_t st);
-void launch_poly(const DevModel& m, const DevSystem& s, hipStream_t st);
-// threads per workgroup of the force kernel: 64, 128 (default), 192 or 256
-void launch_force(const DevModel& m, const DevSystem& s, int precision, int threads, hipStream_t st);
+hipError_t launch_q34_dyn(const DevModel& m, const DevSystem& s, bool feat, hipStream_t st);
+hipError_t launch_poly(const DevModel& m, const DevSystem& s, hipStream_t st);
+// orders 5/6: polynomial features (feat) or their share of the adjoint G and
+// of the energy of linear families (qhigh.hip); no-op for models without them
+hipError_t launch_qhigh(const DevModel& m, const DevSystem& s, bool feat, hipStream_t st);
+// Hq = D W per species on the matrix cores (slot blocks grouped by species,
+// nblk_type[t] blocks of species t in order)
+hipError_t launch_poly_gemm(const DevModel& m, const DevSystem& s, const int* nblk_type, hipStream_t st);
+// threads per workgroup of the force kernel: 64, 128, 192 (default) or 256
+hipError_t launch_force(const DevModel& m, const DevSystem& s, int precision, int threads, hipStream_t st);
 // per-structure energy + virial (7 doubles each) of a batch; soff = atom offsets
-void launch_struct_reduce(const DevSystem& s, const int32_t* sbeg, const int32_t* send, int nstruct, double* out,
-                          hipStream_t st);
-void launch_gather(const DevModel& m, const DevSystem& s, hipStream_t st);
+hipError_t launch_struct_reduce(const DevSystem& s, const int32_t* sbeg, const int32_t* send, int nstruct,
+                                double* out, hipStream_t st);
+hipError_t launch_gather(const DevModel& m, const DevSystem& s, hipStream_t st);
 // radial-only style: features into D, energy and H = dE/dd, pair forces
-void launch_pair_features(const DevModel& m, const DevSystem& s, hipStream_t st);
-void launch_pair_poly(const DevModel& m, const DevSystem& s, hipStream_t st);
-void launch_pair_force(const DevModel& m, const DevSystem& s, hipStream_t st);
+hipError_t launch_pair_features(const DevModel& m, const DevSystem& s, hipStream_t st);
+hipError_t launch_pair_poly(const DevModel& m, const DevSystem& s, hipStream_t st);
+hipError_t launch_pair_force(const DevModel& m, const DevSystem& s, hipStream_t st);
 // per-slot energies; with totals, also the fixed-order whole-system sums
-void launch_energy_virial(const DevModel& m, const DevSystem& s, bool totals, hipStream_t st);
+hipError_t launch_energy_virial(const DevModel& m, const DevSystem& s, bool totals, hipStream_t st);
+// Per-atom energy and virial in the reference CPU style's split
+// (pair_mlip_gtinv.cpp:240-253 with ev_tally_xyz; pair_mlip_pair.cpp:166-230):
+// launch_poly_split writes He = dE_i/dd with every polynomial term weighted
+// by 1 / its degree in a (into s.Hq); the adjoint kernels run on that He and
+// weight_e give the split adjoint in s.G; launch_pair_energy writes, per
+// directed pair p, 7 doubles: its energy share sum w_m Re(G phi_p) and
+// del_p (x) (-g_p) from s.gpair (xx, yy, zz, xy, xz, yz); launch_atom_ev
+// sums half of them over each position's gather list (evat[nall][7]).
+hipError_t launch_poly_split(const DevModel& m, const DevSystem& s, hipStream_t st);
+hipError_t launch_pair_energy(const DevModel& m, const DevSystem& s, double* epv, hipStream_t st);
+hipError_t launch_pair_model_energy(const DevModel& m, const DevSystem& s, double* epv, hipStream_t st);
+hipError_t launch_atom_ev(const DevSystem& s, const double* epv, double* evat, hipStream_t st);
+// n doubles of dst set to zero by a kernel (no memset node in a captured graph)
+hipError_t launch_fill(double* dst, size_t n, hipStream_t st);
 // plain streaming copy used to calibrate hardware byte counters
-void launch_copy(void* dst, const void* src, int64_t nbytes, int reps, hipStream_t st);
+hipError_t launch_copy(void* dst, const void* src, int64_t nbytes, int reps, hipStream_t st);
 
 }  // namespace mlip

@@ -2,7 +2,6 @@
 // on the device, converts LAMMPS neighbor data into the device pair layout,
 // runs the kernel pipeline on one HIP stream and returns E, F, virial.
 #include <hip/hip_runtime.h>
-#include <rocblas/rocblas.h>
 
 #include <algorithm>
 #include <atomic>
@@ -126,6 +125,8 @@ bool upload_model(mlip_ctx* c) {
             if (v.pslot >= 0) written[v.pslot] = 1;
         for (const VisitQ34& v : m.types[t].visits_feat)
             if (v.pslot >= 0) written[v.pslot] = 1;
+        for (const VisitHi& v : m.types[t].hi_feat)
+            if (v.pslot >= 0) written[v.pslot] = 1;
         dm.zoff[t] = static_cast<int32_t>(zlist.size());
         for (int r = 0; r < m.n_pitc; ++r) {
             const bool in = r >= m.prange[t][0] && r < m.prange[t][1];
@@ -142,6 +143,10 @@ bool upload_model(mlip_ctx* c) {
     // species), linear weights, cubic terms transposed by target column
     // (terms with a factor the species does not see vanish and are dropped)
     std::vector<double> wc, linc(static_cast<size_t>(2) * n_pr, 0.0), c3w;
+    std::vector<int32_t> pdeg(static_cast<size_t>(2) * n_pr, 1);
+    std::vector<int> order_of_rank(static_cast<size_t>(std::max(m.n_pitc, 1)), 1);
+    for (int i = 0; i < m.n_gtinv && !m.pair_model; ++i)
+        if (m.prank[i] >= 0) order_of_rank[m.prank[i]] = m.itcs[i].order;
     std::vector<int32_t> c3off(static_cast<size_t>(2) * (n_pr + 1), 0), c3bc;
     for (int t = 0; t < 2; ++t) {
         const bool on = t < m.n_type && m.n_p > 0;
@@ -157,7 +162,10 @@ bool upload_model(mlip_ctx* c) {
         for (int ci = 0; ci < wf; ++ci)
             for (int cj = 0; cj < wf; ++cj)
                 wc.push_back(m.wmat[static_cast<size_t>(gof[ci]) * m.n_p + gof[cj]]);
-        for (int ci = 0; ci < wf; ++ci) linc[static_cast<size_t>(t) * n_pr + ci] = m.lin_p[gof[ci]];
+        for (int ci = 0; ci < wf; ++ci) {
+            linc[static_cast<size_t>(t) * n_pr + ci] = m.lin_p[gof[ci]];
+            pdeg[static_cast<size_t>(t) * n_pr + ci] = order_of_rank[gof[ci] % m.n_pitc];
+        }
         std::vector<std::vector<std::pair<std::array<int, 2>, double>>> c3(static_cast<size_t>(wf));
         for (size_t k = 0; on && k < m.c3_w.size(); ++k) {
             const int a = cof[m.c3_idx[3 * k]], b = cof[m.c3_idx[3 * k + 1]], cc = cof[m.c3_idx[3 * k + 2]];
@@ -178,7 +186,14 @@ bool upload_model(mlip_ctx* c) {
                 }
         }
     }
+    // linear weights of the per-atom energy split: w / order of the family
+    std::vector<double> wsplit(m.weight.begin(), m.weight.end());
+    if (!m.pair_model)
+        for (int f = 0; f < m.n_des && f < static_cast<int>(wsplit.size()); ++f)
+            wsplit[f] /= m.itcs[f % m.n_gtinv].order;
     bool ok = upload(c, c->d_coef, m.coef.data(), m.coef.size(), "coefficients") &&
+              upload(c, c->d_weight_e, wsplit.data(), wsplit.size(), "split weights") &&
+              upload(c, c->d_pdeg, pdeg.data(), pdeg.size(), "feature degrees") &&
               upload(c, c->d_weight, m.weight.data(), m.weight.size(), "weights") &&
               upload(c, c->d_wmat, wc.data(), wc.size(), "quadratic form") &&
               upload(c, c->d_linp, linc.data(), linc.size(), "linear poly weights") &&
@@ -203,6 +218,29 @@ bool upload_model(mlip_ctx* c) {
         dm.ph_off = c->d_phoff.p;
         dm.ph_ent = c->d_phent.p;
     }
+    if (m.has_hi) {
+        std::vector<VisitHi> hv;
+        for (int t = 0; t < m.n_type && t < 2; ++t) {
+            dm.hadj_beg[t] = static_cast<int32_t>(hv.size());
+            hv.insert(hv.end(), m.types[t].hi_adj.begin(), m.types[t].hi_adj.end());
+            dm.hadj_end[t] = static_cast<int32_t>(hv.size());
+            dm.hfeat_beg[t] = static_cast<int32_t>(hv.size());
+            hv.insert(hv.end(), m.types[t].hi_feat.begin(), m.types[t].hi_feat.end());
+            dm.hfeat_end[t] = static_cast<int32_t>(hv.size());
+        }
+        if (!upload(c, c->d_hvis, hv.data(), hv.size(), "order-5/6 visits") ||
+            !upload(c, c->d_hterm, m.hi_term.data(), m.hi_term.size(), "order-5/6 terms") ||
+            !upload(c, c->d_hcoef, m.hi_coef.data(), m.hi_coef.size(), "order-5/6 coefficients"))
+            return false;
+        dm.hvis = c->d_hvis.p;
+        dm.hterm = c->d_hterm.p;
+        dm.hcoef = c->d_hcoef.p;
+        dm.lmaxh = m.lmaxh;
+        dm.n_lmhh = n_lmh_of(m.lmaxh);
+    } else {
+        dm.lmaxh = -1;
+        dm.n_lmhh = 0;
+    }
     {
         YTab<double> t64;
         YTab<float> t32;
@@ -225,8 +263,19 @@ bool upload_model(mlip_ctx* c) {
     dm.c3_off = c->d_c3off.p;
     dm.c3_bc = c->d_c3bc.p;
     dm.c3_w = c->d_c3w.p;
+    dm.weight_e = c->d_weight_e.p;
+    dm.pdeg = c->d_pdeg.p;
+    dm.wmat = c->d_wmat.p;
+    for (int t = 0; t < 2; ++t) {
+        dm.wofs[t] = static_cast<int32_t>(c->wofs[t]);
+        dm.wdim[t] = c->wdim[t];
+    }
     return hip_ok(c, hipStreamSynchronize(c->stream), "model upload");
 }
+
+}  // namespace
+
+namespace mlip {
 
 // Slot assignment: centres sorted by element (stable), each element padded
 // to whole blocks of kBlock lanes.
@@ -253,76 +302,55 @@ void assign_slots(const mlip_ctx* c, int nown, const std::vector<int>& elem_of_o
     }
 }
 
+}  // namespace mlip
+
+namespace {
+
 // Slot layout, work buffers and the DevSystem view of a structure whose
 // positions, elements, pair lists and gather lists are already on the device.
 bool setup_layout(mlip_ctx* c, int nall, int nown, int64_t npairs, const std::vector<int32_t>& elem_own, bool has_pcen,
                   bool has_shift) {
     const Model& m = c->model;
     drop_graph(c);
-    c->nstruct = 0;
-    c->sres = nullptr;
-    c->total_reduce = true;
     std::vector<int32_t> atom_of_slot, slot_of_atom, block_type;
     assign_slots(c, nown, elem_own, atom_of_slot, slot_of_atom, block_type);
-    const int nblocks = static_cast<int>(block_type.size());
-    c->nblk_type[0] = c->nblk_type[1] = 0;
-    for (int32_t t : block_type) ++c->nblk_type[t];
-    c->owners_per_type.assign(m.n_type, 0);
-    for (int e : elem_own)
-        if (e >= 0 && e < m.n_type) ++c->owners_per_type[e];
-    c->npairs = npairs;
-    const size_t nslot = static_cast<size_t>(nblocks) * kBlock;
-    const size_t asz = nslot * m.n_fn * m.n_type * m.n_lmh * 2;
-    bool ok = upload(c, c->atom_of_slot, atom_of_slot.data(), atom_of_slot.size(), "slots") &&
-              upload(c, c->slot_of_atom, slot_of_atom.data(), slot_of_atom.size(), "slots") &&
-              upload(c, c->block_type, block_type.data(), block_type.size(), "block types");
-    if (!ok) return false;
-    const size_t np = static_cast<size_t>(m.n_p);
-    if (!c->A.ensure(asz) || !c->G.ensure(asz) || !c->D.ensure(nslot * np) || !c->Hq.ensure(nslot * np) ||
-        !c->H.ensure(nslot * np) || !c->Ep.ensure(nslot) || !c->Eq12.ensure(nslot * m.n_fn) ||
-        !c->Eq34.ensure(nslot * m.n_fn * kWavesPerWg) || !c->gpair.ensure(static_cast<size_t>(npairs) * 3) ||
+    const WorkSizes z = work_sizes(m, nall, nown, npairs, static_cast<int>(block_type.size()));
+    if (!c->atom_of_slot.ensure(z.nslot) || !c->slot_of_atom.ensure(nown) || !c->block_type.ensure(z.nblocks) ||
+        !c->A.ensure(z.asz) || !c->G.ensure(z.asz) || !c->D.ensure(z.psz) || !c->Hq.ensure(z.psz) ||
+        !c->H.ensure(z.psz) || !c->Ep.ensure(z.nslot) || !c->Eq12.ensure(z.nslot * m.n_fn) ||
+        !c->Eq34.ensure(z.nslot * m.n_fn * kWavesPerWg) || !c->gpair.ensure(static_cast<size_t>(npairs) * 3) ||
         !c->vown.ensure(static_cast<size_t>(nown) * 6) || !c->f.ensure(static_cast<size_t>(nall) * 3) ||
-        !c->e_slot.ensure(nslot) || !c->red.ensure(7 * 128) || !c->out.ensure(8)) {
+        !c->e_slot.ensure(z.nslot) || !c->red.ensure(7 * 128) || !c->out.ensure(8)) {
         c->err = "device allocation failed for structure buffers";
         return false;
     }
-    if (!hip_ok(c, hipMemsetAsync(c->A.p, 0, asz * sizeof(double), c->stream), "clear a_nlm")) return false;
-    if (!hip_ok(c, hipMemsetAsync(c->vown.p, 0, static_cast<size_t>(nown) * 6 * sizeof(double), c->stream),
-                "clear virial"))
-        return false;
-    DevSystem& s = c->ds;
-    s.nall = nall;
-    s.nown = nown;
-    s.nblocks = nblocks;
-    s.npairs = npairs;
-    s.x = c->x.p;
-    s.elem = c->elem.p;
-    s.poff = c->poff.p;
-    s.pend = c->poff.p + 1;
-    s.pnbr = c->pnbr.p;
-    s.pcen = has_pcen ? c->pcen.p : nullptr;
-    s.pshift = has_shift ? c->pshift.p : nullptr;
-    s.atom_of_slot = c->atom_of_slot.p;
-    s.slot_of_atom = c->slot_of_atom.p;
-    s.block_type = c->block_type.p;
-    s.A = c->A.p;
-    s.G = c->G.p;
-    s.D = c->D.p;
-    s.Hq = c->Hq.p;
-    s.H = c->H.p;
-    s.Ep = c->Ep.p;
-    s.Eq12 = c->Eq12.p;
-    s.Eq34 = c->Eq34.p;
-    s.gpair = c->gpair.p;
-    s.vown = c->vown.p;
-    s.goff = c->goff.p;
-    s.gend = c->goff.p + 1;
-    s.gent = c->gent.p;
-    s.f = c->f.p;
-    s.e_slot = c->e_slot.p;
-    s.red = c->red.p;
-    s.out = c->out.p;
-    return true;
+    SysBufs b{};
+    b.x = c->x.p;
+    b.pshift = has_shift ? c->pshift.p : nullptr;
+    b.elem = c->elem.p;
+    b.poff = c->poff.p;
+    b.pnbr = c->pnbr.p;
+    b.pcen = has_pcen ? c->pcen.p : nullptr;
+    b.goff = c->goff.p;
+    b.gent = c->gent.p;
+    b.atom_of_slot = c->atom_of_slot.p;
+    b.slot_of_atom = c->slot_of_atom.p;
+    b.block_type = c->block_type.p;
+    b.A = c->A.p;
+    b.G = c->G.p;
+    b.D = c->D.p;
+    b.Hq = c->Hq.p;
+    b.H = c->H.p;
+    b.Ep = c->Ep.p;
+    b.Eq12 = c->Eq12.p;
+    b.Eq34 = c->Eq34.p;
+    b.gpair = c->gpair.p;
+    b.vown = c->vown.p;
+    b.f = c->f.p;
+    b.e_slot = c->e_slot.p;
+    b.red = c->red.p;
+    b.out = c->out.p;
+    return bind_system(c, nall, nown, npairs, elem_own, atom_of_slot, slot_of_atom, block_type, b);
 }
 
 // Upload a structure with host-built lists and lay it out.
@@ -413,6 +441,82 @@ void build_gather(int nall, const std::vector<int32_t>& poff, const std::vector<
     }
 }
 
+WorkSizes work_sizes(const Model& m, int nall, int nown, int64_t npairs, int nblocks) {
+    WorkSizes z;
+    z.nblocks = static_cast<size_t>(nblocks);
+    z.nslot = z.nblocks * kBlock;
+    z.asz = z.nslot * m.n_fn * m.n_type * m.n_lmh * 2;
+    z.psz = z.nslot * static_cast<size_t>(m.n_p);
+    (void)nall;
+    (void)nown;
+    (void)npairs;
+    return z;
+}
+
+// Slot arrays into b (copied on the stream from host vectors that must live
+// until the stream has passed them: the caller synchronises before it
+// returns), a_nlm cleared, and the DevSystem view of the structure.
+bool bind_system(mlip_ctx* c, int nall, int nown, int64_t npairs, const std::vector<int32_t>& elem_own,
+                 const std::vector<int32_t>& atom_of_slot, const std::vector<int32_t>& slot_of_atom,
+                 const std::vector<int32_t>& block_type, const SysBufs& b) {
+    const Model& m = c->model;
+    c->nstruct = 0;
+    c->sres = nullptr;
+    c->total_reduce = true;
+    const int nblocks = static_cast<int>(block_type.size());
+    c->nblk_type[0] = c->nblk_type[1] = 0;
+    for (int32_t t : block_type) ++c->nblk_type[t];
+    c->owners_per_type.assign(m.n_type, 0);
+    for (int e : elem_own)
+        if (e >= 0 && e < m.n_type) ++c->owners_per_type[e];
+    c->npairs = npairs;
+    const WorkSizes z = work_sizes(m, nall, nown, npairs, nblocks);
+    hipStream_t st = c->stream;
+    bool ok = hip_ok(c, hipMemcpyAsync(b.atom_of_slot, atom_of_slot.data(), z.nslot * sizeof(int32_t),
+                                       hipMemcpyHostToDevice, st), "slots") &&
+              hip_ok(c, hipMemcpyAsync(b.slot_of_atom, slot_of_atom.data(), nown * sizeof(int32_t),
+                                       hipMemcpyHostToDevice, st), "slots") &&
+              hip_ok(c, hipMemcpyAsync(b.block_type, block_type.data(), z.nblocks * sizeof(int32_t),
+                                       hipMemcpyHostToDevice, st), "block types") &&
+              hip_ok(c, launch_fill(b.A, z.asz, st), "clear a_nlm") &&
+              hip_ok(c, launch_fill(b.vown, static_cast<size_t>(nown) * 6, st), "clear virial");
+    if (!ok) return false;
+    DevSystem& s = c->ds;
+    s = DevSystem{};
+    s.nall = nall;
+    s.nown = nown;
+    s.nblocks = nblocks;
+    s.npairs = npairs;
+    s.x = b.x;
+    s.elem = b.elem;
+    s.poff = b.poff;
+    s.pend = b.poff + 1;
+    s.pnbr = b.pnbr;
+    s.pcen = b.pcen;
+    s.pshift = b.pshift;
+    s.atom_of_slot = b.atom_of_slot;
+    s.slot_of_atom = b.slot_of_atom;
+    s.block_type = b.block_type;
+    s.A = b.A;
+    s.G = b.G;
+    s.D = b.D;
+    s.Hq = b.Hq;
+    s.H = b.H;
+    s.Ep = b.Ep;
+    s.Eq12 = b.Eq12;
+    s.Eq34 = b.Eq34;
+    s.gpair = b.gpair;
+    s.vown = b.vown;
+    s.goff = b.goff;
+    s.gend = b.goff + 1;
+    s.gent = b.gent;
+    s.f = b.f;
+    s.e_slot = b.e_slot;
+    s.red = b.red;
+    s.out = b.out;
+    return true;
+}
+
 }  // namespace mlip
 
 namespace {
@@ -449,59 +553,50 @@ bool run_pipeline(mlip_ctx* c) {
     const DevSystem& s = c->ds;
     const Model& mm = c->model;
     hipStream_t st = c->stream;
+    // the first failing launch names the call's error; later launches are
+    // not enqueued after one failed
+    bool ok = true;
+    auto run = [&](const char* name, auto&& launch) {
+        if (!ok) return;
+        Scope sc(c, name);
+        const hipError_t e = launch();
+        if (e != hipSuccess) {
+            ok = false;
+            c->err = std::string("kernel launch ") + name + ": " + hipGetErrorString(e);
+        }
+    };
     if (mm.pair_model) {
         // radial-only style: features, polynomial and H, pair forces
-        { Scope sc(c, "pair_features"); launch_pair_features(m, s, st); }
-        { Scope sc(c, "pair_poly"); launch_pair_poly(m, s, st); }
-        { Scope sc(c, "force"); launch_pair_force(m, s, st); }
-        { Scope sc(c, "gather"); launch_gather(m, s, st); }
-        { Scope sc(c, "energy_virial"); launch_energy_virial(m, s, c->total_reduce, st); }
-        if (c->nstruct > 0) {
-            Scope sc(c, "struct_reduce");
-            launch_struct_reduce(s, c->sbeg, c->send, c->nstruct, c->sres, st);
-        }
-        return hip_ok(c, hipGetLastError(), "kernel launch");
+        run("pair_features", [&] { return launch_pair_features(m, s, st); });
+        run("pair_poly", [&] { return launch_pair_poly(m, s, st); });
+        run("force", [&] { return launch_pair_force(m, s, st); });
+        run("gather", [&] { return launch_gather(m, s, st); });
+        run("energy_virial", [&] { return launch_energy_virial(m, s, c->total_reduce, st); });
+        if (c->nstruct > 0)
+            run("struct_reduce", [&] { return launch_struct_reduce(s, c->sbeg, c->send, c->nstruct, c->sres, st); });
+        return ok;
     }
-    { Scope sc(c, "anlm"); launch_anlm(m, s, c->precision, st); }
+    run("anlm", [&] { return launch_anlm(m, s, c->precision, st); });
     if (m.n_p > 0) {
-        { Scope sc(c, "q12_feat"); launch_q12(m, s, true, st); }
-        if (mm.has_feat34) { Scope sc(c, "q34_feat"); launch_q34_feat(m, s, st); }
-        {
-            Scope sc(c, "poly_gemm");
-            // Hq = D W per species: slot blocks are grouped by species and a
-            // species' row holds only the features it sees, so each species
-            // is one dense GEMM on its rows and its own quadratic form
-            const double one = 1.0, zero = 0.0;
-            int b0 = 0;
-            for (int t = 0; t < mm.n_type && t < 2; ++t) {
-                const int nslot_t = c->nblk_type[t] * kBlock, w = c->wdim[t];
-                const size_t r0 = static_cast<size_t>(b0) * kBlock * m.n_pr;
-                b0 += c->nblk_type[t];
-                if (nslot_t == 0 || w == 0) continue;
-                if (rocblas_dgemm(c->blas, rocblas_operation_none, rocblas_operation_none, w, nslot_t, w, &one,
-                                  c->d_wmat.p + c->wofs[t], w, s.D + r0, m.n_pr, &zero, s.Hq + r0, m.n_pr) !=
-                    rocblas_status_success) {
-                    c->err = "rocblas_dgemm failed";
-                    return false;
-                }
-            }
-            if (b0 != s.nblocks) {
-                c->err = "internal: slot blocks per species do not add up";
-                return false;
-            }
-        }
-        { Scope sc(c, "poly"); launch_poly(m, s, st); }
+        run("q12_feat", [&] { return launch_q12(m, s, true, st); });
+        if (mm.has_feat34) run("q34_feat", [&] { return launch_q34_feat(m, s, st); });
+        if (mm.has_hi) run("qhigh_feat", [&] { return launch_qhigh(m, s, true, st); });
+        if (!ok) return false;
+        // Hq = D W per species: slot blocks are grouped by species and a
+        // species' row holds only the features it sees, so each species is
+        // one dense product on its rows and its own quadratic form
+        run("poly_gemm", [&] { return launch_poly_gemm(m, s, c->nblk_type, st); });
+        run("poly", [&] { return launch_poly(m, s, st); });
     }
-    { Scope sc(c, "q12_adj"); launch_q12(m, s, false, st); }
-    if (mm.has_order34) { Scope sc(c, "q34_adj"); launch_q34_adj(m, s, st); }
-    { Scope sc(c, "force"); launch_force(m, s, c->precision, c->force_threads, st); }
-    { Scope sc(c, "gather"); launch_gather(m, s, st); }
-    { Scope sc(c, "energy_virial"); launch_energy_virial(m, s, c->total_reduce, st); }
-    if (c->nstruct > 0) {
-        Scope sc(c, "struct_reduce");
-        launch_struct_reduce(s, c->sbeg, c->send, c->nstruct, c->sres, st);
-    }
-    return hip_ok(c, hipGetLastError(), "kernel launch");
+    run("q12_adj", [&] { return launch_q12(m, s, false, st); });
+    if (mm.has_order34) run("q34_adj", [&] { return launch_q34_adj(m, s, st); });
+    if (mm.has_hi) run("qhigh_adj", [&] { return launch_qhigh(m, s, false, st); });
+    run("force", [&] { return launch_force(m, s, c->precision, c->force_threads, st); });
+    run("gather", [&] { return launch_gather(m, s, st); });
+    run("energy_virial", [&] { return launch_energy_virial(m, s, c->total_reduce, st); });
+    if (c->nstruct > 0)
+        run("struct_reduce", [&] { return launch_struct_reduce(s, c->sbeg, c->send, c->nstruct, c->sres, st); });
+    return ok;
 }
 
 void drop_graph(mlip_ctx* c) {
@@ -511,9 +606,9 @@ void drop_graph(mlip_ctx* c) {
 }
 
 // One evaluation: replays the captured graph when graph mode is on.  After a
-// structure or setting change the first call runs eagerly (rocBLAS loads its
-// kernels and workspace outside any capture) and the second captures.
-// Profiling events are not recorded inside a capture.
+// structure or setting change the first call runs eagerly (kernel attributes
+// such as the dynamic LDS limits are set outside any capture) and the second
+// captures.  Profiling events are not recorded inside a capture.
 bool run_eval(mlip_ctx* c) {
     if (!c->use_graph || c->prof) return run_pipeline(c);
     if (!c->gexec && !c->graph_warm) {
@@ -538,6 +633,39 @@ bool run_eval(mlip_ctx* c) {
         }
     }
     return hip_ok(c, hipGraphLaunch(c->gexec, c->stream), "graph launch");
+}
+
+// Per-atom energy and virial in the reference's pair split, after an
+// evaluation: the split adjoint (every polynomial term of dE/da weighted by
+// 1 / its degree, linear families by 1 / their order) overwrites G, then
+// per-pair shares (epv [npairs][7]) and per-position half sums (evat
+// [nall][7]).
+bool run_atom_split(mlip_ctx* c, double* epv, double* evat) {
+    const Model& mm = c->model;
+    DevModel me = c->dm;
+    DevSystem se = c->ds;
+    hipStream_t st = c->stream;
+    bool ok = true;
+    auto run = [&](const char* name, hipError_t e) {
+        if (ok && e != hipSuccess) {
+            ok = false;
+            c->err = std::string("kernel launch ") + name + ": " + hipGetErrorString(e);
+        }
+        return ok;
+    };
+    if (mm.pair_model) {
+        run("pair_model_energy", launch_pair_model_energy(me, se, epv, st));
+    } else {
+        me.weight = c->d_weight_e.p;
+        se.H = se.Hq;
+        if (me.n_p > 0) run("poly_split", launch_poly_split(me, se, st));
+        if (ok) run("q12_split", launch_q12(me, se, false, st));
+        if (ok && mm.has_order34) run("q34_split", launch_q34_adj(me, se, st));
+        if (ok && mm.has_hi) run("qhigh_split", launch_qhigh(me, se, false, st));
+        if (ok) run("pair_energy", launch_pair_energy(me, se, epv, st));
+    }
+    if (ok) run("atom_ev", launch_atom_ev(se, epv, evat, st));
+    return ok;
 }
 
 void collect_profile(mlip_ctx* c) {
@@ -568,19 +696,51 @@ int lammps_compute(mlip_ctx* c, int nlocal, int nghost, const double* x, const i
                    double* eng, double* virial, double* eatom, double* vatom) {
     (void)eflag;
     (void)vflag;
+    LammpsLists L;
+    const int rc = lammps_lists(c, nlocal, nghost, x, type, tag, inum, ilist, numneigh, firstneigh, neigh_off,
+                                neigh_idx, neigh_full, L);
+    if (rc != MLIP_OK) return rc;
+    c->resident = false;
+    c->nstruct = 0;
+    if (!setup_system(c, L.nall, inum, L.npairs, L.x, L.elem, L.elem_own, L.poff, L.pnbr, &L.pcen, nullptr, L.gof,
+                      L.gen))
+        return MLIP_ERR_DEVICE;
+    if (!run_pipeline(c)) return MLIP_ERR_DEVICE;
+    double* epv = nullptr;
+    double* evat = nullptr;
+    if ((eatom || vatom) && c->atom_mode == MLIP_ATOM_PAIR_SPLIT) {
+        if (!c->epv.ensure(7 * static_cast<size_t>(std::max<int64_t>(L.npairs, 1))) ||
+            !c->evat.ensure(7 * static_cast<size_t>(std::max(L.nall, 1)))) {
+            c->err = "device allocation failed for per-atom energies";
+            return MLIP_ERR_DEVICE;
+        }
+        epv = c->epv.p;
+        evat = c->evat.p;
+    }
+    return lammps_results(c, L.nall, inum, ilist, f, eng, virial, eatom, vatom, epv, evat);
+}
+
+}  // namespace
+
+namespace mlip {
+
+int lammps_lists(mlip_ctx* c, int nlocal, int nghost, const double* x, const int* type, const int64_t* tag, int inum,
+                 const int* ilist, const int* numneigh, const int* const* firstneigh, const int* neigh_off,
+                 const int* neigh_idx, int neigh_full, LammpsLists& L) {
     const int nall = nlocal + nghost;
-    if (nlocal < 0 || nghost < 0 || inum < 0 || inum > nlocal || !x || !type) {
+    if (nlocal < 0 || nghost < 0 || inum < 0 || inum > nlocal || !x || !type || (inum > 0 && (!ilist || !numneigh))) {
         c->err = "bad atom counts or null arrays";
         return MLIP_ERR_ARG;
     }
-    std::vector<int32_t> elem(nall);
+    L.nall = nall;
+    L.elem.assign(nall, 0);
     for (int k = 0; k < nall; ++k) {
         const int t = type[k];
         if (t < 1 || t > static_cast<int>(c->elem_of_type.size()) || c->elem_of_type[t - 1] < 0) {
             c->err = "atom type without an element mapping";
             return MLIP_ERR_ARG;
         }
-        elem[k] = c->elem_of_type[t - 1];
+        L.elem[k] = c->elem_of_type[t - 1];
     }
     // owners: ilist order; half lists need the owner of each ghost (by tag)
     std::vector<int> owner_of_local(nlocal, -1);
@@ -610,6 +770,10 @@ int lammps_compute(mlip_ctx* c, int nlocal, int nghost, const double* x, const i
         int n = 0;
         const int* js = list_of(ii, n);
         const int i = ilist[ii];
+        if (n < 0 || (n > 0 && !js)) {
+            c->err = "bad neighbor count or list";
+            return MLIP_ERR_ARG;
+        }
         for (int jj = 0; jj < n; ++jj) {
             const int j = js[jj];
             if (j < 0 || j >= nall) {
@@ -627,46 +791,77 @@ int lammps_compute(mlip_ctx* c, int nlocal, int nghost, const double* x, const i
             }
         }
     }
-    std::vector<int32_t> poff(inum + 1, 0), pnbr, pcen;
+    L.poff.assign(inum + 1, 0);
+    L.pnbr.clear();
+    L.pcen.clear();
     for (int ii = 0; ii < inum; ++ii) {
-        poff[ii + 1] = poff[ii] + static_cast<int32_t>(per[ii].size());
+        L.poff[ii + 1] = L.poff[ii] + static_cast<int32_t>(per[ii].size());
         for (const auto& e : per[ii]) {
-            pcen.push_back(e.first);
-            pnbr.push_back(e.second);
+            L.pcen.push_back(e.first);
+            L.pnbr.push_back(e.second);
         }
     }
-    const int64_t npairs = static_cast<int64_t>(pnbr.size());
+    L.npairs = static_cast<int64_t>(L.pnbr.size());
+    if (L.npairs > (INT32_MAX - 1) / 2) {
+        c->err = "more than 2^30 pairs";  // gather offsets reach 2 * npairs
+        return MLIP_ERR_ARG;
+    }
     // owners are numbered by ilist position; centre and neighbor positions
     // keep their LAMMPS indices through pcen / pnbr
-    std::vector<double> xv(x, x + 3 * static_cast<size_t>(nall));
-    std::vector<int32_t> elem_own(inum);
-    for (int ii = 0; ii < inum; ++ii) elem_own[ii] = elem[ilist[ii]];
-    std::vector<int32_t> gof, gen;
-    build_gather(nall, poff, pnbr, &pcen, gof, gen);
-    c->resident = false;
-    c->nstruct = 0;
-    if (!setup_system(c, nall, inum, npairs, xv, elem, elem_own, poff, pnbr, &pcen, nullptr, gof, gen))
-        return MLIP_ERR_DEVICE;
-    if (!run_pipeline(c)) return MLIP_ERR_DEVICE;
+    L.x.assign(x, x + 3 * static_cast<size_t>(nall));
+    L.elem_own.assign(inum, 0);
+    for (int ii = 0; ii < inum; ++ii) L.elem_own[ii] = L.elem[ilist[ii]];
+    build_gather(nall, L.poff, L.pnbr, &L.pcen, L.gof, L.gen);
+    return MLIP_OK;
+}
+
+int lammps_results(mlip_ctx* c, int nall, int inum, const int* ilist, double* f, double* eng, double* virial,
+                   double* eatom, double* vatom, double* epv, double* evat) {
     std::vector<double> fh(3 * static_cast<size_t>(nall)), outv(7);
-    if (!hip_ok(c, hipMemcpyAsync(fh.data(), c->f.p, fh.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream),
+    if (!hip_ok(c, hipMemcpyAsync(fh.data(), c->ds.f, fh.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream),
                 "copy forces") ||
-        !hip_ok(c, hipMemcpyAsync(outv.data(), c->out.p, 7 * sizeof(double), hipMemcpyDeviceToHost, c->stream),
+        !hip_ok(c, hipMemcpyAsync(outv.data(), c->ds.out, 7 * sizeof(double), hipMemcpyDeviceToHost, c->stream),
                 "copy energy") ||
         !hip_ok(c, hipStreamSynchronize(c->stream), "compute"))
         return MLIP_ERR_DEVICE;
     collect_profile(c);
-    for (size_t k = 0; k < fh.size(); ++k) f[k] += fh[k];
+    if (f)
+        for (size_t k = 0; k < fh.size(); ++k) f[k] += fh[k];
     if (eng) *eng += outv[0];
     if (virial)
         for (int k = 0; k < 6; ++k) virial[k] += outv[1 + k];
-    if (eatom || vatom) {
+    if ((eatom || vatom) && c->atom_mode == MLIP_ATOM_PAIR_SPLIT) {
+        // the reference's ev_tally_xyz split: every position gets half of
+        // each pair energy and pair virial it takes part in
+        if (!run_atom_split(c, epv, evat)) return MLIP_ERR_DEVICE;
+        std::vector<double> ev(7 * static_cast<size_t>(nall));
+        if (!hip_ok(c, hipMemcpyAsync(ev.data(), evat, ev.size() * sizeof(double), hipMemcpyDeviceToHost,
+                                      c->stream),
+                    "copy per-atom energy") ||
+            !hip_ok(c, hipStreamSynchronize(c->stream), "per-atom energy"))
+            return MLIP_ERR_DEVICE;
+        for (int k = 0; k < nall; ++k) {
+            if (eatom) eatom[k] += ev[7 * static_cast<size_t>(k)];
+            if (vatom)
+                for (int q = 0; q < 6; ++q)
+                    vatom[6 * static_cast<size_t>(k) + q] += ev[7 * static_cast<size_t>(k) + 1 + q];
+        }
+    } else if (eatom || vatom) {
+        // site energies E_i and the virial of the pairs each centre owns
         const size_t nslot = static_cast<size_t>(c->ds.nblocks) * kBlock;
         std::vector<double> es(nslot), vo(static_cast<size_t>(inum) * 6);
         std::vector<int32_t> sl(inum);
-        (void)hipMemcpy(es.data(), c->e_slot.p, nslot * sizeof(double), hipMemcpyDeviceToHost);
-        (void)hipMemcpy(vo.data(), c->vown.p, vo.size() * sizeof(double), hipMemcpyDeviceToHost);
-        (void)hipMemcpy(sl.data(), c->slot_of_atom.p, inum * sizeof(int32_t), hipMemcpyDeviceToHost);
+        if (!hip_ok(c, hipMemcpyAsync(es.data(), c->ds.e_slot, nslot * sizeof(double), hipMemcpyDeviceToHost,
+                                      c->stream),
+                    "copy site energies") ||
+            !hip_ok(c, hipMemcpyAsync(vo.data(), c->ds.vown, vo.size() * sizeof(double), hipMemcpyDeviceToHost,
+                                      c->stream),
+                    "copy site virials") ||
+            !hip_ok(c, hipMemcpyAsync(sl.data(), c->ds.slot_of_atom, inum * sizeof(int32_t), hipMemcpyDeviceToHost,
+                                      c->stream),
+                    "copy slots") ||
+            !hip_ok(c, hipStreamSynchronize(c->stream), "site energies"))
+            return MLIP_ERR_DEVICE;
         for (int ii = 0; ii < inum; ++ii) {
             if (eatom) eatom[ilist[ii]] += es[sl[ii]];
             if (vatom)
@@ -675,6 +870,10 @@ int lammps_compute(mlip_ctx* c, int nlocal, int nghost, const double* x, const i
     }
     return MLIP_OK;
 }
+
+}  // namespace mlip
+
+namespace {
 
 }  // namespace
 
@@ -721,10 +920,8 @@ int mlip_create(const char* potfile, int ntypes, const char* const* elem_of_type
         return MLIP_ERR_DEVICE;
     }
     (void)hipGetDevice(&c->device);
-    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-        rocblas_create_handle(&c->blas) != rocblas_status_success ||
-        rocblas_set_stream(c->blas, c->stream) != rocblas_status_success) {
-        set_err(errbuf, errlen, "failed to create HIP stream / rocBLAS handle");
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        set_err(errbuf, errlen, "failed to create a HIP stream");
         mlip_destroy(c);
         return MLIP_ERR_DEVICE;
     }
@@ -744,11 +941,12 @@ void mlip_destroy(mlip_ctx* c) {
         (void)hipEventDestroy(e.a);
         (void)hipEventDestroy(e.b);
     }
-    DBuf<double>* dbs[] = {&c->d_coef, &c->d_weight, &c->d_wmat, &c->d_linp, &c->d_c3w, &c->x, &c->pshift,
+    DBuf<double>* dbs[] = {&c->d_coef, &c->d_weight, &c->d_weight_e, &c->d_wmat, &c->d_linp, &c->d_c3w, &c->x,
+                           &c->pshift, &c->epv, &c->evat,
                            &c->A, &c->G, &c->D, &c->Hq, &c->H, &c->Ep, &c->Eq12, &c->Eq34, &c->gpair,
                            &c->vown, &c->f, &c->e_slot, &c->red, &c->out, &c->sout};
     for (auto* b : dbs) b->release();
-    DBuf<int32_t>* ibs[] = {&c->d_q12off, &c->d_c3off, &c->d_c3bc, &c->elem, &c->poff, &c->pnbr, &c->pcen,
+    DBuf<int32_t>* ibs[] = {&c->d_q12off, &c->d_c3off, &c->d_c3bc, &c->d_pdeg, &c->elem, &c->poff, &c->pnbr, &c->pcen,
                             &c->atom_of_slot, &c->slot_of_atom, &c->block_type, &c->goff, &c->gent,
                             &c->soff, &c->d_zlist};
     for (auto* b : ibs) b->release();
@@ -762,13 +960,15 @@ void mlip_destroy(mlip_ctx* c) {
     c->d_pairw.release();
     c->d_phoff.release();
     c->d_phent.release();
+    c->d_hvis.release();
+    c->d_hterm.release();
+    c->d_hcoef.release();
     drop_graph(c);
     pool_free(c);
     nb_free(c);
     train_free(c);
     c->soff.release();
     c->sout.release();
-    if (c->blas) rocblas_destroy_handle(c->blas);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -786,6 +986,12 @@ int mlip_set_precision(mlip_ctx* c, int mode) {
     if (!c || (mode != MLIP_FP64 && mode != MLIP_MIXED)) return MLIP_ERR_ARG;
     if (c->precision != mode) drop_graph(c);
     c->precision = mode;
+    return MLIP_OK;
+}
+
+int mlip_set_atom_mode(mlip_ctx* c, int mode) {
+    if (!c || (mode != MLIP_ATOM_PAIR_SPLIT && mode != MLIP_ATOM_SITE)) return MLIP_ERR_ARG;
+    c->atom_mode = mode;
     return MLIP_OK;
 }
 
@@ -1096,12 +1302,17 @@ int mlip_autotune(mlip_ctx* c, int reps, int* chosen) {
         hipEvent_t a, b;
         if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return MLIP_ERR_DEVICE;
         (void)hipEventRecord(a, c->stream);
-        for (int r = 0; r < reps; ++r) launch_force(c->dm, c->ds, c->precision, nt, c->stream);
+        hipError_t le = hipSuccess;
+        for (int r = 0; r < reps && le == hipSuccess; ++r) le = launch_force(c->dm, c->ds, c->precision, nt, c->stream);
         (void)hipEventRecord(b, c->stream);
         float ms = 0.0f;
         const bool ok = hipEventSynchronize(b) == hipSuccess && hipEventElapsedTime(&ms, a, b) == hipSuccess;
         (void)hipEventDestroy(a);
         (void)hipEventDestroy(b);
+        if (le != hipSuccess) {
+            c->err = std::string("force kernel launch: ") + hipGetErrorString(le);
+            return MLIP_ERR_DEVICE;
+        }
         if (!ok) return MLIP_ERR_DEVICE;
         if (ms < best) {
             best = ms;
@@ -1211,6 +1422,18 @@ int mlip_flops(const mlip_ctx* c, double* out7) {
         };
         out7[2] += na * terms(tt.groups_feat, tt.visits_feat, false);
         out7[3] += na * terms(tt.groups_adj, tt.visits_adj, true);
+        // orders 5/6: per term the product of the other factors (6 per
+        // complex product after the first), then 4 (adjoint: one complex
+        // FMA pair into G) or 5 (features); per visit the G update and the
+        // energy dot product, 10 (L + 1)
+        auto hi_terms = [&](const std::vector<VisitHi>& vs, bool adj) {
+            double f = 0.0;
+            for (const VisitHi& v : vs)
+                f += (v.tend - v.tbeg) * (6.0 * (v.order - 2) + (adj ? 4.0 : 5.0)) + (adj ? 10.0 * (v.L + 1) : 0.0);
+            return f;
+        };
+        out7[2] += na * hi_terms(tt.hi_feat, false);
+        out7[3] += na * hi_terms(tt.hi_adj, true);
         const double np = m.n_p, w = static_cast<double>(m.prange[t][1] - m.prange[t][0]) * m.n_fn;
         out7[4] += static_cast<double>(c->owners_per_type[t]) * (np > 0 ? 2.0 * w * w + 6.0 * w + 12.0 * m.c3_w.size() / np : 0.0);
     }
@@ -1290,8 +1513,8 @@ int mlip_calibration_copy(mlip_ctx* c, int64_t nbytes, int reps) {
         return MLIP_ERR_DEVICE;
     }
     (void)hipMemsetAsync(a, 1, nbytes, c->stream);
-    launch_copy(b, a, nbytes, reps, c->stream);
-    const bool ok = hipStreamSynchronize(c->stream) == hipSuccess;
+    const bool ok = launch_copy(b, a, nbytes, reps, c->stream) == hipSuccess &&
+                    hipStreamSynchronize(c->stream) == hipSuccess;
     (void)hipFree(a);
     (void)hipFree(b);
     return ok ? MLIP_OK : MLIP_ERR_DEVICE;
@@ -1315,6 +1538,22 @@ int64_t mlip_debug_fetch(mlip_ctx* c, int what, double* out, int64_t cap) {
     } else if (what == 3) {
         src = c->G.p;
         n = nslot * m.n_fn * m.n_type * m.n_lmh * 2;
+    } else if (what == 4) {
+        src = c->D.p ? c->Hq.p : nullptr;
+        n = nslot * m.n_p;
+    } else if (what == 5) {
+        src = c->d_wmat.p;
+        n = c->wofs[1] + static_cast<size_t>(c->wdim[1]) * c->wdim[1];
+    } else if (what == 6) {
+        // layout of the per-species products: row pitch, then per species
+        // (offset of its quadratic form, its dimension, its slot blocks)
+        const double v[7] = {static_cast<double>(c->dm.n_pr), static_cast<double>(c->wofs[0]),
+                             static_cast<double>(c->wdim[0]), static_cast<double>(c->nblk_type[0]),
+                             static_cast<double>(c->wofs[1]), static_cast<double>(c->wdim[1]),
+                             static_cast<double>(c->nblk_type[1])};
+        if (cap < 7) return -1;
+        std::copy(v, v + 7, out);
+        return 7;
     }
     if (!src || static_cast<int64_t>(n) > cap) return -1;
     if (hipStreamSynchronize(c->stream) != hipSuccess) return -1;

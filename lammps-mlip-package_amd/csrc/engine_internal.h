@@ -4,7 +4,6 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
-#include <rocblas/rocblas.h>
 
 #include <string>
 #include <vector>
@@ -56,12 +55,11 @@ struct mlip_ctx {
     int precision = mlip::kFP64;
     std::string err;
     hipStream_t stream = nullptr;
-    rocblas_handle blas = nullptr;
     int device = 0;
     // model tables
-    mlip::DBuf<double> d_coef, d_weight, d_wmat, d_linp, d_c3w;
+    mlip::DBuf<double> d_coef, d_weight, d_wmat, d_linp, d_c3w, d_weight_e;
     mlip::DBuf<mlip::VisitQ12> d_q12;
-    mlip::DBuf<int32_t> d_q12off, d_c3off, d_c3bc;
+    mlip::DBuf<int32_t> d_q12off, d_c3off, d_c3bc, d_pdeg;
     mlip::DBuf<mlip::VisitPack> d_vadj, d_vfeat;
     mlip::DBuf<mlip::GroupQ34> d_gadj, d_gfeat;
     mlip::DBuf<mlip::YTab<double>> d_yt64;
@@ -69,9 +67,16 @@ struct mlip_ctx {
     mlip::DBuf<double> d_pairw;
     mlip::DBuf<int32_t> d_phoff, d_zlist;
     mlip::DBuf<mlip::PairHEntry> d_phent;
+    mlip::DBuf<mlip::VisitHi> d_hvis;
+    mlip::DBuf<int32_t> d_hterm;
+    mlip::DBuf<double> d_hcoef;
     mlip::DevModel dm{};
     // structure buffers
     mlip::DBuf<double> x, pshift, A, G, D, Hq, H, Ep, Eq12, Eq34, gpair, vown, f, e_slot, red, out;
+    // per-atom energy and virial in the reference's pair split: per directed
+    // pair (energy share, virial[6]) and per position (the same 7 sums)
+    mlip::DBuf<double> epv, evat;
+    int atom_mode = MLIP_ATOM_PAIR_SPLIT;
     mlip::DBuf<int32_t> elem, poff, pnbr, pcen, atom_of_slot, slot_of_atom, block_type, goff, gent;
     mlip::DevSystem ds{};
     bool resident = false;
@@ -120,6 +125,26 @@ namespace mlip {
 
 bool hip_ok(mlip_ctx* c, hipError_t e, const char* what);
 
+// Device buffers of one structure (positions, lists, slots, work arrays).
+struct SysBufs {
+    double *x, *pshift, *A, *G, *D, *Hq, *H, *Ep, *Eq12, *Eq34, *gpair, *vown, *f, *e_slot, *red, *out;
+    int32_t *elem, *poff, *pnbr, *pcen, *goff, *gent, *atom_of_slot, *slot_of_atom, *block_type;
+};
+// Element counts of the slot-indexed work arrays of a structure.
+struct WorkSizes {
+    size_t nblocks, nslot, asz, psz;  // asz: a_nlm / G doubles, psz: D / Hq / H doubles
+};
+WorkSizes work_sizes(const Model& m, int nall, int nown, int64_t npairs, int nblocks);
+void assign_slots(const mlip_ctx* c, int nown, const std::vector<int>& elem_of_owner,
+                  std::vector<int32_t>& atom_of_slot, std::vector<int32_t>& slot_of_atom,
+                  std::vector<int32_t>& block_type);
+
+// Slot arrays uploaded into b, a_nlm and the per-centre virial cleared,
+// c->ds pointed at b (positions, elements and lists must be in b already).
+bool bind_system(mlip_ctx* c, int nall, int nown, int64_t npairs, const std::vector<int32_t>& elem_own,
+                 const std::vector<int32_t>& atom_of_slot, const std::vector<int32_t>& slot_of_atom,
+                 const std::vector<int32_t>& block_type, const SysBufs& b);
+
 template <typename T>
 bool upload(mlip_ctx* c, DBuf<T>& b, const T* h, size_t n, const char* what) {
     if (!b.ensure(n)) {
@@ -133,6 +158,24 @@ bool upload(mlip_ctx* c, DBuf<T>& b, const T* h, size_t n, const char* what) {
 void drop_graph(mlip_ctx* c);
 bool run_pipeline(mlip_ctx* c);
 bool run_eval(mlip_ctx* c);
+bool run_atom_split(mlip_ctx* c, double* epv, double* evat);
+// LAMMPS-shaped input (mlip_compute / mlip_compute_flat) turned into the
+// engine's directed-pair lists: both directions of every pair, centre
+// positions kept (pcen), owners in ilist order, force gather lists.
+struct LammpsLists {
+    int nall = 0;
+    int64_t npairs = 0;
+    std::vector<double> x;
+    std::vector<int32_t> elem, elem_own, poff, pnbr, pcen, gof, gen;
+};
+int lammps_lists(mlip_ctx* c, int nlocal, int nghost, const double* x, const int* type, const int64_t* tag, int inum,
+                 const int* ilist, const int* numneigh, const int* const* firstneigh, const int* neigh_off,
+                 const int* neigh_idx, int neigh_full, LammpsLists& L);
+// After an evaluation of such lists: adds forces, energy and virial into the
+// caller's arrays and fills eatom / vatom (epv, evat: device scratch of
+// 7 npairs and 7 nall doubles for the pair split, unused in site mode).
+int lammps_results(mlip_ctx* c, int nall, int inum, const int* ilist, double* f, double* eng, double* virial,
+                   double* eatom, double* vatom, double* epv, double* evat);
 void collect_profile(mlip_ctx* c);
 void build_gather(int nall, const std::vector<int32_t>& poff, const std::vector<int32_t>& pnbr,
                   const std::vector<int32_t>* pcen, std::vector<int32_t>& goff, std::vector<int32_t>& gent);

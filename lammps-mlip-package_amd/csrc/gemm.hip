@@ -1,0 +1,130 @@
+// Quadratic form of the polynomial model on the matrix cores (FP64 MFMA).
+//
+// Per centre species t, for the species' slot rows (blocks are grouped by
+// species, and a row holds only the features the species sees):
+//   Hq[slot][a] = sum_b D[slot][b] W_t[b][a]
+// with W_t the symmetric quadratic form of the species' features
+// (reference: the comb2 part of compute_polynomial_model_uniq_products and
+// compute_anlm_polynomial_model_correction, pair_mlip_gtinv.cpp:321-343,
+// 481-521, and compute_polynomial_adjoints, mlipkk.cpp:668-731).
+//
+// Workgroup tile 64 rows x 128 columns, K staged through LDS 16 at a time
+// (the next stage's global loads in flight while the current one computes);
+// four waves of 32 x 64 each, i.e. 2 x 4 blocks of v_mfma_f64_16x16x4f64.
+// Lane layout of that instruction: A[i][k] from lane 16 k + i, B[k][j] from
+// lane 16 k + j, result register r of lane l is D[4 (l / 16) + r][l % 16].
+#include <hip/hip_runtime.h>
+
+#include "device_api.h"
+#include "device_common.h"
+
+namespace mlip {
+
+namespace {
+
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+
+constexpr int kGM = 64;        // rows per workgroup
+constexpr int kGN = 128;       // columns per workgroup
+constexpr int kGK = 16;        // K per LDS stage
+constexpr int kGThreads = 256;
+constexpr int kGPad = 1;       // LDS row padding (doubles)
+
+__global__ __launch_bounds__(kGThreads) void k_gemm_rows(const double* __restrict__ A, int lda,
+                                                         const double* __restrict__ B, int ldb,
+                                                         double* __restrict__ C, int ldc, int K, int N,
+                                                         int mtiles, int ntiles) {
+    __shared__ double As[kGK][kGM + kGPad];  // [k][row]
+    __shared__ double Bs[kGK][kGN + kGPad];  // [k][column]
+    // XCD-aware order: workgroups are dealt round-robin over the 8 XCDs, and
+    // consecutive workgroups of one XCD take the column tiles of one row
+    // tile, so its A rows are fetched into that XCD's L2 once
+    const int g = blockIdx.x;
+    const int xcd = g & 7, k8 = g >> 3;
+    const int mt = xcd + 8 * (k8 / ntiles), nt = k8 % ntiles;
+    if (mt >= mtiles) return;
+    const int m0 = mt * kGM, n0 = nt * kGN;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wr = wave >> 1, wc = wave & 1;
+    // global -> register staging roles
+    const int arow = tid >> 2, ak = (tid & 3) * 4;   // A: 64 rows x 16 k, 4 per thread
+    const int bk = tid >> 4, bc = (tid & 15) * 8;    // B: 16 k x 128 columns, 8 per thread
+    const double* arow_p = A + (size_t)(m0 + arow) * lda;
+    double ra[4], rb[8];
+    auto load = [&](int kb) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int k = kb + ak + q;
+            ra[q] = k < K ? arow_p[k] : 0.0;
+        }
+        const int k = kb + bk;
+        const double* brow = B + (size_t)k * ldb;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int c = n0 + bc + q;
+            rb[q] = (k < K && c < N) ? brow[c] : 0.0;
+        }
+    };
+    dbl4 acc[2][4];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = dbl4{0.0, 0.0, 0.0, 0.0};
+    const int nk = (K + kGK - 1) / kGK;
+    load(0);
+    for (int kt = 0; kt < nk; ++kt) {
+        if (kt) __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 4; ++q) As[ak + q][arow] = ra[q];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) Bs[bk][bc + q] = rb[q];
+        __syncthreads();
+        if (kt + 1 < nk) load((kt + 1) * kGK);
+#pragma unroll
+        for (int kk = 0; kk < kGK / 4; ++kk) {
+            const int ka = kk * 4 + (lane >> 4);
+            double a[2], b[4];
+#pragma unroll
+            for (int i = 0; i < 2; ++i) a[i] = As[ka][wr * 32 + i * 16 + (lane & 15)];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) b[j] = Bs[ka][wc * 64 + j * 16 + (lane & 15)];
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int col = n0 + wc * 64 + j * 16 + (lane & 15);
+            if (col >= N) continue;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = m0 + wr * 32 + i * 16 + 4 * (lane >> 4) + r;
+                C[(size_t)row * ldc + col] = acc[i][j][r];
+            }
+        }
+}
+
+}  // namespace
+
+hipError_t launch_poly_gemm(const DevModel& m, const DevSystem& s, const int* nblk_type, hipStream_t st) {
+    int b0 = 0;
+    for (int t = 0; t < m.n_type && t < 2; ++t) {
+        const int nb = nblk_type[t], w = m.wdim[t];
+        const size_t r0 = static_cast<size_t>(b0) * kBlock;
+        b0 += nb;
+        if (nb == 0 || w == 0) continue;
+        const int mtiles = nb * kBlock / kGM, ntiles = (w + kGN - 1) / kGN;
+        const int grid = 8 * ntiles * ((mtiles + 7) / 8);
+        hipLaunchKernelGGL(k_gemm_rows, dim3(grid), dim3(kGThreads), 0, st, s.D + r0 * m.n_pr, m.n_pr,
+                           m.wmat + m.wofs[t], w, s.Hq + r0 * m.n_pr, m.n_pr, w, w, mtiles, ntiles);
+        const hipError_t e = launched();
+        if (e != hipSuccess) return e;
+    }
+    return b0 == s.nblocks ? hipSuccess : hipErrorInvalidValue;
+}
+
+}  // namespace mlip

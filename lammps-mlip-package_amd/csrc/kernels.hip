@@ -6,7 +6,7 @@
 //   k_q12    order-1/2 invariants: polynomial features or adjoint G
 //   k_q34    order-3/4 invariants: polynomial features or adjoint G,
 //            a_nlm tile of 64 same-species centres resident in LDS
-//   rocBLAS  Hq = D W (quadratic form of the polynomial model)
+//   k_gemm_rows  Hq = D W (quadratic form of the polynomial model, FP64 MFMA; gemm.hip)
 //   k_poly   E_i polynomial part and H = dE_i/dd                 (:321-343,481-521)
 //   k_force  per-pair dE_i/d(del) from G and dphi/d(del)          (:161-256, pair_kernels.hip)
 //   k_gather atomics-free per-position force sums, deterministic
@@ -134,6 +134,60 @@ struct Q12Levels<LMAX + 1, LMAX, FEAT> {
     __device__ static void run(const DevModel&, const DevSystem&, int, int, int, int, double&, double*) {}
 };
 
+// Levels above kMaxLQ12 (order-2 families up to l = 20): the same visits
+// with a and G read and updated in HBM term by term instead of held in
+// registers (the row of 2 (L+1) complex values per species would not fit).
+template <bool FEAT>
+__device__ void q12_level_dyn(const DevModel& M, const DevSystem& S, int blk, int n, int lane, int t, int L,
+                              double& eacc, double* tile) {
+    const int ns = M.n_type;
+    const int slot = blk * kBlock + lane;
+    if (!FEAT)
+        for (int s = 0; s < ns; ++s)
+            for (int k = 0; k <= L; ++k) {
+                S.G[a_index(M, blk, n, s, lmh(L, -k), 0, lane)] = 0.0;
+                S.G[a_index(M, blk, n, s, lmh(L, -k), 1, lane)] = 0.0;
+            }
+    const int vb = M.q12_off[t * (M.maxl + 2) + L], ve = M.q12_off[t * (M.maxl + 2) + L + 1];
+    for (int vi = vb; vi < ve; ++vi) {
+        const VisitQ12 v = M.q12[vi];
+        const int pslot = uniform(v.pslot);
+        if (FEAT && pslot < 0) continue;
+        if (uniform(v.order) != 2) continue;  // order-1 families live at l = 0
+        const int s0 = uniform(v.s0), s1 = uniform(v.s1);
+        const double* cp = M.coef + uniform(v.coef);
+        double H = 0.0;
+        if (!FEAT) H = pslot >= 0 ? S.H[(size_t)slot * M.n_pr + pcol(M, t, n, pslot)] : M.weight[n * M.n_gtinv + v.itc];
+        double d = 0.0;
+        for (int k = 0; k <= L; ++k) {
+            const double sg = (k & 1) ? -1.0 : 1.0;
+            const int lm = lmh(L, -k);
+            const double xr = S.A[a_index(M, blk, n, s0, lm, 0, lane)], xi = S.A[a_index(M, blk, n, s0, lm, 1, lane)];
+            const double yr = S.A[a_index(M, blk, n, s1, lm, 0, lane)], yi = S.A[a_index(M, blk, n, s1, lm, 1, lane)];
+            const double cm = cp[L - k], cpk = cp[L + k];
+            d = fma(k == 0 ? cm : cm + cpk, sg * (xr * yr + xi * yi), d);
+            if (!FEAT) {
+                const double h0 = H * cm * sg, h1 = H * cpk * sg;
+                double* g0 = S.G + a_index(M, blk, n, s0, lm, 0, lane);
+                double* g1 = S.G + a_index(M, blk, n, s1, lm, 0, lane);
+                const size_t im = kBlock;  // imaginary part: next component, same lane
+                g0[0] = fma(h0, yr, g0[0]);
+                g0[im] = fma(-h0, yi, g0[im]);
+                g1[0] = fma(h1, xr, g1[0]);
+                g1[im] = fma(-h1, xi, g1[im]);
+            }
+        }
+        if (FEAT) {
+            if (tile)
+                tile[(pslot - M.prange[t][0]) * kQ12Pitch + lane] = d;
+            else
+                S.D[(size_t)slot * M.n_pr + pcol(M, t, n, pslot)] = d;
+        } else if (pslot < 0) {
+            eacc = fma(M.weight[n * M.n_gtinv + v.itc], d, eacc);
+        }
+    }
+}
+
 // The feature pass (stage = 1) collects the block's row segments of radial
 // index n in LDS, [rank - lo][lane] with a pitch of kQ12Pitch, zeroed first
 // (features no visit writes stay zero), and writes them out row by row in
@@ -159,7 +213,8 @@ __global__ __launch_bounds__(64) void k_q12(DevModel M, DevSystem S, int stage) 
             for (int r = z0; r < z1; ++r) row[M.zlist[r]] = 0.0;
         }
     }
-    Q12Levels<0, kMaxL, FEAT>::run(M, S, blk, n, lane, t, e, tile);
+    Q12Levels<0, kMaxLQ12, FEAT>::run(M, S, blk, n, lane, t, e, tile);
+    for (int L = kMaxLQ12 + 1; L <= M.maxl; ++L) q12_level_dyn<FEAT>(M, S, blk, n, lane, t, L, e, tile);
     if (!FEAT) S.Eq12[(size_t)(blk * kBlock + lane) * M.n_fn + n] = e;
     if (FEAT && tile && w > 0) {
         __syncthreads();
@@ -212,6 +267,52 @@ __global__ __launch_bounds__(256) void k_poly(DevModel M, DevSystem S) {
         for (int k = 0; k < 64; ++k) s += red[threadIdx.x >> 6][k];
         S.Ep[slot] = s;
     }
+}
+
+// Split polynomial adjoint for the per-atom energy (one wave per slot):
+//   He_a = lin_a / q_a + sum_b W_ab d_b / (q_a + q_b) + sum_cubic w d_b d_c / (q_a + q_b + q_c)
+// with q the degree (family order) of each feature: the reference divides
+// every term of dE/da by the term's total degree in a
+// (pair_mlip_gtinv.cpp:308-340, mlip_polynomial_gtinv.cpp:147-166).
+__global__ __launch_bounds__(256) void k_poly_split(DevModel M, DevSystem S) {
+    const int slot = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (slot >= S.nblocks * kBlock) return;
+    const int t = S.block_type[slot / kBlock];
+    const double* d = S.D + (size_t)slot * M.n_pr;
+    double* he = S.Hq + (size_t)slot * M.n_pr;
+    const int w = M.n_fn * (M.prange[t][1] - M.prange[t][0]);
+    const int32_t* deg = M.pdeg + (size_t)t * M.n_pr;
+    const double* W = M.wmat + M.wofs[t];
+    const int32_t* c3o = M.c3_off + t * (M.n_pr + 1);
+    const double* lin = M.lin_p + (size_t)t * M.n_pr;
+    for (int a = lane; a < w; a += 64) {
+        const int qa = deg[a];
+        double h = lin[a] / qa;
+        // W is symmetric: W[b][a] for fixed b is contiguous across lanes
+        for (int b = 0; b < w; ++b) h = fma(W[(size_t)b * w + a], d[b] / (qa + deg[b]), h);
+        for (int k = c3o[a]; k < c3o[a + 1]; ++k) {
+            const int b = M.c3_bc[2 * k], c = M.c3_bc[2 * k + 1];
+            h = fma(M.c3_w[k], d[b] * d[c] / (qa + deg[b] + deg[c]), h);
+        }
+        he[a] = h;
+    }
+}
+
+// Per position k: half the energy share and virial of every pair in its
+// gather list (centre or neighbor side), in list order.
+__global__ __launch_bounds__(256) void k_atom_ev(DevSystem S, const double* __restrict__ epv, double* evat) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= S.nall) return;
+    double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+    for (int e = S.goff[k]; e < S.gend[k]; ++e) {
+        const int code = S.gent[e];
+        const int64_t p = code > 0 ? code - 1 : -code - 1;
+#pragma unroll
+        for (int c = 0; c < 7; ++c) acc[c] += epv[7 * p + c];
+    }
+#pragma unroll
+    for (int c = 0; c < 7; ++c) evat[7 * (size_t)k + c] = 0.5 * acc[c];
 }
 
 // ---------------------------------------------------------------- gather
@@ -332,23 +433,43 @@ __global__ __launch_bounds__(kRedThreads) void k_reduce2(const double* part, int
     }
 }
 
+__global__ __launch_bounds__(256) void k_fill(double* __restrict__ dst, int64_t n) {
+    for (int64_t i = blockIdx.x * 256LL + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) dst[i] = 0.0;
+}
+
 __global__ __launch_bounds__(256) void k_copy(double4* __restrict__ dst, const double4* __restrict__ src, int64_t n) {
     for (int64_t i = blockIdx.x * 256LL + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
         dst[i] = src[i];
-}
-
-void check(hipError_t e, const char* what) {
-    if (e != hipSuccess) {
-        std::fprintf(stderr, "mlip_hip: %s failed: %s\n", what, hipGetErrorString(e));
-    }
 }
 
 }  // namespace
 
 
 
-void launch_q12(const DevModel& m, const DevSystem& s, bool feat, hipStream_t st) {
-    if (s.nblocks == 0) return;
+size_t lds_limit() {
+    static int cached_dev = -1;
+    static size_t cached = 0;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) dev = 0;
+    if (dev != cached_dev) {
+        int a = 0, b = 0;
+        if (hipDeviceGetAttribute(&a, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) a = 0;
+        if (hipDeviceGetAttribute(&b, hipDeviceAttributeSharedMemPerBlockOptin, dev) != hipSuccess) b = 0;
+        cached = static_cast<size_t>(std::max(a, b));
+        if (cached == 0) cached = 160 * 1024;  // CDNA4: 160 KB of LDS per workgroup
+        cached_dev = dev;
+    }
+    return cached;
+}
+
+hipError_t launched(size_t lds_bytes) {
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return lds_bytes > lds_limit() ? hipErrorInvalidConfiguration : hipSuccess;
+}
+
+hipError_t launch_q12(const DevModel& m, const DevSystem& s, bool feat, hipStream_t st) {
+    if (s.nblocks == 0) return hipSuccess;
     if (feat) {
         // staged feature writes when the widest species' tile is small
         int wmax = 0;
@@ -359,50 +480,71 @@ void launch_q12(const DevModel& m, const DevSystem& s, bool feat, hipStream_t st
     } else {
         hipLaunchKernelGGL(k_q12<false>, dim3(s.nblocks, m.n_fn), dim3(64), 0, st, m, s, 0);
     }
-    check(hipGetLastError(), "k_q12 launch");
+    return launched();
 }
 
-void launch_poly(const DevModel& m, const DevSystem& s, hipStream_t st) {
+hipError_t launch_poly(const DevModel& m, const DevSystem& s, hipStream_t st) {
     const int nslot = s.nblocks * kBlock;
-    if (nslot == 0 || m.n_p == 0) return;
+    if (nslot == 0 || m.n_p == 0) return hipSuccess;
     hipLaunchKernelGGL(k_poly, dim3((nslot + 3) / 4), dim3(256), 0, st, m, s);
-    check(hipGetLastError(), "k_poly launch");
+    return launched();
 }
 
-void launch_gather(const DevModel&, const DevSystem& s, hipStream_t st) {
-    if (s.nall == 0) return;
+hipError_t launch_poly_split(const DevModel& m, const DevSystem& s, hipStream_t st) {
+    const int nslot = s.nblocks * kBlock;
+    if (nslot == 0 || m.n_p == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_poly_split, dim3((nslot + 3) / 4), dim3(256), 0, st, m, s);
+    return launched();
+}
+
+hipError_t launch_atom_ev(const DevSystem& s, const double* epv, double* evat, hipStream_t st) {
+    if (s.nall == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_atom_ev, dim3((s.nall + 255) / 256), dim3(256), 0, st, s, epv, evat);
+    return launched();
+}
+
+hipError_t launch_gather(const DevModel&, const DevSystem& s, hipStream_t st) {
+    if (s.nall == 0) return hipSuccess;
     hipLaunchKernelGGL(k_gather, dim3((s.nall + 255) / 256), dim3(256), 0, st, s);
-    check(hipGetLastError(), "k_gather launch");
+    return launched();
 }
 
-void launch_copy(void* dst, const void* src, int64_t nbytes, int reps, hipStream_t st) {
+hipError_t launch_fill(double* dst, size_t n, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    const int64_t blocks = std::min<int64_t>(4096, (static_cast<int64_t>(n) + 255) / 256);
+    hipLaunchKernelGGL(k_fill, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, st, dst, static_cast<int64_t>(n));
+    return launched();
+}
+
+hipError_t launch_copy(void* dst, const void* src, int64_t nbytes, int reps, hipStream_t st) {
     const int64_t n = nbytes / static_cast<int64_t>(sizeof(double4));
     for (int r = 0; r < reps; ++r)
         hipLaunchKernelGGL(k_copy, dim3(2048), dim3(256), 0, st, static_cast<double4*>(dst),
                            static_cast<const double4*>(src), n);
-    check(hipGetLastError(), "k_copy launch");
+    return launched();
 }
 
-void launch_struct_reduce(const DevSystem& s, const int32_t* sbeg, const int32_t* send, int nstruct, double* out,
-                          hipStream_t st) {
-    if (nstruct <= 0) return;
+hipError_t launch_struct_reduce(const DevSystem& s, const int32_t* sbeg, const int32_t* send, int nstruct,
+                                double* out, hipStream_t st) {
+    if (nstruct <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_struct_reduce, dim3(nstruct), dim3(256), 0, st, s, sbeg, send, out);
-    check(hipGetLastError(), "k_struct_reduce launch");
+    return launched();
 }
 
-void launch_energy_virial(const DevModel& m, const DevSystem& s, bool totals, hipStream_t st) {
+hipError_t launch_energy_virial(const DevModel& m, const DevSystem& s, bool totals, hipStream_t st) {
     const int nslot = s.nblocks * kBlock;
     if (nslot > 0) hipLaunchKernelGGL(k_energy_slot, dim3((nslot + 255) / 256), dim3(256), 0, st, m, s);
-    if (!totals) {
-        check(hipGetLastError(), "energy launch");
-        return;
-    }
+    hipError_t e = launched();
+    if (!totals) return e;
     const int parts = 120;
     hipLaunchKernelGGL(k_reduce1, dim3(parts), dim3(kRedThreads), 0, st, s.e_slot, nslot, 1, 1, s.red);
+    e = first_error(e, launched());
     hipLaunchKernelGGL(k_reduce2, dim3(1), dim3(kRedThreads), 0, st, s.red, parts, 1, s.out);
+    e = first_error(e, launched());
     hipLaunchKernelGGL(k_reduce1, dim3(parts), dim3(kRedThreads), 0, st, s.vown, s.nown, 6, 6, s.red + parts);
+    e = first_error(e, launched());
     hipLaunchKernelGGL(k_reduce2, dim3(1), dim3(kRedThreads), 0, st, s.red + parts, parts, 6, s.out + 1);
-    check(hipGetLastError(), "reduction launch");
+    return first_error(e, launched());
 }
 
 }  // namespace mlip

@@ -233,6 +233,48 @@ void build_type_tables(Model& m, int t, const std::vector<std::map<std::vector<i
             by_l[L].push_back(v);
             continue;
         }
+        if (q >= 5) {
+            // orders 5 and 6: one visit per factor position, terms listed
+            for (int p = 0; p < q; ++p) {
+                VisitHi v{};
+                v.S = sp[p];
+                v.L = ir.l[p];
+                v.order = q;
+                v.itc = i;
+                v.pslot = pslot;
+                int no = 0;
+                std::vector<int> oth;
+                for (int k = 0; k < q; ++k)
+                    if (k != p) {
+                        v.ls[no++] = ir.l[k] | (sp[k] << 8);
+                        oth.push_back(k);
+                    }
+                for (int k = no; k < 5; ++k) v.ls[k] = 0;
+                auto emit_terms = [&](bool feat) {
+                    VisitHi w = v;
+                    w.tbeg = static_cast<int32_t>(m.hi_term.size());
+                    for (size_t ti = 0; ti < ir.m.size(); ++ti) {
+                        const std::vector<int>& mt = ir.m[ti];
+                        if (mt[p] > 0) continue;  // outputs m <= 0 only
+                        double c = ir.c[ti];
+                        if (feat && mt[p] < 0) {
+                            std::vector<int> neg(mt);
+                            for (int& x : neg) x = -x;
+                            c += lookup(cm, neg);
+                        }
+                        int32_t code = -mt[p];
+                        for (int k = 0; k < no; ++k) code |= (mt[oth[k]] + 8) << (4 + 4 * k);
+                        m.hi_term.push_back(code);
+                        m.hi_coef.push_back(c);
+                    }
+                    w.tend = static_cast<int32_t>(m.hi_term.size());
+                    return w;
+                };
+                tt.hi_adj.push_back(emit_terms(false));
+                if (p == 0 && pslot >= 0) tt.hi_feat.push_back(emit_terms(true));
+            }
+            continue;
+        }
         // orders 3 and 4: one visit per factor position
         for (int p = 0; p < q; ++p) {
             std::vector<int> oth;
@@ -724,7 +766,7 @@ bool load_model(const std::string& path, const GtinvTable& table, Model& m, std:
         return false;
     }
     if (!(m.cutoff > 0.0) || m.maxl < 0 || m.maxl > kMaxL) {
-        err = "cutoff must be positive and max_l in [0,12]";
+        err = "cutoff must be positive and max_l in [0," + std::to_string(kMaxL) + "]";
         return false;
     }
     if (!first_token(in, m.gtinv_order) || m.gtinv_order < 1 || m.gtinv_order > 6) {
@@ -746,7 +788,7 @@ bool load_model(const std::string& path, const GtinvTable& table, Model& m, std:
     }
     int npar = 0;
     if (!first_token(in, npar) || npar <= 0 || npar > kMaxFn) {
-        err = "number of radial functions must be in [1,16]";
+        err = "number of radial functions must be in [1," + std::to_string(kMaxFn) + "]";
         return false;
     }
     m.beta.resize(npar);
@@ -807,11 +849,21 @@ bool load_model(const std::string& path, const GtinvTable& table, Model& m, std:
         }
         m.weight[i] = m.coeff_file[i] / m.scale_file[i];
     }
-    for (const Irrep& ir : m.irreps)
-        if (static_cast<int>(ir.l.size()) > kMaxOrderGpu) {
-            err = "invariants of order > 4 are not supported by the HIP path";
+    for (const Irrep& ir : m.irreps) {
+        const int q = static_cast<int>(ir.l.size());
+        if (q > kMaxOrderGpu) {
+            err = "invariants of order > 6 are not supported by the HIP path";
             return false;
         }
+        if (q >= 5 && ir.l.back() > kMaxLHi) {
+            err = "order-5/6 invariants with l > " + std::to_string(kMaxLHi) + " are not supported by the HIP path";
+            return false;
+        }
+        if ((q == 3 || q == 4) && ir.l.back() > 15) {
+            err = "order-3/4 invariants with l > 15 are not supported by the HIP path";
+            return false;
+        }
+    }
     // polynomial features: per itc (all n or none)
     m.prank.assign(m.n_gtinv, -1);
     {
@@ -875,6 +927,12 @@ bool load_model(const std::string& path, const GtinvTable& table, Model& m, std:
             m.lmax3 = std::max(m.lmax3, ir.l.back());
         }
     m.n_lmh3 = n_lmh_of(m.lmax3);
+    m.lmaxh = -1;
+    for (const Irrep& ir : m.irreps)
+        if (ir.l.size() >= 5) m.lmaxh = std::max(m.lmaxh, ir.l.back());
+    m.has_hi = m.lmaxh >= 0;
+    m.hi_term.clear();
+    m.hi_coef.clear();
     std::vector<std::map<std::vector<int>, double>> cmap(m.irreps.size());
     for (size_t i = 0; i < m.irreps.size(); ++i)
         for (size_t t = 0; t < m.irreps[i].m.size(); ++t) cmap[i][m.irreps[i].m[t]] = m.irreps[i].c[t];

@@ -17,9 +17,10 @@
 
 namespace mlip {
 
-constexpr int kMaxFn = 16;      // radial functions supported by the kernels
-constexpr int kMaxL = 12;       // max angular momentum of a_nlm
-constexpr int kMaxOrderGpu = 4; // highest invariant order of the HIP path
+constexpr int kMaxFn = 32;      // radial functions supported by the kernels
+constexpr int kMaxL = 20;       // max angular momentum of a_nlm (the gtinv table's largest l)
+constexpr int kMaxOrderGpu = 6; // highest invariant order of the HIP path (the table's largest)
+constexpr int kMaxLQ12 = 12;    // order-1/2 levels up to this l run with the row in registers
 constexpr int kWavesPerWg = 12; // waves sharing one a_nlm tile in the order-3/4 kernel
 constexpr int kBlockLanes = 64; // centres per slot block (the lanes of one wave)
 constexpr int kUnrollCap3 = 8;  // order-3 visits with all l <= 8 run fully unrolled (q34_signatures.inc)
@@ -97,6 +98,21 @@ struct GroupQ34 {
     int32_t red;       // partial-row offset of this sub-group (split rows only)
 };
 
+// Order-5/6 visit (the table's order-5 and order-6 invariants, all factors
+// l <= 1): one (feature family, factor position) with its terms listed
+// explicitly.  Term codes hold k (output m = -k) in bits 0-3 and m + 8 of
+// each other factor in 4 bits from bit 4; coefficients are parallel doubles.
+// Adjoint visits add H C prod(other factors) into G_{S,L,-k}; feature visits
+// (position 0 of a polynomial family) form d = sum C' Re(a_{S,L,-k} prod)
+// with C' the coefficient summed with that of the negated m-tuple for k > 0.
+constexpr int kMaxLHi = 4;      // largest l of an order-5/6 factor the kernel holds
+struct VisitHi {
+    int32_t S, L;
+    int32_t order, itc, pslot;
+    int32_t ls[5];              // other factors: l | species << 8
+    int32_t tbeg, tend;         // term range
+};
+
 // Radial-only model (des_type pair): one entry of the per-feature
 // derivative lists, H_a = w_a + sum_entries w prod(d_b, d_c) (b, c = -1 for
 // absent factors); ew = w / order is the Euler energy share of the term.
@@ -114,6 +130,8 @@ struct TypeTables {
     // order 3/4: adjoint pass and polynomial-feature pass
     std::vector<GroupQ34> groups_adj, groups_feat;
     std::vector<VisitQ34> visits_adj, visits_feat;
+    // orders 5/6: adjoint and polynomial-feature visits
+    std::vector<VisitHi> hi_adj, hi_feat;
 };
 
 struct Model {
@@ -156,6 +174,11 @@ struct Model {
     std::vector<int32_t> ph_off;                   // n_des + 1
     std::vector<PairHEntry> ph_ent;
     int q34_red = 0;           // doubles of LDS partial rows (adjoint pass), max over types
+    // orders 5/6: term codes and coefficients of every VisitHi, tile l_max
+    std::vector<int32_t> hi_term;
+    std::vector<double> hi_coef;
+    int lmaxh = -1;            // largest l of an order-5/6 factor (-1: none)
+    bool has_hi = false;
     bool q34_dyn_adj = false;  // visits outside the unrolled signatures (general loop kernel)
     bool q34_dyn_feat = false;
 };

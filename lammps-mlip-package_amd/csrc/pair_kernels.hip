@@ -79,34 +79,36 @@ __device__ __forceinline__ PairDel pair_del(const DevModel& M, const DevSystem& 
 }
 
 // ---------------------------------------------------------------- a_nlm
-// One workgroup of kAnlmThreads per centre.  Neighbors are taken in blocks
-// of kAnlmThreads (geometry once per neighbor into LDS), then in chunks of
-// kAnlmJ: phase 1 has kAnlmParts threads per neighbor evaluate g_n
-// (n = part, part + kAnlmParts) and the Y_lm columns m = part, part +
-// kAnlmParts into LDS; phase 2 has thread (lm, group) accumulate
-// a_{s,n,lm} for its NPT radial indices n = group*NPT + k and both species
-// in FP64 registers.
-constexpr int kAnlmThreads = 192;
+// One workgroup of NTH threads per centre (192, or 256 when l_max > 18).
+// Neighbors are taken in blocks of NTH (geometry once per neighbor into
+// LDS), then in chunks of kAnlmJ: phase 1 has NTH / kAnlmJ threads per
+// neighbor evaluate g_n and the Y_lm columns m = part, part + NTH / kAnlmJ,
+// ... into LDS; phase 2 has thread (lm, group) accumulate a_{s,n,lm} for its
+// NPT radial indices n = n0 + group*NPT + k and both species in FP64
+// registers, n0 = blockIdx.y * ngroup * NPT (models with more radial
+// functions than one pass holds run several passes over the neighbors).
 constexpr int kAnlmJ = 16;
-constexpr int kAnlmParts = kAnlmThreads / kAnlmJ;  // 12
+constexpr int kAnlmGW = kMaxFn + 16;  // g_n row width: every pass's NPT-wide read stays in the row
 static_assert(kAnlmJ <= 64, "a chunk's neighbor species fit one per lane of a wave");
 
-template <typename T>
+template <typename T, int NTH>
 struct AnlmLds {
     T diag[kL1 + 1];
     T alm[kL1][kL1];
     T ainv[kL1][kL1];
     double beta[kMaxFn], mu[kMaxFn];
-    T ux[kAnlmThreads], uy[kAnlmThreads], uz[kAnlmThreads];
-    double rr[kAnlmThreads], fc[kAnlmThreads];
-    int spec[kAnlmThreads];
-    alignas(16) double g[kAnlmJ][kMaxFn];
+    T ux[NTH], uy[NTH], uz[NTH];
+    double rr[NTH], fc[NTH];
+    int spec[NTH];
+    alignas(16) double g[kAnlmJ][kAnlmGW];
 };
 
-template <int NPT, typename T>
-__global__ __launch_bounds__(kAnlmThreads) void k_anlm(DevModel M, DevSystem S, int ngroup) {
+template <int NPT, int NTH, typename T>
+__global__ __launch_bounds__(NTH) void k_anlm(DevModel M, DevSystem S, int ngroup) {
+    constexpr int kAnlmThreads = NTH;
+    constexpr int kAnlmParts = NTH / kAnlmJ;
     using T2 = typename Vec2<T>::type;
-    __shared__ AnlmLds<T> L;
+    __shared__ AnlmLds<T, NTH> L;
     extern __shared__ unsigned char anlm_dyn[];
     T2* ys = reinterpret_cast<T2*>(anlm_dyn);  // [kAnlmJ][n_lmh]
     const int slot = xcd_slot(blockIdx.x, S.nblocks);
@@ -131,7 +133,7 @@ __global__ __launch_bounds__(kAnlmThreads) void k_anlm(DevModel M, DevSystem S, 
     // phase-2 role
     const int lm2 = tid % nlmh, grp = tid / nlmh;
     const bool act2 = grp < ngroup;
-    const int nbase = grp * NPT;
+    const int nbase = (blockIdx.y * ngroup + grp) * NPT;
     double a0r[NPT], a0i[NPT], a1r[NPT], a1i[NPT];
 #pragma unroll
     for (int k = 0; k < NPT; ++k) a0r[k] = a0i[k] = a1r[k] = a1i[k] = 0.0;
@@ -162,8 +164,8 @@ __global__ __launch_bounds__(kAnlmThreads) void k_anlm(DevModel M, DevSystem S, 
                 // radial functions in FP64 in both modes: FP32 cannot carry
                 // the exponent beta (r - mu)^2 to the digits exp needs
                 const double r = L.rr[q], fcq = L.fc[q];
-                for (int n = part; n < kMaxFn; n += kAnlmParts) {
-                    const double d = r - L.mu[n];
+                for (int n = part; n < kAnlmGW; n += kAnlmParts) {
+                    const double d = n < nfn ? r - L.mu[n] : 0.0;
                     L.g[j1][n] = n < nfn ? exp(-L.beta[n] * (d * d)) * fcq : 0.0;
                 }
                 const T x = L.ux[q], y = L.uy[q], z = L.uz[q];
@@ -275,9 +277,13 @@ struct ForceLds {
 // unconstrained it takes 214 and runs 2 waves), FP32 at most 128 (4 waves).
 // The spills this costs are cheaper than the lost latency hiding: 28.0 ->
 // 19.6 ms (FP64) and 14.2 -> 12.9 ms (FP32) at 256k atoms with 192 threads,
-// where the workgroups' G tiles still fit the LDS.
-template <int NF, int NT, typename T>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 3))) void k_force(DevModel M, DevSystem S) {
+// where the workgroups' G tiles still fit the LDS.  Models whose G tile does
+// not fit (l_max and n_fn near their limits) read G from HBM instead (GL =
+// false; every thread of the workgroup reads the same entries, served by the
+// caches), and models with more than 16 radial functions keep the register
+// arrays they need (no occupancy cap).
+template <int NF, int NT, typename T, bool GL>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NF > 16 ? 1 : (sizeof(T) == 4 ? 4 : 3)))) void k_force(DevModel M, DevSystem S) {
     extern __shared__ __attribute__((aligned(16))) unsigned char force_dyn[];
     T* Gs = reinterpret_cast<T*>(force_dyn);  // [ns][n_lmh][NF][2]
     __shared__ ForceLds<T> C;
@@ -288,7 +294,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(T) ==
     if (atom < 0) return;
     const int nfn = M.n_fn, ns = M.n_type, nlmh = M.n_lmh, maxl = M.maxl;
     const int blk = slot / kBlock, lane = slot % kBlock;
-    for (int k = threadIdx.x; k < ns * nlmh * NF; k += NT) {
+    for (int k = threadIdx.x; GL && k < ns * nlmh * NF; k += NT) {
         const int n = k % NF;
         const int lm = (k / NF) % nlmh;
         const int s = k / (NF * nlmh);
@@ -376,7 +382,13 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(T) ==
                     // 16-byte LDS reads (one n in FP64, two in FP32)
                     const T* gl = Gsp + (size_t)lm * NF * 2;
                     T grv[2 * NF];
-                    if constexpr (sizeof(T) == 8) {
+                    if constexpr (!GL) {
+#pragma unroll
+                        for (int n = 0; n < NF; ++n) {
+                            grv[2 * n] = n < nfn ? static_cast<T>(S.G[a_index(M, blk, n, spec, lm, 0, lane)]) : T(0);
+                            grv[2 * n + 1] = n < nfn ? static_cast<T>(S.G[a_index(M, blk, n, spec, lm, 1, lane)]) : T(0);
+                        }
+                    } else if constexpr (sizeof(T) == 8) {
                         const double2* g2 = reinterpret_cast<const double2*>(gl);
 #pragma unroll
                         for (int n = 0; n < NF; ++n) {
@@ -496,69 +508,167 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(T) ==
     }
 }
 
-void check(hipError_t e, const char* what) {
-    if (e != hipSuccess) std::fprintf(stderr, "mlip_hip: %s failed: %s\n", what, hipGetErrorString(e));
+// ---------------------------------------------------------------- per-pair energy share
+// Thread per directed pair of the slot's centre (64 threads, FP64):
+//   e_p = sum_{n,l,m<=0} w_m Re(G_{s,n,l,m} g_n(r) Y_lm(del)),  w_0 = 1, w_{m<0} = 2
+// with G read from HBM (every lane of the workgroup reads the same entry),
+// plus the pair's virial del (x) (-g_p) from the force pass.  With G holding
+// the split adjoint, e_p + e_p' over the two directions of a pair is the
+// reference's pair energy evdwl (pair_mlip_gtinv.cpp:240-253).
+__global__ __launch_bounds__(64) void k_pair_energy(DevModel M, DevSystem S, double* __restrict__ epv) {
+    const int slot = xcd_slot(blockIdx.x, S.nblocks);
+    if (slot < 0) return;
+    const int atom = S.atom_of_slot[slot];
+    if (atom < 0) return;
+    const int blk = slot / kBlock, lane = slot % kBlock;
+    const int nfn = M.n_fn, maxl = M.maxl;
+    const YTab<double>* yt = M.yt64;
+    const double pir = kPiRef / M.cutoff;
+    const int p0 = S.poff[atom], p1 = S.pend[atom];
+    for (int64_t p = p0 + threadIdx.x; p < p1; p += 64) {
+        const PairDel pg = pair_del(M, S, atom, p);
+        double e = 0.0;
+        double v[6] = {0, 0, 0, 0, 0, 0};
+        if (pg.valid) {
+            const int spec = S.elem[S.pnbr[p]];
+            const double ir = 1.0 / pg.r;
+            const double x = pg.dx * ir, y = pg.dy * ir, z = pg.dz * ir;
+            const double fc = 0.5 * (cos(pir * pg.r) + 1.0);
+            double gn[kMaxFn];
+            for (int n = 0; n < kMaxFn; ++n) {
+                const double dd = pg.r - M.mu[n];
+                gn[n] = n < nfn ? exp(-M.beta[n] * (dd * dd)) * fc : 0.0;
+            }
+            double cr = 1.0, ci = 0.0;  // conj(u)^m
+            for (int m = 0; m <= maxl; ++m) {
+                double q1 = 0.0, q2 = 0.0, sr = 0.0, si = 0.0;
+                int lm = m * (m + 1) / 2;  // lmh(m, -m)
+                for (int l = m; l <= maxl; ++l) {
+                    const double q = (l == m) ? yt->diag[m] : yt->alm[l][m] * (z * q1 - yt->ainv[l][m] * q2);
+                    q2 = q1;
+                    q1 = q;
+                    double gr = 0.0, gi = 0.0;
+                    for (int n = 0; n < nfn; ++n) {
+                        gr = fma(gn[n], S.G[a_index(M, blk, n, spec, lm, 0, lane)], gr);
+                        gi = fma(gn[n], S.G[a_index(M, blk, n, spec, lm, 1, lane)], gi);
+                    }
+                    sr = fma(q, gr, sr);
+                    si = fma(q, gi, si);
+                    lm += l + 2;  // lmh(l+1, -m)
+                }
+                const double ws = (m == 0) ? 1.0 : ((m & 1) ? -2.0 : 2.0);  // w_m (-1)^m
+                e += ws * (sr * cr - si * ci);
+                const double nr = cr * x + ci * y, ni = ci * x - cr * y;
+                cr = nr;
+                ci = ni;
+            }
+            const double gx = S.gpair[3 * p], gy = S.gpair[3 * p + 1], gz = S.gpair[3 * p + 2];
+            v[0] = -pg.dx * gx;
+            v[1] = -pg.dy * gy;
+            v[2] = -pg.dz * gz;
+            v[3] = -pg.dx * gy;
+            v[4] = -pg.dx * gz;
+            v[5] = -pg.dy * gz;
+        }
+        epv[7 * p] = e;
+#pragma unroll
+        for (int c = 0; c < 6; ++c) epv[7 * p + 1 + c] = v[c];
+    }
 }
 
 int slot_grid(const DevSystem& s) { return 8 * kBlock * ((s.nblocks + 7) / 8); }
 
-// radial indices per phase-2 thread of the a_nlm kernel
-int anlm_npt(const DevModel& m, int* ngroup) {
-    const int gmax = kAnlmThreads / m.n_lmh;
+// radial indices per phase-2 thread of the a_nlm kernel, lm groups per
+// pass and passes over the neighbors (grid.y)
+int anlm_npt(const DevModel& m, int nth, int* ngroup, int* passes) {
+    const int gmax = nth / m.n_lmh;
     for (int npt : {2, 4, 6, 8, 16}) {
         const int g = (m.n_fn + npt - 1) / npt;
         if (g <= gmax) {
             *ngroup = g;
+            *passes = 1;
             return npt;
         }
     }
-    *ngroup = 1;
+    *ngroup = gmax;
+    *passes = (m.n_fn + 16 * gmax - 1) / (16 * gmax);
     return 16;
 }
 
-template <typename T>
-void launch_anlm_t(const DevModel& m, const DevSystem& s, hipStream_t st) {
-    int ngroup = 1;
-    const int npt = anlm_npt(m, &ngroup);
+template <int NTH, typename T>
+hipError_t launch_anlm_nth(const DevModel& m, const DevSystem& s, hipStream_t st) {
+    int ngroup = 1, passes = 1;
+    const int npt = anlm_npt(m, NTH, &ngroup, &passes);
     const size_t lds = sizeof(typename Vec2<T>::type) * kAnlmJ * m.n_lmh;
-    const dim3 grid(slot_grid(s)), block(kAnlmThreads);
+    if (lds + sizeof(AnlmLds<T, NTH>) > lds_limit()) return hipErrorInvalidConfiguration;
+    const dim3 grid(slot_grid(s), passes), block(NTH);
     switch (npt) {
-        case 2: hipLaunchKernelGGL((k_anlm<2, T>), grid, block, lds, st, m, s, ngroup); break;
-        case 4: hipLaunchKernelGGL((k_anlm<4, T>), grid, block, lds, st, m, s, ngroup); break;
-        case 6: hipLaunchKernelGGL((k_anlm<6, T>), grid, block, lds, st, m, s, ngroup); break;
-        case 8: hipLaunchKernelGGL((k_anlm<8, T>), grid, block, lds, st, m, s, ngroup); break;
-        default: hipLaunchKernelGGL((k_anlm<16, T>), grid, block, lds, st, m, s, ngroup); break;
+        case 2: hipLaunchKernelGGL((k_anlm<2, NTH, T>), grid, block, lds, st, m, s, ngroup); break;
+        case 4: hipLaunchKernelGGL((k_anlm<4, NTH, T>), grid, block, lds, st, m, s, ngroup); break;
+        case 6: hipLaunchKernelGGL((k_anlm<6, NTH, T>), grid, block, lds, st, m, s, ngroup); break;
+        case 8: hipLaunchKernelGGL((k_anlm<8, NTH, T>), grid, block, lds, st, m, s, ngroup); break;
+        default: hipLaunchKernelGGL((k_anlm<16, NTH, T>), grid, block, lds, st, m, s, ngroup); break;
     }
+    return launched();
 }
 
-template <int NF, int NT, typename T>
-void launch_force_k(const DevModel& m, const DevSystem& s, hipStream_t st) {
-    const size_t lds = sizeof(T) * 2 * m.n_type * m.n_lmh * NF;
-    check(hipFuncSetAttribute(reinterpret_cast<const void*>(k_force<NF, NT, T>),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)),
-          "k_force lds attr");
-    hipLaunchKernelGGL((k_force<NF, NT, T>), dim3(slot_grid(s)), dim3(NT), lds, st, m, s);
+template <typename T>
+hipError_t launch_anlm_t(const DevModel& m, const DevSystem& s, hipStream_t st) {
+    // phase 2 needs a thread per lm column: 192 threads up to l_max 18
+    return m.n_lmh <= 192 ? launch_anlm_nth<192, T>(m, s, st) : launch_anlm_nth<256, T>(m, s, st);
+}
+
+template <int NF, int NT, typename T, bool GL>
+hipError_t launch_force_k(const DevModel& m, const DevSystem& s, hipStream_t st) {
+    const size_t lds = GL ? sizeof(T) * 2 * m.n_type * m.n_lmh * NF : 0;
+    if (lds + sizeof(ForceLds<T>) + sizeof(double) * 6 * (NT / 64) > lds_limit()) return hipErrorInvalidConfiguration;
+    if (GL) {
+        const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k_force<NF, NT, T, GL>),
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL((k_force<NF, NT, T, GL>), dim3(slot_grid(s)), dim3(NT), lds, st, m, s);
+    return launched();
+}
+
+// G tile staged in LDS when it fits beside the kernel's static LDS
+template <typename T>
+bool force_tile_fits(const DevModel& m, int nf, int nt) {
+    const size_t lds = sizeof(T) * 2 * m.n_type * m.n_lmh * nf;
+    return lds + sizeof(ForceLds<T>) + sizeof(double) * 6 * (nt / 64) <= lds_limit();
 }
 
 template <int NT, typename T>
-void launch_force_nt(const DevModel& m, const DevSystem& s, hipStream_t st) {
-    if (m.n_fn <= 4)
-        launch_force_k<4, NT, T>(m, s, st);
-    else if (m.n_fn <= 8)
-        launch_force_k<8, NT, T>(m, s, st);
-    else if (m.n_fn <= 12)
-        launch_force_k<12, NT, T>(m, s, st);
-    else
-        launch_force_k<16, NT, T>(m, s, st);
+hipError_t launch_force_nt(const DevModel& m, const DevSystem& s, hipStream_t st) {
+    const int nf = m.n_fn <= 4 ? 4 : m.n_fn <= 8 ? 8 : m.n_fn <= 12 ? 12 : m.n_fn <= 16 ? 16 : m.n_fn <= 24 ? 24 : 32;
+    if (nf <= 16 && force_tile_fits<T>(m, nf, NT)) {
+        if (nf == 4) return launch_force_k<4, NT, T, true>(m, s, st);
+        if (nf == 8) return launch_force_k<8, NT, T, true>(m, s, st);
+        if (nf == 12) return launch_force_k<12, NT, T, true>(m, s, st);
+        return launch_force_k<16, NT, T, true>(m, s, st);
+    }
+    // large models: one launch shape
+    if (force_tile_fits<T>(m, nf, 128)) {
+        if (nf == 24) return launch_force_k<24, 128, T, true>(m, s, st);
+        if (nf == 32) return launch_force_k<32, 128, T, true>(m, s, st);
+    }
+    switch (nf) {
+        case 4: return launch_force_k<4, 128, T, false>(m, s, st);
+        case 8: return launch_force_k<8, 128, T, false>(m, s, st);
+        case 12: return launch_force_k<12, 128, T, false>(m, s, st);
+        case 16: return launch_force_k<16, 128, T, false>(m, s, st);
+        case 24: return launch_force_k<24, 128, T, false>(m, s, st);
+        default: return launch_force_k<32, 128, T, false>(m, s, st);
+    }
 }
 
 template <typename T>
-void launch_force_t(const DevModel& m, const DevSystem& s, int threads, hipStream_t st) {
+hipError_t launch_force_t(const DevModel& m, const DevSystem& s, int threads, hipStream_t st) {
     switch (threads) {
-        case 64: launch_force_nt<64, T>(m, s, st); break;
-        case 192: launch_force_nt<192, T>(m, s, st); break;
-        case 256: launch_force_nt<256, T>(m, s, st); break;
-        default: launch_force_nt<128, T>(m, s, st); break;
+        case 64: return launch_force_nt<64, T>(m, s, st);
+        case 192: return launch_force_nt<192, T>(m, s, st);
+        case 256: return launch_force_nt<256, T>(m, s, st);
+        default: return launch_force_nt<128, T>(m, s, st);
     }
 }
 
@@ -595,22 +705,20 @@ void build_ylm_tables(YTab<double>& t64, YTab<float>& t32) {
         }
 }
 
-void launch_anlm(const DevModel& m, const DevSystem& s, int precision, hipStream_t st) {
-    if (s.nblocks == 0) return;
-    if (precision == kMixed)
-        launch_anlm_t<float>(m, s, st);
-    else
-        launch_anlm_t<double>(m, s, st);
-    check(hipGetLastError(), "k_anlm launch");
+hipError_t launch_anlm(const DevModel& m, const DevSystem& s, int precision, hipStream_t st) {
+    if (s.nblocks == 0) return hipSuccess;
+    return precision == kMixed ? launch_anlm_t<float>(m, s, st) : launch_anlm_t<double>(m, s, st);
 }
 
-void launch_force(const DevModel& m, const DevSystem& s, int precision, int threads, hipStream_t st) {
-    if (s.nblocks == 0) return;
-    if (precision == kMixed)
-        launch_force_t<float>(m, s, threads, st);
-    else
-        launch_force_t<double>(m, s, threads, st);
-    check(hipGetLastError(), "k_force launch");
+hipError_t launch_pair_energy(const DevModel& m, const DevSystem& s, double* epv, hipStream_t st) {
+    if (s.nblocks == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_pair_energy, dim3(slot_grid(s)), dim3(64), 0, st, m, s, epv);
+    return launched();
+}
+
+hipError_t launch_force(const DevModel& m, const DevSystem& s, int precision, int threads, hipStream_t st) {
+    if (s.nblocks == 0) return hipSuccess;
+    return precision == kMixed ? launch_force_t<float>(m, s, threads, st) : launch_force_t<double>(m, s, threads, st);
 }
 
 }  // namespace mlip

@@ -99,6 +99,9 @@ __global__ __launch_bounds__(kPairThreads) void k_pair_poly(DevModel M, DevSyste
             he = fma(q.ew, v, he);
         }
         h[a] = w[a] + hs;
+        // the same derivative with each term weighted by 1 / its degree: the
+        // reference's per-pair energy split (pair_mlip_pair.cpp:159-162)
+        S.Hq[static_cast<size_t>(slot) * M.n_p + a] = w[a] + he;
         e = fma(d[a], w[a] + he, e);
     }
     S.Ep[slot] = e;
@@ -152,15 +155,55 @@ __global__ __launch_bounds__(kPairThreads) void k_pair_force(DevModel M, DevSyst
     vo[5] = v5;
 }
 
-void check(hipError_t e, const char* what) {
-    if (e != hipSuccess) std::fprintf(stderr, "mlip_hip: %s failed: %s\n", what, hipGetErrorString(e));
+// Per directed pair: energy share e_p = sum_n g_n(r) He_{tc,n} (He: the
+// split derivative k_pair_poly leaves in Hq) and the pair's virial
+// del (x) (-g_p); the reference's half-pair evdwl is e_p + e_p' over the two
+// directions (pair_mlip_pair.cpp:196-212).
+__global__ __launch_bounds__(kPairThreads) void k_pair_model_energy(DevModel M, DevSystem S, double* epv) {
+    const int slot = blockIdx.x * kPairThreads + threadIdx.x;
+    if (slot >= S.nblocks * kBlock) return;
+    const int atom = S.atom_of_slot[slot];
+    if (atom < 0) return;
+    const int t = S.elem[atom], nfn = M.n_fn;
+    const double* he = S.Hq + static_cast<size_t>(slot) * M.n_p;
+    const double pir = kPiRef / M.cutoff;
+    for (int64_t p = S.poff[atom]; p < S.pend[atom]; ++p) {
+        double dx, dy, dz, r;
+        double e = 0.0, v[6] = {0, 0, 0, 0, 0, 0};
+        if (pair_geom(M, S, atom, p, dx, dy, dz, r)) {
+            const int s = S.elem[S.pnbr[p]];
+            const double* hr = he + M.pair_tc[t][s] * nfn;
+            const double fc = 0.5 * (cos(pir * r) + 1.0);
+            for (int n = 0; n < nfn; ++n) {
+                const double d = r - M.mu[n];
+                e = fma(exp(-M.beta[n] * d * d) * fc, hr[n], e);
+            }
+            const double gx = S.gpair[3 * p], gy = S.gpair[3 * p + 1], gz = S.gpair[3 * p + 2];
+            v[0] = -dx * gx;
+            v[1] = -dy * gy;
+            v[2] = -dz * gz;
+            v[3] = -dx * gy;
+            v[4] = -dx * gz;
+            v[5] = -dy * gz;
+        }
+        epv[7 * p] = e;
+        for (int c = 0; c < 6; ++c) epv[7 * p + 1 + c] = v[c];
+    }
 }
 
 }  // namespace
 
-void launch_pair_features(const DevModel& m, const DevSystem& s, hipStream_t st) {
+hipError_t launch_pair_model_energy(const DevModel& m, const DevSystem& s, double* epv, hipStream_t st) {
     const int nslot = s.nblocks * kBlock;
-    if (nslot == 0) return;
+    if (nslot == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_pair_model_energy, dim3((nslot + kPairThreads - 1) / kPairThreads), dim3(kPairThreads), 0,
+                       st, m, s, epv);
+    return launched();
+}
+
+hipError_t launch_pair_features(const DevModel& m, const DevSystem& s, hipStream_t st) {
+    const int nslot = s.nblocks * kBlock;
+    if (nslot == 0) return hipSuccess;
     const dim3 grid((nslot + kPairThreads - 1) / kPairThreads), block(kPairThreads);
     if (m.n_fn <= 8)
         hipLaunchKernelGGL(k_pair_feat<8>, grid, block, 0, st, m, s);
@@ -168,22 +211,22 @@ void launch_pair_features(const DevModel& m, const DevSystem& s, hipStream_t st)
         hipLaunchKernelGGL(k_pair_feat<16>, grid, block, 0, st, m, s);
     else
         hipLaunchKernelGGL(k_pair_feat<32>, grid, block, 0, st, m, s);
-    check(hipGetLastError(), "k_pair_feat launch");
+    return launched();
 }
 
-void launch_pair_poly(const DevModel& m, const DevSystem& s, hipStream_t st) {
+hipError_t launch_pair_poly(const DevModel& m, const DevSystem& s, hipStream_t st) {
     const int nslot = s.nblocks * kBlock;
-    if (nslot == 0) return;
+    if (nslot == 0) return hipSuccess;
     hipLaunchKernelGGL(k_pair_poly, dim3((nslot + kPairThreads - 1) / kPairThreads), dim3(kPairThreads), 0, st, m, s);
-    check(hipGetLastError(), "k_pair_poly launch");
+    return launched();
 }
 
-void launch_pair_force(const DevModel& m, const DevSystem& s, hipStream_t st) {
+hipError_t launch_pair_force(const DevModel& m, const DevSystem& s, hipStream_t st) {
     const int nslot = s.nblocks * kBlock;
-    if (nslot == 0) return;
+    if (nslot == 0) return hipSuccess;
     hipLaunchKernelGGL(k_pair_force, dim3((nslot + kPairThreads - 1) / kPairThreads), dim3(kPairThreads), 0, st, m,
                        s);
-    check(hipGetLastError(), "k_pair_force launch");
+    return launched();
 }
 
 }  // namespace mlip

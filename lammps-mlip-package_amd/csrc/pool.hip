@@ -15,6 +15,15 @@
 // metal units.  Admission is by capacity: a request that does not fit the
 // pool's atom or pair budget is refused with MLIP_ERR_FULL and can be
 // offered again after completions free space.
+//
+// NVE requests need a skin: their list holds pairs within cutoff + skin of
+// the positions it was built from (kept on the device as the request's
+// reference positions).  At the end of every tick the device computes, per
+// continuing request, the largest distance any atom will have moved from
+// those reference positions after the next step's drift (the drift is known
+// exactly from v and F at that point); when it exceeds skin / 2 the host
+// rebuilds that request's list from the current positions before the next
+// tick, so no evaluation ever runs on a list that can miss a pair.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -85,6 +94,34 @@ __global__ __launch_bounds__(256) void k_ke(const int32_t* sbeg, const int32_t* 
         if (threadIdx.x == 0) out[6 * r + c] = kMvv2e * sh[0];
         __syncthreads();
     }
+}
+
+// Largest displacement from the list's reference positions after the next
+// drift x + dt (v + dtf F / m), per batch request (n = 0: no next step)
+__global__ __launch_bounds__(256) void k_disp_next(const NveDesc* d, const int32_t* elem, const double* mass,
+                                                   const double* f, const double* v, const double* x,
+                                                   const double* xref, double* out) {
+    __shared__ double sh[256];
+    const NveDesc q = d[blockIdx.x];
+    double m2 = 0.0;
+    for (int i = threadIdx.x; i < q.n; i += 256) {
+        const int a = q.a0 + i;
+        const double dtfm = q.dtf / mass[elem[a]];
+        double d2 = 0.0;
+        for (int c = 0; c < 3; ++c) {
+            const double xn = x[3 * a + c] + q.dtv * (v[3 * a + c] + dtfm * f[3 * a + c]);
+            const double dd = xn - xref[3 * a + c];
+            d2 += dd * dd;
+        }
+        m2 = fmax(m2, d2);
+    }
+    sh[threadIdx.x] = m2;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[blockIdx.x] = sqrt(sh[0]);
 }
 
 __global__ void k_scatter_slots(const int32_t* atom_of_slot, int nslot, int32_t* slot_of_atom) {
@@ -176,9 +213,12 @@ struct Request {
     int natoms = 0;
     int64_t a0 = 0, p0 = 0, np = 0;
     int nsteps = 0, step = 0;
-    double dt = 0.0;
+    double dt = 0.0, skin = 0.0;
+    double cell[9] = {};
+    int rebuilds = 0;
     std::vector<int32_t> elem;
     bool done = false;
+    std::string error;  // set when the request failed (done with no result)
     // per evaluation: energy, virial[6], kinetic-energy tensor[6]
     std::vector<double> thermo;
     std::vector<double> f, x, v;
@@ -194,14 +234,16 @@ struct Pool {
     RangeAlloc atoms, pairs;
     DBuf<double> x, v, f, pshift, A, G, D, Hq, H, Ep, Eq12, Eq34, gpair, vown, e_slot, red, out, sout, ke, mass;
     DBuf<int32_t> elem, pbeg, pend, pnbr, gbeg, gend, gent, atom_of_slot, slot_of_atom, block_type, sbeg, send;
-    DBuf<NveDesc> first, final_;
+    DBuf<NveDesc> first, final_, disp;
+    DBuf<double> xref, dnext;  // list reference positions; next-step displacement per batch request
+    int64_t rebuilds = 0;      // neighbor-list rebuilds of NVE requests so far
     std::map<int, Request> req;
     int next_id = 1;
     std::vector<int> done_ids;
     // the tick in flight: requests in batch order and host staging
     bool in_flight = false;
     std::vector<int> batch;
-    double* h_out = nullptr;  // pinned: 7 per request, then 6 kinetic per request
+    double* h_out = nullptr;  // pinned: 7 per request, 6 kinetic per request, 1 next-step displacement
     // packed uploads: two pinned arenas used in turn, one device copy
     HostArena arena[2];
     int cur = 0;
@@ -224,6 +266,9 @@ void pool_free(mlip_ctx* c) {
     for (auto* b : ibs) b->release();
     p->first.release();
     p->final_.release();
+    p->disp.release();
+    p->xref.release();
+    p->dnext.release();
     p->dstage.release();
     p->ddesc.release();
     for (auto& a : p->arena) {
@@ -381,9 +426,9 @@ int mlip_pool_init(mlip_ctx* c, int64_t max_atoms, int64_t max_pairs, int max_re
                     p->Eq34.ensure(ns * m.n_fn * kWavesPerWg) && p->e_slot.ensure(ns) &&
                     p->atom_of_slot.ensure(ns) && p->block_type.ensure(ns / kBlock + 1) && p->red.ensure(7 * 128) &&
                     p->out.ensure(8) && p->sout.ensure(7 * nr) && p->ke.ensure(6 * nr) && p->sbeg.ensure(nr) &&
-                    p->send.ensure(nr) && p->first.ensure(nr) && p->final_.ensure(nr) &&
-                    p->mass.ensure(static_cast<size_t>(m.n_type));
-    if (!ok || hipHostMalloc(reinterpret_cast<void**>(&p->h_out), 13 * nr * sizeof(double)) != hipSuccess) {
+                    p->send.ensure(nr) && p->first.ensure(nr) && p->final_.ensure(nr) && p->disp.ensure(nr) &&
+                    p->dnext.ensure(nr) && p->xref.ensure(3 * na) && p->mass.ensure(static_cast<size_t>(m.n_type));
+    if (!ok || hipHostMalloc(reinterpret_cast<void**>(&p->h_out), 14 * nr * sizeof(double)) != hipSuccess) {
         c->err = "device allocation failed for the request pool";
         pool_free(c);
         return MLIP_ERR_DEVICE;
@@ -423,7 +468,7 @@ struct Prepared {
     NeighborList nl;
     std::vector<int32_t> loff, goff, gent;
     int64_t a0 = -1, p0 = -1;        // ranges, once placed
-    size_t off[10] = {};             // arena offsets of the ten pieces
+    size_t off[11] = {};             // arena offsets of the eleven pieces
 };
 
 void prepare(const mlip_ctx* c, int natoms, const double* cell, const double* x, const int* type, double skin,
@@ -467,7 +512,8 @@ void parallel_for(int count, int nthreads, F&& f) {
 // pieces are written into the pinned arena by worker threads and one packed
 // copy moves them.  Returns the number admitted, or -MLIP_ERR_*.
 int commit_many(mlip_ctx* c, Pool* p, std::vector<Prepared>& pr, int count, const int* natoms, const int64_t* at,
-                const double* x, const double* v, const int* nsteps, double dt, int nthreads, int* handles) {
+                const double* cells, const double* x, const double* v, const int* nsteps, double dt,
+                const double* skins, int nthreads, int* handles) {
     int n = 0;
     for (; n < count; ++n) {
         Prepared& q = pr[n];
@@ -491,7 +537,7 @@ int commit_many(mlip_ctx* c, Pool* p, std::vector<Prepared>& pr, int count, cons
         c->err = "request pool: capacity exhausted";
         return 0;
     }
-    // arena layout: ten 16-byte-aligned pieces per request
+    // arena layout: eleven 16-byte-aligned pieces per request
     if (!stage_begin(c, p)) return -MLIP_ERR_DEVICE;
     HostArena& ar = p->arena[p->cur];
     size_t fill = 0;
@@ -503,9 +549,10 @@ int commit_many(mlip_ctx* c, Pool* p, std::vector<Prepared>& pr, int count, cons
     for (int k = 0; k < n; ++k) {
         Prepared& q = pr[k];
         const size_t na = static_cast<size_t>(natoms[k]), np = q.nl.idx.size();
-        const size_t bytes[10] = {24 * na, 24 * na, 4 * na, 4 * na, 4 * na, 4 * na, 4 * na, 4 * np, 24 * np, 8 * np};
+        const size_t bytes[11] = {24 * na, 24 * na, 4 * na, 4 * na, 4 * na, 4 * na, 4 * na, 4 * np, 24 * np, 8 * np,
+                                  24 * na};
         double* dx = p->x.p + 3 * q.a0;
-        void* dst[10] = {dx,
+        void* dst[11] = {dx,
                          p->v.p + 3 * q.a0,
                          p->elem.p + q.a0,
                          p->pbeg.p + q.a0,
@@ -514,8 +561,9 @@ int commit_many(mlip_ctx* c, Pool* p, std::vector<Prepared>& pr, int count, cons
                          p->gend.p + q.a0,
                          p->pnbr.p + q.p0,
                          p->pshift.p + 3 * q.p0,
-                         p->gent.p + 2 * q.p0};
-        for (int j = 0; j < 10; ++j) {
+                         p->gent.p + 2 * q.p0,
+                         p->xref.p + 3 * q.a0};
+        for (int j = 0; j < 11; ++j) {
             q.off[j] = place(bytes[j]);
             if (bytes[j]) p->descs.push_back({q.off[j], reinterpret_cast<uint64_t>(dst[j]), bytes[j] / 4});
         }
@@ -527,6 +575,7 @@ int commit_many(mlip_ctx* c, Pool* p, std::vector<Prepared>& pr, int count, cons
         const int64_t np = static_cast<int64_t>(q.nl.idx.size());
         unsigned char* h = ar.h;
         std::memcpy(h + q.off[0], x + 3 * at[k], 24 * static_cast<size_t>(na));
+        std::memcpy(h + q.off[10], x + 3 * at[k], 24 * static_cast<size_t>(na));
         if (v)
             std::memcpy(h + q.off[1], v + 3 * at[k], 24 * static_cast<size_t>(na));
         else
@@ -560,6 +609,8 @@ int commit_many(mlip_ctx* c, Pool* p, std::vector<Prepared>& pr, int count, cons
         r.natoms = natoms[k];
         r.nsteps = nsteps[k];
         r.dt = dt;
+        r.skin = skins[k];
+        std::copy(cells + 9 * k, cells + 9 * k + 9, r.cell);
         r.a0 = q.a0;
         r.p0 = q.p0;
         r.np = static_cast<int64_t>(q.nl.idx.size());
@@ -575,6 +626,101 @@ int default_threads() {
     return static_cast<int>(std::min(16u, std::max(1u, std::thread::hardware_concurrency())));
 }
 
+// Frees a request's device ranges (its atoms own no pairs or gather entries
+// any more).
+bool release_request(mlip_ctx* c, Pool* p, Request& r) {
+    const size_t a0 = static_cast<size_t>(r.a0), n = static_cast<size_t>(r.natoms);
+    if (!hip_ok(c, hipMemsetAsync(p->pbeg.p + a0, 0, n * 4, c->stream), "pool release") ||
+        !hip_ok(c, hipMemsetAsync(p->pend.p + a0, 0, n * 4, c->stream), "pool release") ||
+        !hip_ok(c, hipMemsetAsync(p->gbeg.p + a0, 0, n * 4, c->stream), "pool release") ||
+        !hip_ok(c, hipMemsetAsync(p->gend.p + a0, 0, n * 4, c->stream), "pool release"))
+        return false;
+    p->atoms.give(r.a0, r.natoms);
+    p->pairs.give(r.p0, std::max<int64_t>(r.np, 1));
+    return true;
+}
+
+// New neighbor lists (cutoff + skin, from the current positions) for NVE
+// requests whose atoms would pass skin / 2 in their next step; runs between
+// ticks (the stream is idle).  A request whose new list does not fit the
+// pair capacity fails with an error (reported by mlip_pool_result).
+int rebuild_lists(mlip_ctx* c, Pool* p, const std::vector<int>& ids, int* ndone) {
+    struct Job {
+        Request* r;
+        std::vector<double> x;
+        NeighborList nl;
+        std::vector<int32_t> loff, goff, gent;
+        std::string err;
+        bool ok = true;
+    };
+    std::vector<Job> jobs(ids.size());
+    for (size_t k = 0; k < ids.size(); ++k) {
+        Request& r = p->req[ids[k]];
+        jobs[k].r = &r;
+        jobs[k].x.resize(3 * static_cast<size_t>(r.natoms));
+        if (!hip_ok(c, hipMemcpy(jobs[k].x.data(), p->x.p + 3 * r.a0, jobs[k].x.size() * 8, hipMemcpyDeviceToHost),
+                    "pool positions for a list rebuild"))
+            return MLIP_ERR_DEVICE;
+    }
+    const double rc = c->model.cutoff;
+    parallel_for(static_cast<int>(jobs.size()), default_threads(), [&](int k) {
+        Job& j = jobs[k];
+        j.ok = build_full_list(j.r->cell, j.r->natoms, j.x.data(), rc + j.r->skin, j.nl, j.err, 1);
+        if (!j.ok) return;
+        j.loff.resize(j.r->natoms + 1);
+        for (int i = 0; i <= j.r->natoms; ++i) j.loff[i] = static_cast<int32_t>(j.nl.off[i]);
+        build_gather(j.r->natoms, j.loff, j.nl.idx, nullptr, j.goff, j.gent);
+    });
+    if (!stage_begin(c, p)) return MLIP_ERR_DEVICE;
+    for (Job& j : jobs) {
+        Request& r = *j.r;
+        const int64_t np = j.ok ? static_cast<int64_t>(j.nl.idx.size()) : 0;
+        p->pairs.give(r.p0, std::max<int64_t>(r.np, 1));
+        const int64_t p0 = j.ok ? p->pairs.take(std::max<int64_t>(np, 1)) : -1;
+        if (p0 < 0) {
+            // no list for the request: it ends here with an error
+            r.error = j.ok ? "request pool: no pair capacity for an NVE neighbor-list rebuild"
+                           : "NVE neighbor-list rebuild: " + j.err;
+            r.np = 0;
+            r.p0 = p->pairs.take(1);
+            r.done = true;
+            if (!release_request(c, p, r)) return MLIP_ERR_DEVICE;
+            p->done_ids.push_back(r.id);
+            ++*ndone;
+            continue;
+        }
+        r.p0 = p0;
+        r.np = np;
+        ++r.rebuilds;
+        ++p->rebuilds;
+        const int na = r.natoms;
+        std::vector<int32_t> pb(na), pe(na), gb(na), ge(na), nb(static_cast<size_t>(np)), gt(j.gent.size());
+        const int64_t g0 = 2 * p0;
+        for (int i = 0; i < na; ++i) {
+            pb[i] = static_cast<int32_t>(p0 + j.loff[i]);
+            pe[i] = static_cast<int32_t>(p0 + j.loff[i + 1]);
+            gb[i] = static_cast<int32_t>(g0 + j.goff[i]);
+            ge[i] = static_cast<int32_t>(g0 + j.goff[i + 1]);
+        }
+        for (int64_t e = 0; e < np; ++e) nb[e] = static_cast<int32_t>(r.a0 + j.nl.idx[e]);
+        for (size_t e = 0; e < j.gent.size(); ++e) {
+            const int32_t en = j.gent[e];
+            gt[e] = en > 0 ? static_cast<int32_t>(en + p0) : static_cast<int32_t>(en - p0);
+        }
+        const size_t a3 = 3 * static_cast<size_t>(r.a0);
+        if (!stage_upload(c, p, p->pbeg.p + r.a0, pb.data(), pb.size(), "rebuild") ||
+            !stage_upload(c, p, p->pend.p + r.a0, pe.data(), pe.size(), "rebuild") ||
+            !stage_upload(c, p, p->gbeg.p + r.a0, gb.data(), gb.size(), "rebuild") ||
+            !stage_upload(c, p, p->gend.p + r.a0, ge.data(), ge.size(), "rebuild") ||
+            !stage_upload(c, p, p->pnbr.p + p0, nb.data(), nb.size(), "rebuild") ||
+            !stage_upload(c, p, p->pshift.p + 3 * p0, j.nl.shift.data(), j.nl.shift.size(), "rebuild") ||
+            !stage_upload(c, p, p->gent.p + 2 * p0, gt.data(), gt.size(), "rebuild") ||
+            !stage_upload(c, p, p->xref.p + a3, j.x.data(), j.x.size(), "rebuild"))
+            return MLIP_ERR_DEVICE;
+    }
+    return stage_flush(c, p) ? MLIP_OK : MLIP_ERR_DEVICE;
+}
+
 }  // namespace
 }  // namespace mlip
 
@@ -585,10 +731,14 @@ int mlip_pool_admit(mlip_ctx* c, int natoms, const double* cell, const double* x
     if (!c || !c->pool || natoms < 1 || !cell || !x || !type || !handle || nsteps < 0 || skin < 0.0 ||
         (nsteps > 0 && !(dt > 0.0)))
         return MLIP_ERR_ARG;
+    if (nsteps > 0 && !(skin > 0.0)) {
+        c->err = "NVE requests need a neighbor skin > 0 (the list is rebuilt when an atom moves skin / 2)";
+        return MLIP_ERR_ARG;
+    }
     std::vector<Prepared> pr(1);
     prepare(c, natoms, cell, x, type, skin, pr[0], default_threads());
     const int64_t at[2] = {0, natoms};
-    const int k = commit_many(c, c->pool, pr, 1, &natoms, at, x, v, &nsteps, dt, 1, handle);
+    const int k = commit_many(c, c->pool, pr, 1, &natoms, at, cell, x, v, &nsteps, dt, &skin, 1, handle);
     if (k < 0) return -k;
     return k == 1 ? MLIP_OK : MLIP_ERR_FULL;
 }
@@ -603,6 +753,10 @@ int mlip_pool_admit_many(mlip_ctx* c, int count, const int* natoms, const double
     std::vector<int64_t> at(count + 1, 0);
     for (int k = 0; k < count; ++k) {
         if (natoms[k] < 1 || nsteps[k] < 0 || skin[k] < 0.0 || (nsteps[k] > 0 && !(dt > 0.0))) return -MLIP_ERR_ARG;
+        if (nsteps[k] > 0 && !(skin[k] > 0.0)) {
+            c->err = "NVE requests need a neighbor skin > 0 (the list is rebuilt when an atom moves skin / 2)";
+            return -MLIP_ERR_ARG;
+        }
         at[k + 1] = at[k] + natoms[k];
     }
     const int nt = nthreads > 0 ? nthreads : default_threads();
@@ -612,7 +766,7 @@ int mlip_pool_admit_many(mlip_ctx* c, int count, const int* natoms, const double
     parallel_for(count, std::min(nt, count), [&](int k) {
         prepare(c, natoms[k], cells + 9 * k, x + 3 * at[k], type + at[k], skin[k], pr[k], inner);
     });
-    return commit_many(c, c->pool, pr, count, natoms, at.data(), x, v, nsteps, dt, nt, handles);
+    return commit_many(c, c->pool, pr, count, natoms, at.data(), cells, x, v, nsteps, dt, skin, nt, handles);
 }
 
 int mlip_pool_tick(mlip_ctx* c, int* nbatch) {
@@ -631,7 +785,7 @@ int mlip_pool_tick(mlip_ctx* c, int* nbatch) {
     const int nb = static_cast<int>(p->batch.size());
     // slot blocks by species over the batch, request ranges, NVE steps
     std::vector<int32_t> aos, btype, sbeg(nb), send(nb);
-    std::vector<NveDesc> first, final_;
+    std::vector<NveDesc> first, final_, disp(nb);
     for (int t = 0; t < m.n_type; ++t) {
         size_t k = 0;
         for (int id : p->batch) {
@@ -652,11 +806,14 @@ int mlip_pool_tick(mlip_ctx* c, int* nbatch) {
         const Request& r = p->req[p->batch[b]];
         sbeg[b] = static_cast<int32_t>(r.a0);
         send[b] = static_cast<int32_t>(r.a0 + r.natoms);
+        const NveDesc d{static_cast<int32_t>(r.a0), r.natoms, 0.5 * r.dt * kFtm2v, r.dt};
         if (r.nsteps > 0 && r.step > 0) {
-            const NveDesc d{static_cast<int32_t>(r.a0), r.natoms, 0.5 * r.dt * kFtm2v, r.dt};
             first.push_back(d);
             final_.push_back(d);
         }
+        // a step follows this tick: check the displacement it will reach
+        disp[b] = d;
+        if (!(r.nsteps > 0 && r.step < r.nsteps)) disp[b].n = 0;
     }
     const int nslot = static_cast<int>(aos.size());
     if (nslot > p->cap_slots) {
@@ -671,7 +828,8 @@ int mlip_pool_tick(mlip_ctx* c, int* nbatch) {
               stage_upload(c, p, p->sbeg.p, sbeg.data(), sbeg.size(), "pool request ranges") &&
               stage_upload(c, p, p->send.p, send.data(), send.size(), "pool request ranges") &&
               stage_upload(c, p, p->first.p, first.data(), first.size(), "pool nve") &&
-              stage_upload(c, p, p->final_.p, final_.data(), final_.size(), "pool nve") && stage_flush(c, p);
+              stage_upload(c, p, p->final_.p, final_.data(), final_.size(), "pool nve") &&
+              stage_upload(c, p, p->disp.p, disp.data(), disp.size(), "pool displacement") && stage_flush(c, p);
     if (!ok) return MLIP_ERR_DEVICE;
     hipStream_t st = c->stream;
     if (!first.empty())
@@ -679,6 +837,7 @@ int mlip_pool_tick(mlip_ctx* c, int* nbatch) {
                            st, p->first.p, p->elem.p, p->mass.p, p->f.p, p->v.p, p->x.p);
     hipLaunchKernelGGL(k_scatter_slots, dim3((nslot + 255) / 256), dim3(256), 0, st, p->atom_of_slot.p, nslot,
                        p->slot_of_atom.p);
+    if (!hip_ok(c, launched(), "tick launch")) return MLIP_ERR_DEVICE;
     c->nblk_type[0] = c->nblk_type[1] = 0;
     for (int32_t t : btype) ++c->nblk_type[t];
     bind_pool(c, p, static_cast<int>(btype.size()));
@@ -699,12 +858,18 @@ int mlip_pool_tick(mlip_ctx* c, int* nbatch) {
     if (!final_.empty())
         hipLaunchKernelGGL(k_nve_final, dim3((maxn + 255) / 256, static_cast<unsigned>(final_.size())), dim3(256), 0,
                            st, p->final_.p, p->elem.p, p->mass.p, p->f.p, p->v.p);
+    if (!hip_ok(c, launched(), "nve launch")) return MLIP_ERR_DEVICE;
     hipLaunchKernelGGL(k_ke, dim3(nb), dim3(256), 0, st, p->sbeg.p, p->send.p, p->elem.p, p->mass.p, p->v.p, p->ke.p);
-    if (!hip_ok(c, hipGetLastError(), "pool kernels") ||
+    if (!hip_ok(c, launched(), "kinetic launch")) return MLIP_ERR_DEVICE;
+    hipLaunchKernelGGL(k_disp_next, dim3(nb), dim3(256), 0, st, p->disp.p, p->elem.p, p->mass.p, p->f.p, p->v.p,
+                       p->x.p, p->xref.p, p->dnext.p);
+    if (!hip_ok(c, launched(), "displacement launch") ||
         !hip_ok(c, hipMemcpyAsync(p->h_out, p->sout.p, 7 * nb * sizeof(double), hipMemcpyDeviceToHost, st),
                 "pool results") ||
         !hip_ok(c, hipMemcpyAsync(p->h_out + 7 * nb, p->ke.p, 6 * nb * sizeof(double), hipMemcpyDeviceToHost, st),
-                "pool kinetic"))
+                "pool kinetic") ||
+        !hip_ok(c, hipMemcpyAsync(p->h_out + 13 * nb, p->dnext.p, nb * sizeof(double), hipMemcpyDeviceToHost, st),
+                "pool displacement"))
         return MLIP_ERR_DEVICE;
     p->in_flight = true;
     return MLIP_OK;
@@ -720,6 +885,7 @@ int mlip_pool_sync(mlip_ctx* c, int* ncompleted) {
     if (c->prof) collect_profile(c);
     const int nb = static_cast<int>(p->batch.size());
     int ndone = 0;
+    std::vector<int> rebuild;
     for (int b = 0; b < nb; ++b) {
         Request& r = p->req[p->batch[b]];
         for (int k = 0; k < 7; ++k) r.thermo.push_back(p->h_out[7 * b + k]);
@@ -727,6 +893,7 @@ int mlip_pool_sync(mlip_ctx* c, int* ncompleted) {
         const bool last = r.nsteps == 0 || r.step == r.nsteps;
         if (!last) {
             ++r.step;
+            if (p->h_out[13 * nb + b] > 0.5 * r.skin) rebuild.push_back(r.id);
             continue;
         }
         const size_t n3 = 3 * static_cast<size_t>(r.natoms);
@@ -739,21 +906,19 @@ int mlip_pool_sync(mlip_ctx* c, int* ncompleted) {
             !hip_ok(c, hipMemcpy(r.v.data(), p->v.p + a3, n3 * 8, hipMemcpyDeviceToHost), "pool velocities"))
             return MLIP_ERR_DEVICE;
         r.done = true;
-        // the freed atoms own no pairs or gather entries any more
-        const size_t a0 = static_cast<size_t>(r.a0), n = static_cast<size_t>(r.natoms);
-        if (!hip_ok(c, hipMemsetAsync(p->pbeg.p + a0, 0, n * 4, c->stream), "pool release") ||
-            !hip_ok(c, hipMemsetAsync(p->pend.p + a0, 0, n * 4, c->stream), "pool release") ||
-            !hip_ok(c, hipMemsetAsync(p->gbeg.p + a0, 0, n * 4, c->stream), "pool release") ||
-            !hip_ok(c, hipMemsetAsync(p->gend.p + a0, 0, n * 4, c->stream), "pool release"))
-            return MLIP_ERR_DEVICE;
-        p->atoms.give(r.a0, r.natoms);
-        p->pairs.give(r.p0, std::max<int64_t>(r.np, 1));
+        if (!release_request(c, p, r)) return MLIP_ERR_DEVICE;
         p->done_ids.push_back(r.id);
         ++ndone;
+    }
+    if (!rebuild.empty()) {
+        const int rc = rebuild_lists(c, p, rebuild, &ndone);
+        if (rc != MLIP_OK) return rc;
     }
     if (ncompleted) *ncompleted = ndone;
     return MLIP_OK;
 }
+
+int64_t mlip_pool_rebuilds(const mlip_ctx* c) { return c && c->pool ? c->pool->rebuilds : -1; }
 
 int mlip_pool_done(mlip_ctx* c, int* ids, int cap) {
     if (!c || !c->pool || (!ids && cap > 0)) return -MLIP_ERR_ARG;
@@ -774,6 +939,11 @@ int mlip_pool_result(mlip_ctx* c, int id, double* energy, double* virial, double
         return -MLIP_ERR_STATE;
     }
     const Request& r = it->second;
+    if (!r.error.empty()) {
+        c->err = r.error;
+        p->req.erase(it);
+        return -MLIP_ERR_FULL;
+    }
     const size_t nev = r.thermo.size() / kThermo;
     const double* lastv = r.thermo.data() + (nev - 1) * kThermo;
     if (energy) *energy = lastv[0];

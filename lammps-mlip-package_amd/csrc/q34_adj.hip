@@ -3,10 +3,11 @@
 
 namespace mlip {
 
-void launch_q34_adj(const DevModel& m, const DevSystem& s, hipStream_t st) {
-    if (s.nblocks == 0) return;
-    q34::launch<false, false>(m, s, st);
-    if (m.q34_dyn_adj) launch_q34_dyn(m, s, false, st);
+hipError_t launch_q34_adj(const DevModel& m, const DevSystem& s, hipStream_t st) {
+    if (s.nblocks == 0) return hipSuccess;
+    hipError_t e = q34::launch<false, false>(m, s, st);
+    if (e == hipSuccess && m.q34_dyn_adj) e = launch_q34_dyn(m, s, false, st);
+    return e;
 }
 
 }  // namespace mlip

@@ -4,12 +4,9 @@
 
 namespace mlip {
 
-void launch_q34_dyn(const DevModel& m, const DevSystem& s, bool feat, hipStream_t st) {
-    if (s.nblocks == 0) return;
-    if (feat)
-        q34::launch<true, true>(m, s, st);
-    else
-        q34::launch<false, true>(m, s, st);
+hipError_t launch_q34_dyn(const DevModel& m, const DevSystem& s, bool feat, hipStream_t st) {
+    if (s.nblocks == 0) return hipSuccess;
+    return feat ? q34::launch<true, true>(m, s, st) : q34::launch<false, true>(m, s, st);
 }
 
 }  // namespace mlip

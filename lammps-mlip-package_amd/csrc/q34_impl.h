@@ -521,15 +521,14 @@ __global__ __launch_bounds__(kQ34Threads) void k_q34(DevModel M, DevSystem S) {
 }
 
 template <bool FEAT, bool DYN>
-void launch(const DevModel& m, const DevSystem& s, hipStream_t st) {
+hipError_t launch(const DevModel& m, const DevSystem& s, hipStream_t st) {
     const size_t lds = sizeof(double) * (static_cast<size_t>(m.n_type) * m.n_lmh3 * 2 * kBlock + (FEAT ? 0 : m.q34_red));
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k_q34<FEAT, DYN>),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
-    if (e == hipSuccess) {
-        hipLaunchKernelGGL((k_q34<FEAT, DYN>), dim3(s.nblocks, m.n_fn), dim3(kQ34Threads), lds, st, m, s);
-        e = hipGetLastError();
-    }
-    if (e != hipSuccess) std::fprintf(stderr, "mlip_hip: k_q34 launch failed: %s\n", hipGetErrorString(e));
+    if (lds > lds_limit()) return hipErrorInvalidConfiguration;
+    const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k_q34<FEAT, DYN>),
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_q34<FEAT, DYN>), dim3(s.nblocks, m.n_fn), dim3(kQ34Threads), lds, st, m, s);
+    return launched();
 }
 
 }  // namespace q34

@@ -74,8 +74,10 @@ using namespace dev;
 
 constexpr int kTrThreads = 256;
 constexpr int kTrPairs = 8;     // pairs per chunk of k_tr_dirj
+constexpr int kTrMaxFn = 16;    // radial functions of the training path (kTrPairs * kTrMaxFn threads fill g_n)
 constexpr int kTrFeatY = 16;    // features per workgroup of k_tr_feat
-constexpr int kTrL1 = kMaxL + 1;
+constexpr int kTrMaxL = 12;     // angular momenta of the training path (kTrPairs * (kTrMaxL + 1) <= 128 threads)
+constexpr int kTrL1 = kTrMaxL + 1;
 
 struct TrArgs {
     const TrItc* itc;
@@ -160,7 +162,7 @@ struct TrLds {
     double yt_grad[kTrL1][kTrL1];
     double ux[kTrPairs], uy[kTrPairs], uz[kTrPairs], rr[kTrPairs];
     int spec[kTrPairs], valid[kTrPairs];
-    double gn[kTrPairs][kMaxFn], gd[kTrPairs][kMaxFn];
+    double gn[kTrPairs][kTrMaxFn], gd[kTrPairs][kTrMaxFn];
 };
 
 // dynamic LDS: a of the centre [n][s][lm] (re, im), then per chunk pair
@@ -185,9 +187,10 @@ __global__ __launch_bounds__(kTrThreads) void k_tr_dirj(DevModel M, DevSystem S,
     {
         const YTab<double>* yt = M.yt64;
         for (int k = tid; k < kTrL1 * kTrL1; k += kTrThreads) {
-            (&C.yt_alm[0][0])[k] = (&yt->alm[0][0])[k];
-            (&C.yt_ainv[0][0])[k] = (&yt->ainv[0][0])[k];
-            (&C.yt_grad[0][0])[k] = (&yt->grad[0][0])[k];
+            const int l = k / kTrL1, mm = k % kTrL1;
+            C.yt_alm[l][mm] = yt->alm[l][mm];
+            C.yt_ainv[l][mm] = yt->ainv[l][mm];
+            C.yt_grad[l][mm] = yt->grad[l][mm];
         }
         if (tid <= kTrL1) C.yt_diag[tid] = yt->diag[tid];
     }
@@ -288,8 +291,8 @@ __global__ __launch_bounds__(kTrThreads) void k_tr_dirj(DevModel M, DevSystem S,
                     o[7] = sg * gzi * ir;
                 }
             }
-        } else if (tid >= 128 && tid < 128 + kTrPairs * kMaxFn) {
-            const int j = (tid - 128) / kMaxFn, n = (tid - 128) % kMaxFn;
+        } else if (tid >= 128 && tid < 128 + kTrPairs * kTrMaxFn) {
+            const int j = (tid - 128) / kTrMaxFn, n = (tid - 128) % kTrMaxFn;
             if (n < nfn && C.valid[j]) {
                 const double r = C.rr[j];
                 double sa, ca;
@@ -514,7 +517,13 @@ int train_features(mlip_ctx* c, double* X, double* P, double* FF, double* SF) {
         c->err = "training features are defined for the gtinv descriptor";
         return MLIP_ERR_MODEL;
     }
-    if (m.n_type > 2 || m.n_fn > kMaxFn) {
+    static_assert(128 + kTrPairs * kTrMaxFn <= kTrThreads, "g_n threads of k_tr_dirj");
+    static_assert(kTrPairs * kTrL1 <= 128, "Y_lm threads of k_tr_dirj");
+    if (m.maxl > kTrMaxL) {
+        c->err = "training features support max_l up to 12";
+        return MLIP_ERR_MODEL;
+    }
+    if (m.n_type > 2 || m.n_fn > kTrMaxFn) {
         c->err = "training features support up to two species and 16 radial functions";
         return MLIP_ERR_MODEL;
     }
@@ -563,30 +572,44 @@ int train_features(mlip_ctx* c, double* X, double* P, double* FF, double* SF) {
     }
     TrArgs a{w.itc.p, w.term.p, w.rows.p, m.n_des, m.n_coeff, m.n_gtinv, w.X.p, w.P.p, w.FF.p, w.SF.p, w.dirj.p,
              w.pcen.p};
-    launch_anlm(c->dm, s, kFP64, st);
-    if (s.nblocks > 0)
+    if (!hip_ok(c, launch_anlm(c->dm, s, kFP64, st), "a_nlm launch")) return MLIP_ERR_DEVICE;
+    if (s.nblocks > 0) {
         hipLaunchKernelGGL(k_tr_feat, dim3(s.nblocks, (m.n_des + kTrFeatY - 1) / kTrFeatY), dim3(64), 0, st, c->dm,
                            s, a);
+        if (!hip_ok(c, launched(), "training feature launch")) return MLIP_ERR_DEVICE;
+    }
     const int64_t npw = static_cast<int64_t>(n) * m.n_coeff;
-    if (npw > 0)
+    if (npw > 0) {
         hipLaunchKernelGGL(k_tr_poly, dim3(static_cast<unsigned>((npw + kTrThreads - 1) / kTrThreads)),
                            dim3(kTrThreads), 0, st, n, a);
+        if (!hip_ok(c, launched(), "polynomial feature launch")) return MLIP_ERR_DEVICE;
+    }
     if (n > 0 && np > 0 && (FF || SF)) {
         hipLaunchKernelGGL(k_tr_pcen, dim3((n + kTrThreads - 1) / kTrThreads), dim3(kTrThreads), 0, st, s, w.pcen.p);
+        if (!hip_ok(c, launched(), "pair centre launch")) return MLIP_ERR_DEVICE;
         const size_t lds = sizeof(double) * (2 * static_cast<size_t>(m.n_fn) * m.n_type * m.n_lmh +
                                              static_cast<size_t>(kTrPairs) * m.n_lmh * 8);
+        if (lds > lds_limit()) {
+            c->err = "training features: the model's per-centre LDS tile exceeds the device limit";
+            return MLIP_ERR_MODEL;
+        }
         if (!hip_ok(c, hipFuncSetAttribute(reinterpret_cast<const void*>(k_tr_dirj),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)),
                     "training LDS"))
             return MLIP_ERR_DEVICE;
         hipLaunchKernelGGL(k_tr_dirj, dim3(n), dim3(kTrThreads), lds, st, c->dm, s, a);
+        if (!hip_ok(c, launched(), "feature derivative launch")) return MLIP_ERR_DEVICE;
         const int pchunks = (m.n_coeff + kTrThreads - 1) / kTrThreads;
-        if (FF)
+        if (FF) {
             hipLaunchKernelGGL(k_tr_force, dim3(static_cast<unsigned>(static_cast<int64_t>(n) * pchunks)),
                                dim3(kTrThreads), 0, st, s, a, pchunks);
-        if (SF)
+            if (!hip_ok(c, launched(), "force feature launch")) return MLIP_ERR_DEVICE;
+        }
+        if (SF) {
             hipLaunchKernelGGL(k_tr_stress, dim3(static_cast<unsigned>(nstruct * m.n_coeff)), dim3(kTrThreads), 0, st,
                                c->dm, s, a, sb, se);
+            if (!hip_ok(c, launched(), "stress feature launch")) return MLIP_ERR_DEVICE;
+        }
     } else {
         if (FF && !hip_ok(c, hipMemsetAsync(w.FF.p, 0, fbytes, st), "force features")) return MLIP_ERR_DEVICE;
         if (SF && !hip_ok(c, hipMemsetAsync(w.SF.p, 0, sizeof(double) * nstruct * m.n_coeff * 6, st),

@@ -21,6 +21,11 @@ PairMLIPGtinvHIP::PairMLIPGtinvHIP(LAMMPS *lmp) : Pair(lmp) {
     restartinfo = 0;
     one_coeff = 1;
     manybody_flag = 1;
+    // the engine returns the pair virial itself: without this flag ev_init
+    // would take the F.r route (vflag_fdotr) and clear vflag_global, and
+    // the virial would be lost (the Kokkos style sets it too,
+    // pair_mlip_gtinv_kokkos_impl.h:278)
+    no_virial_fdotr_compute = 1;
 }
 
 PairMLIPGtinvHIP::~PairMLIPGtinvHIP() {
@@ -81,10 +86,15 @@ double PairMLIPGtinvHIP::init_one(int i, int j) {
 void PairMLIPGtinvHIP::compute(int eflag, int vflag) {
     ev_init(eflag, vflag);
     double eng = 0.0, vir[6] = {0, 0, 0, 0, 0, 0};
-    const int rc = mlip_compute(ctx, atom->nlocal, atom->nghost, &atom->x[0][0], atom->type,
-                                reinterpret_cast<const int64_t *>(atom->tag), list->inum, list->ilist,
-                                list->numneigh, list->firstneigh, full_list, eflag, 1, &atom->f[0][0], &eng, vir,
-                                eflag_atom ? eatom : nullptr, vflag_atom ? &vatom[0][0] : nullptr);
+    // tagint is 32-bit in a default build and 64-bit with -DLAMMPS_BIGBIG:
+    // widen it to the ABI's int64 instead of reinterpreting the array
+    const int nall = atom->nlocal + atom->nghost;
+    tag64.resize(nall);
+    for (int i = 0; i < nall; ++i) tag64[i] = static_cast<int64_t>(atom->tag[i]);
+    const int rc = mlip_compute(ctx, atom->nlocal, atom->nghost, &atom->x[0][0], atom->type, tag64.data(),
+                                list->inum, list->ilist, list->numneigh, list->firstneigh, full_list, eflag, 1,
+                                &atom->f[0][0], &eng, vir, eflag_atom ? eatom : nullptr,
+                                vflag_atom ? &vatom[0][0] : nullptr);
     if (rc != MLIP_OK) error->one(FLERR, mlip_last_error(ctx));
     if (eflag_global) eng_vdwl += eng;
     if (vflag_global)

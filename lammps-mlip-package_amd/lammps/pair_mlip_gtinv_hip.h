@@ -13,6 +13,9 @@ PairStyle(mlip_pair/hip, PairMLIPGtinvHIP)
 #ifndef LMP_PAIR_MLIP_GTINV_HIP_H
 #define LMP_PAIR_MLIP_GTINV_HIP_H
 
+#include <cstdint>
+#include <vector>
+
 #include "mlip_hip.h"
 #include "pair.h"
 
@@ -31,6 +34,7 @@ class PairMLIPGtinvHIP : public Pair {
    protected:
     mlip_ctx *ctx = nullptr;
     int full_list = 0;
+    std::vector<int64_t> tag64;  // atom->tag widened to the ABI's int64
     void allocate();
 };
 

@@ -46,6 +46,7 @@ def library():
     L.mlip_masses.argtypes = [vp, dp]
     L.mlip_settings.argtypes = [c_int]
     L.mlip_set_precision.argtypes = [vp, c_int]
+    L.mlip_set_atom_mode.argtypes = [vp, c_int]
     L.mlip_last_error.argtypes = [vp]
     L.mlip_last_error.restype = c_char_p
     L.mlip_compute.argtypes = [vp, c_int, c_int, dp, ip, i64p, c_int, ip, ip, ctypes.POINTER(ip), c_int, c_int,
@@ -81,6 +82,8 @@ def library():
     L.mlip_pool_done.argtypes = [vp, ip, c_int]
     L.mlip_pool_result.argtypes = [vp, c_int, dp, dp, dp, dp, dp, dp, c_int]
     L.mlip_pool_stats.argtypes = [vp, i64p]
+    L.mlip_pool_rebuilds.argtypes = [vp]
+    L.mlip_pool_rebuilds.restype = i64
     L.mlip_batch_set.argtypes = [vp, c_int, ip, dp, dp, ip]
     L.mlip_batch_get.argtypes = [vp, dp, dp, dp]
     L.mlip_graph_mode.argtypes = [vp, c_int]

@@ -83,6 +83,46 @@ class PairMLIPGtinv:
     def set_precision(self, mode):
         self._check(self.lib.mlip_set_precision(self.ctx, {"fp64": 0, "mixed": 1}[mode]))
 
+    def set_atom_mode(self, mode):
+        """Per-atom energy/virial: "pair" (the reference CPU style's
+        ev_tally_xyz split, default) or "site" (site energies E_i)."""
+        self._check(self.lib.mlip_set_atom_mode(self.ctx, {"pair": 0, "site": 1}[mode]))
+
+    def compute_lammps(self, nl, types, eflag=1, vflag=1, eatom=False, vatom=False, tag_dtype=np.int64):
+        """The LAMMPS-shaped entry point ``mlip_compute``: ``firstneigh`` as
+        an array of per-atom pointers into the list (``int**``), exactly as
+        ``NeighList::firstneigh`` hands it to ``Pair::compute``."""
+        f = np.zeros((nl.nall, 3))
+        eng = np.zeros(1)
+        vir = np.zeros(6)
+        ea = np.zeros(nl.nall) if eatom else None
+        va = np.zeros((nl.nall, 6)) if vatom else None
+        typ = np.ascontiguousarray(types, dtype=np.int32)
+        tag = np.ascontiguousarray(np.asarray(nl.tag).astype(tag_dtype).astype(np.int64))
+        idx = np.ascontiguousarray(nl.idx, dtype=np.int32)
+        ip = ctypes.POINTER(ctypes.c_int)
+        rows = (ip * max(nl.nlocal, 1))()
+        base = idx.ctypes.data
+        for ii in range(nl.inum):
+            rows[int(nl.ilist[ii])] = ctypes.cast(base + 4 * int(nl.off[ii]), ip)
+        rc = self.lib.mlip_compute(
+            self.ctx, nl.nlocal, nl.nall - nl.nlocal, _p(nl.x, ctypes.c_double), _p(typ, ctypes.c_int),
+            _p(tag, ctypes.c_int64), nl.inum, _p(nl.ilist, ctypes.c_int), _p(nl.numneigh, ctypes.c_int),
+            ctypes.cast(rows, ctypes.POINTER(ip)), 1 if nl.full else 0, eflag, vflag, _p(f, ctypes.c_double),
+            _p(eng, ctypes.c_double), _p(vir, ctypes.c_double), _p(ea, ctypes.c_double) if eatom else _null(),
+            _p(va, ctypes.c_double) if vatom else _null())
+        self._check(rc)
+        out = {"energy": float(eng[0]), "forces_all": f, "virial": vir}
+        if nl.owner is not None:
+            fo = np.zeros((nl.nlocal, 3))
+            np.add.at(fo, nl.owner, f)
+            out["forces"] = fo
+        if eatom:
+            out["eatom"] = ea
+        if vatom:
+            out["vatom"] = va
+        return out
+
     def compute(self, nl, types, eflag=1, vflag=1, eatom=False, vatom=False):
         """Evaluate on a ``workloads.NeighborList``; ``types`` are the
         1-based LAMMPS types of all nall positions.  Returns a dict with the
@@ -306,8 +346,13 @@ class RequestPool:
     def bytes_for(pair, max_atoms, max_pairs, max_requests):
         return int(pair.lib.mlip_pool_bytes(pair.ctx, int(max_atoms), int(max_pairs), int(max_requests)))
 
-    def admit(self, s, nsteps=0, dt=0.001, skin=0.0, velocities=None):
-        """Returns a handle, or None when the pool has no room now."""
+    DEFAULT_SKIN = 1.0  # A, for NVE requests (their lists are rebuilt when an atom moves skin / 2)
+
+    def admit(self, s, nsteps=0, dt=0.001, skin=None, velocities=None):
+        """Returns a handle, or None when the pool has no room now.  skin
+        defaults to 0 for single evaluations and DEFAULT_SKIN for NVE."""
+        if skin is None:
+            skin = self.DEFAULT_SKIN if nsteps > 0 else 0.0
         cell = np.ascontiguousarray(s.cell, dtype=np.float64).reshape(9)
         x = np.ascontiguousarray(s.positions, dtype=np.float64)
         t = np.ascontiguousarray(s.types, dtype=np.int32)
@@ -334,7 +379,9 @@ class RequestPool:
         x = np.ascontiguousarray(np.concatenate([s.positions for s in structures]), dtype=np.float64)
         t = np.ascontiguousarray(np.concatenate([s.types for s in structures]), dtype=np.int32)
         ns = np.ascontiguousarray(nsteps, dtype=np.int32)
-        sk = np.ascontiguousarray(np.zeros(n) if skins is None else skins, dtype=np.float64)
+        if skins is None:
+            skins = [self.DEFAULT_SKIN if k > 0 else 0.0 for k in nsteps]
+        sk = np.ascontiguousarray(skins, dtype=np.float64)
         v = None
         if velocities is not None:
             v = np.ascontiguousarray(np.concatenate([np.zeros((s.natoms, 3)) if vv is None else vv
@@ -384,6 +431,10 @@ class RequestPool:
             self.pair._check(-nev)
         return {"energy": float(e[0]), "virial": vir, "forces": f, "positions": x, "velocities": v,
                 "thermo": th[:nev * self.THERMO].reshape(nev, self.THERMO)}
+
+    def rebuilds(self):
+        """Neighbor-list rebuilds of NVE requests so far."""
+        return int(self.lib.mlip_pool_rebuilds(self.pair.ctx))
 
     def stats(self):
         out = np.zeros(6, dtype=np.int64)

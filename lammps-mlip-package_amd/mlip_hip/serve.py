@@ -20,7 +20,7 @@ MVV2E = 1.0364269e-4
 
 
 class Request:
-    __slots__ = ("t_arrival", "structure", "nsteps", "v0", "handle", "t_done", "evals")
+    __slots__ = ("t_arrival", "structure", "nsteps", "v0", "handle", "t_done", "evals", "energy", "forces")
 
     def __init__(self, t_arrival, structure, nsteps, v0):
         self.t_arrival = t_arrival
@@ -30,6 +30,8 @@ class Request:
         self.handle = None
         self.t_done = None
         self.evals = 0
+        self.energy = None
+        self.forces = None
 
 
 def maxwell(s, mass_of_type, temperature, rng):
@@ -140,6 +142,8 @@ def serve(pool, stream, dt=0.001, skin=0.5, drain_timeout=120.0, nthreads=16, ma
             r = by_handle.pop(h)
             r.t_done = tn
             r.evals = res["thermo"].shape[0]
+            r.energy = res["energy"]
+            r.forces = res["forces"]
             atom_evals += r.evals * r.structure.natoms
             done.append(r)
         host["results_s"] += time.perf_counter() - tr
@@ -181,6 +185,8 @@ def serve_sequential(pool, stream, dt=0.001, skin=0.5, drain_timeout=120.0):
             res = pool.result(hh)
             r.t_done = time.perf_counter() - t0
             r.evals = res["thermo"].shape[0]
+            r.energy = res["energy"]
+            r.forces = res["forces"]
             atom_evals += r.evals * r.structure.natoms
             done.append(r)
     makespan = time.perf_counter() - t0
@@ -194,6 +200,29 @@ def serve_sequential(pool, stream, dt=0.001, skin=0.5, drain_timeout=120.0):
         "makespan_s": makespan,
         "atom_evals_per_s": atom_evals / makespan if makespan > 0 else 0.0,
     }
+
+
+def stream_errors(stream, reference):
+    """Per-request error of a served stream's results against the same
+    requests evaluated in another run (the FP64 one-at-a-time stream):
+    energy per atom and largest force difference of the last evaluation,
+    separately for single evaluations and NVE runs (whose final state also
+    carries the trajectories' divergence)."""
+    out = {}
+    for kind, sel in (("single", lambda r: r.nsteps == 0), ("nve", lambda r: r.nsteps > 0)):
+        de, df = [], []
+        for a, b in zip(stream, reference):
+            if not sel(a) or a.energy is None or b.energy is None:
+                continue
+            de.append(abs(a.energy - b.energy) / a.structure.natoms)
+            df.append(float(np.abs(a.forces - b.forces).max()) if a.forces.size else 0.0)
+        out[kind] = {"requests": len(de),
+                     "energy_per_atom_max": float(max(de)) if de else None,
+                     "energy_per_atom_median": float(np.median(de)) if de else None,
+                     "force_max": float(max(df)) if df else None,
+                     "force_median": float(np.median(df)) if df else None}
+    out["unit"] = "eV/atom, eV/A"
+    return out
 
 
 def pool_for_budget(pair, hbm_bytes, max_requests=4096, pairs_per_atom=160):

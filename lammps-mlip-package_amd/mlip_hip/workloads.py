@@ -336,6 +336,23 @@ CONFIG_MODELS = {
     "binary_order4_quadratic": dict(elements=["Ti", "Al"], masses=[47.87, 26.98], cutoff=8.0, model_type=4,
                                     maxp=2, maxl=12, gtinv_order=4, gtinv_maxl=[12, 8, 2], gtinv_sym=[0, 0, 0],
                                     centers=(0.0, 7.0, 12), beta=1.0, sigma=1e-4, seed=5),
+    # the whole gtinv table: invariants of order 2..6 with gtinv_maxl
+    # [20, 10, 2, 1, 1] (l up to 20, the order-5 and order-6 families), 20
+    # Gaussians, two species; linear (model_type 2, max_p 1) and quadratic in
+    # the order<=2 invariants (model_type 4, max_p 2)
+    "binary_order6_l20_linear": dict(elements=["Ti", "Al"], masses=[47.87, 26.98], cutoff=6.0, model_type=2,
+                                     maxp=1, maxl=20, gtinv_order=6, gtinv_maxl=[20, 10, 2, 1, 1],
+                                     gtinv_sym=[0, 0, 0, 0, 0], centers=(0.0, 5.0, 20), beta=1.0, sigma=1e-3,
+                                     seed=21),
+    "binary_order6_l20_quadratic": dict(elements=["Ti", "Al"], masses=[47.87, 26.98], cutoff=6.0, model_type=4,
+                                        maxp=2, maxl=20, gtinv_order=6, gtinv_maxl=[20, 10, 2, 1, 1],
+                                        gtinv_sym=[0, 0, 0, 0, 0], centers=(0.0, 5.0, 20), beta=1.0, sigma=1e-4,
+                                        seed=22),
+    # order 3 with l up to 10: the order-3 general-loop kernel (signatures
+    # beyond the unrolled set)
+    "binary_order3_l10": dict(elements=["Ti", "Al"], masses=[47.87, 26.98], cutoff=6.0, model_type=2, maxp=1,
+                              maxl=12, gtinv_order=3, gtinv_maxl=[12, 10], gtinv_sym=[0, 0], centers=(0.0, 5.0, 10),
+                              beta=1.0, sigma=1e-3, seed=23),
 }
 
 

@@ -442,10 +442,13 @@ vec2i permutation(int order) {  // Polynomial::permutation, mlip_polynomial.cpp:
     return s;
 }
 
-int find_in_set(const std::set<vec1i>& st, vec1i v) {
+// Position of the sorted v in the ordered set (the reference's find_comb:
+// std::distance from begin), looked up in the set's sorted copy by binary
+// search so that large tables do not cost a linear walk per lookup.
+int find_in_set(const std::vector<vec1i>& sorted_set, vec1i v) {
     std::sort(v.begin(), v.end());
-    auto it = st.find(v);
-    return static_cast<int>(std::distance(st.begin(), it));
+    auto it = std::lower_bound(sorted_set.begin(), sorted_set.end(), v);
+    return static_cast<int>(it - sorted_set.begin());
 }
 
 void Potential::polynomial_gtinv() {
@@ -471,6 +474,7 @@ void Potential::polynomial_gtinv() {
             }
     }
     // set_gtinv
+    const std::vector<vec1i> lmt_sorted(uniq_lmt_prod_set.begin(), uniq_lmt_prod_set.end());
     gtinv_array.assign(n_tc, std::vector<std::vector<GtEntry>>(n_lm_all));
     gtinv_all.assign(linear.size(), {});
     for (size_t i1 = 0; i1 < linear.size(); ++i1) {
@@ -488,7 +492,7 @@ void Potential::polynomial_gtinv() {
                     int s = seq[j][k2];
                     info.push_back(lmtc2seq(lma[k1][s], tc[s]));
                 }
-                const int lmt_pi = info.empty() ? -1 : find_in_set(uniq_lmt_prod_set, info);
+                const int lmt_pi = info.empty() ? -1 : find_in_set(lmt_sorted, info);
                 GtEntry gt{static_cast<int>(i1), lmt_pi, coeffs[k1], order, tc0, lm0};
                 gtinv_array[tc0][lm0].push_back(gt);
                 gtinv_all[i1].push_back(gt);
@@ -513,6 +517,7 @@ void Potential::polynomial_gtinv() {
     };
     uniq_comb_add(comb2);
     uniq_comb_add(comb3);
+    const std::vector<vec1i> comb_sorted(uniq_comb_set.begin(), uniq_comb_set.end());
     auto set_poly = [&](const vec2i& comb, int sindex) {
         if (comb.empty()) return;
         const int order = static_cast<int>(comb[0].size());
@@ -524,7 +529,7 @@ void Potential::polynomial_gtinv() {
             for (int ci : c) poly_order += gtinv_all[ci % n_gtinv][0].order;
             for (const auto& s : seq) {
                 for (int i = 1; i < order; ++i) info[i - 1] = c[s[i]];
-                const int cindex = find_in_set(uniq_comb_set, info);
+                const int cindex = find_in_set(comb_sorted, info);
                 const int c0 = c[s[0]];
                 const int n0 = c0 / n_gtinv, ig0 = c0 % n_gtinv;
                 for (const auto& gt : gtinv_all[ig0])
@@ -607,6 +612,7 @@ struct ComputeOut {
     double eng;
     double virial[6];
     double* eatom;
+    double* vatom;      // optional [nall][6] (xx, yy, zz, xy, xz, yz)
     double* anlm_out;   // optional [inum][n_type][n_fn][n_lm_all][2]
     double* feat_out;   // optional [inum][n_des]
 };
@@ -886,12 +892,15 @@ void run_compute(const Oracle& o, int nlocal, const double* x, const int64_t* ta
                 out.eatom[i] += 0.5 * e;
                 out.eatom[j] += 0.5 * e;
             }
-            out.virial[0] += dx * fx;
-            out.virial[1] += dy * fy;
-            out.virial[2] += dz * fz;
-            out.virial[3] += dx * fy;
-            out.virial[4] += dx * fz;
-            out.virial[5] += dy * fz;
+            const double v[6] = {dx * fx, dy * fy, dz * fz, dx * fy, dx * fz, dy * fz};
+            for (int k = 0; k < 6; ++k) {
+                out.virial[k] += v[k];
+                if (out.vatom) {
+                    // ev_tally_xyz with newton_pair: half to each atom of the pair
+                    out.vatom[6 * size_t(i) + k] += 0.5 * v[k];
+                    out.vatom[6 * size_t(j) + k] += 0.5 * v[k];
+                }
+            }
         }
     }
     (void)nlocal;
@@ -936,7 +945,7 @@ void run_train(const Oracle& o, int nlocal, const double* x, const int64_t* tag,
             for (int jj = 0; jj < L.numneigh[i]; ++jj) nall = std::max(nall, L.idx[L.off[ii] + jj] + 1);
         }
         ftmp.assign(3 * size_t(nall), 0.0);
-        ComputeOut out{ftmp.data(), 0.0, {0, 0, 0, 0, 0, 0}, nullptr, an.data(), feat.data()};
+        ComputeOut out{ftmp.data(), 0.0, {0, 0, 0, 0, 0, 0}, nullptr, nullptr, an.data(), feat.data()};
         run_compute(o, nlocal, x, tag, L, out);
     }
     auto A = [&](int owner, int s, int n, int lm) {
@@ -1157,16 +1166,18 @@ void oracle_set_types_by_tag(void* h, int n, const int* elem_of_tag) {
 
 // LAMMPS Pair::compute contract restated for a half (or any) neighbor list
 // given as ilist/numneigh plus a flattened firstneigh (per-ilist offsets).
-// f (nall*3) is accumulated; eng and virial (xx,yy,zz,xy,xz,yz) overwritten.
+// f (nall*3) is accumulated; eng and virial (xx,yy,zz,xy,xz,yz) overwritten;
+// eatom [nall] and vatom [nall][6] (nullable) accumulated as ev_tally_xyz
+// does with newton_pair on (half of each pair's energy and virial per atom).
 int oracle_compute(void* h, int nlocal, int nall, const double* x, const int64_t* tag, int inum,
                    const int* ilist, const int* numneigh, const int* neigh_off, const int* neigh_idx,
-                   double* f, double* eng, double* virial, double* eatom, double* anlm_out,
+                   double* f, double* eng, double* virial, double* eatom, double* vatom, double* anlm_out,
                    double* feat_out) {
     auto* o = static_cast<Oracle*>(h);
     if (static_cast<int>(o->types_by_tag.size()) < nlocal) return -1;
     (void)nall;
     List L{inum, ilist, neigh_off, neigh_idx, numneigh};
-    ComputeOut out{f, 0.0, {0, 0, 0, 0, 0, 0}, eatom, anlm_out, feat_out};
+    ComputeOut out{f, 0.0, {0, 0, 0, 0, 0, 0}, eatom, vatom, anlm_out, feat_out};
     run_compute(*o, nlocal, x, tag, L, out);
     *eng = out.eng;
     for (int k = 0; k < 6; ++k) virial[k] = out.virial[k];

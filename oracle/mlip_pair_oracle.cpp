@@ -276,7 +276,7 @@ void pair_oracle_set_types_by_tag(void* h, int n, const int* elem_of_tag) {
 // feat_out (nlocal x n_des, nullable) receives the per-atom features.
 int pair_oracle_compute(void* h, int nlocal, int nall, const double* x, const int64_t* tag, int inum,
                         const int* ilist, const int* numneigh, const int* neigh_off, const int* neigh_idx,
-                        double* f, double* eng, double* virial, double* feat_out) {
+                        double* f, double* eng, double* virial, double* eatom, double* vatom, double* feat_out) {
     auto* o = static_cast<PairOracle*>(h);
     const PairPot& pot = o->pot;
     if (static_cast<int>(o->types_by_tag.size()) < nlocal) return -1;
@@ -389,12 +389,20 @@ int pair_oracle_compute(void* h, int nlocal, int nall, const double* x, const in
                 f[3 * j + 1] -= fpair * dy;
                 f[3 * j + 2] -= fpair * dz;
                 E += ev[ii][jj];
-                V[0] += dx * dx * fpair;
-                V[1] += dy * dy * fpair;
-                V[2] += dz * dz * fpair;
-                V[3] += dx * dy * fpair;
-                V[4] += dx * dz * fpair;
-                V[5] += dy * dz * fpair;
+                const double v[6] = {dx * dx * fpair, dy * dy * fpair, dz * dz * fpair,
+                                     dx * dy * fpair, dx * dz * fpair, dy * dz * fpair};
+                for (int k = 0; k < 6; ++k) V[k] += v[k];
+                // ev_tally with newton_pair: half of the pair's energy and
+                // virial to each atom
+                if (eatom) {
+                    eatom[i] += 0.5 * ev[ii][jj];
+                    eatom[j] += 0.5 * ev[ii][jj];
+                }
+                if (vatom)
+                    for (int k = 0; k < 6; ++k) {
+                        vatom[6 * static_cast<size_t>(i) + k] += 0.5 * v[k];
+                        vatom[6 * static_cast<size_t>(j) + k] += 0.5 * v[k];
+                    }
             }
         }
     }

@@ -39,7 +39,7 @@ def lib():
         L.oracle_combs.argtypes = [vp, ctypes.c_int, ip]
         L.oracle_set_types_by_tag.argtypes = [vp, ctypes.c_int, ip]
         L.oracle_compute.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp, lp, ctypes.c_int, ip, ip,
-                                     ip, ip, dp, dp, dp, dp, dp, dp]
+                                     ip, ip, dp, dp, dp, dp, dp, dp, dp]
         L.oracle_train.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp, lp, ctypes.c_int, ip, ip, ip, ip, dp, dp, dp,
                                    dp]
         L.oracle_weights.argtypes = [vp, dp]
@@ -58,7 +58,7 @@ def lib():
         L.pair_oracle_combs.argtypes = [vp, ctypes.c_int, ip]
         L.pair_oracle_set_types_by_tag.argtypes = [vp, ctypes.c_int, ip]
         L.pair_oracle_compute.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp, lp, ctypes.c_int, ip, ip, ip, ip,
-                                          dp, dp, dp, dp]
+                                          dp, dp, dp, dp, dp, dp]
         _lib = L
     return _lib
 
@@ -115,25 +115,32 @@ class PairOracle:
         lib().pair_oracle_combs(self.h, order, _p(out, ctypes.c_int))
         return out[:n]
 
-    def compute(self, nl, elem_of_atom, want_features=False):
+    def compute(self, nl, elem_of_atom, want_features=False, eatom=False, vatom=False):
         L = lib()
         ea = np.ascontiguousarray(elem_of_atom, dtype=np.int32)
         L.pair_oracle_set_types_by_tag(self.h, ea.size, _p(ea, ctypes.c_int))
         f = np.zeros((nl.nall, 3))
         eng = np.zeros(1)
         vir = np.zeros(6)
+        e_at = np.zeros(nl.nall) if eatom else None
+        v_at = np.zeros((nl.nall, 6)) if vatom else None
         ft = np.zeros((nl.nlocal, self.info["n_des"])) if want_features else None
         null = ctypes.POINTER(ctypes.c_double)()
+        q = lambda a: _p(a, ctypes.c_double) if a is not None else null  # noqa: E731
         rc = L.pair_oracle_compute(self.h, nl.nlocal, nl.nall, _p(nl.x, ctypes.c_double),
                                    _p(nl.tag, ctypes.c_int64), nl.inum, _p(nl.ilist, ctypes.c_int),
                                    _p(nl.numneigh, ctypes.c_int), _p(nl.off, ctypes.c_int),
                                    _p(nl.idx, ctypes.c_int), _p(f, ctypes.c_double), _p(eng, ctypes.c_double),
-                                   _p(vir, ctypes.c_double), _p(ft, ctypes.c_double) if want_features else null)
+                                   _p(vir, ctypes.c_double), q(e_at), q(v_at), q(ft))
         if rc != 0:
             raise RuntimeError("pair_oracle_compute failed (%d)" % rc)
         fo = np.zeros((nl.nlocal, 3))
         np.add.at(fo, nl.owner, f)
         out = {"energy": float(eng[0]), "forces_all": f, "virial": vir, "forces": fo}
+        if eatom:
+            out["eatom"] = e_at
+        if vatom:
+            out["vatom"] = v_at
         if want_features:
             out["features"] = ft
         return out
@@ -172,7 +179,7 @@ class GtinvOracle:
         lib().oracle_combs(self.h, order, _p(out, ctypes.c_int))
         return out[:n]
 
-    def compute(self, nl, elem_of_atom, want_anlm=False, want_features=False, eatom=False):
+    def compute(self, nl, elem_of_atom, want_anlm=False, want_features=False, eatom=False, vatom=False):
         """Evaluate on a NeighborList; elem_of_atom[i] is the potential
         element index of owned atom i (the reference's types[tag-1])."""
         L = lib()
@@ -182,6 +189,7 @@ class GtinvOracle:
         eng = np.zeros(1)
         vir = np.zeros(6)
         e_at = np.zeros(nl.nall) if eatom else None
+        v_at = np.zeros((nl.nall, 6)) if vatom else None
         nt, nf, nlm = self.info["n_type"], self.info["n_fn"], self.info["n_lm_all"]
         an = np.zeros((nl.inum, nt, nf, nlm, 2)) if want_anlm else None
         ft = np.zeros((nl.inum, self.info["n_des"])) if want_features else None
@@ -192,6 +200,7 @@ class GtinvOracle:
                               _p(nl.idx, ctypes.c_int), _p(f, ctypes.c_double),
                               _p(eng, ctypes.c_double), _p(vir, ctypes.c_double),
                               _p(e_at, ctypes.c_double) if eatom else null,
+                              _p(v_at, ctypes.c_double) if vatom else null,
                               _p(an, ctypes.c_double) if want_anlm else null,
                               _p(ft, ctypes.c_double) if want_features else null)
         if rc != 0:
@@ -203,6 +212,8 @@ class GtinvOracle:
         out["forces"] = fo
         if eatom:
             out["eatom"] = e_at
+        if vatom:
+            out["vatom"] = v_at
         if want_anlm:
             out["anlm"] = an[..., 0] + 1j * an[..., 1]
         if want_features:

@@ -16,6 +16,7 @@ import md_oracle as MD
 import oracle_lib as O
 from common import GOLD, config_potential, ref, rel_force_err
 from mlip_hip import BatchSystem, PairMLIPGtinv, RequestPool
+from mlip_hip import md as MDP
 from mlip_hip import workloads as W
 from test_md_oracle import check_thermo, potential_masses
 
@@ -46,11 +47,13 @@ def drain(pool, max_ticks=10000):
 
 
 def thermo_of(row, s):
-    return MD.thermo(s.natoms, s.volume, row[0], row[1:7], row[7:13])
+    # the package's thermo output (mlip_hip.md), checked here against the
+    # reference's logs and the oracle integrator
+    return MDP.thermo(s.natoms, s.volume, row)
 
 
 @pytest.mark.parametrize("name", ["gtinv-197", "gtinv-434", "gtinv-820"])
-def test_pool_nve_regression_runs(name):
+def test_pool_nve_regression_runs(name, tmp_path):
     g = goldens()[name]
     s = W.read_lammps_data(ref(g["structure"]))
     p = gpu_pair(ref(g["potential"]), g["elements"])
@@ -66,6 +69,14 @@ def test_pool_nve_regression_runs(name):
     d = MD.periodic_diff(res["positions"], MD.read_dump(ref(g["dump100_dt0.001"])), s.cell)
     print("%s: pool NVE max |x - dump| = %.2e A" % (name, np.abs(d).max()), flush=True)
     assert np.abs(d).max() < 1e-8
+    # the package's dump writer reproduces the reference's dump file
+    out = os.path.join(str(tmp_path), "dump.atom")
+    MDP.write_dump(out, 100, s, res["positions"])
+    mine, theirs = open(out).read().split("\n"), open(ref(g["dump100_dt0.001"])).read().split("\n")
+    assert mine[:9] == theirs[:9]
+    assert np.abs(MD.read_dump(out) - MD.read_dump(ref(g["dump100_dt0.001"]))).max() < 1e-8
+    log = MDP.thermo_log(th, s, every=50)
+    assert len(log.strip().split("\n")) == 4
     # and against the oracle integrator, step by step in energy
     o = O.Oracle(ref(g["potential"]))
     xo, vo, tho = MD.run_nve(o, s, potential_masses(ref(g["potential"])), 0.001, 100, record=(25, 100))
@@ -220,3 +231,35 @@ def test_serving_loop_small_stream():
     print("serve:", out, flush=True)
     assert out["completed"] == len(stream) and out["unfinished"] == 0
     assert out["latency_p99_ms"] >= out["latency_p50_ms"] > 0
+
+
+def test_pool_nve_rebuilds_lists():
+    """200 NVE steps at 300 K (Maxwell draw, seed 3) with a 0.3 A skin: the
+    pool rebuilds the request's list whenever an atom would pass skin / 2 and
+    matches the oracle integrator, whose list is rebuilt exactly every step,
+    to 1e-9 A; an NVE request without a skin is refused."""
+    from mlip_hip import serve as SV
+    from mlip_hip._lib import MlipError
+    g = goldens()["gtinv-820"]
+    pot = ref(g["potential"])
+    s = W.read_lammps_data(ref(g["structure"]))
+    masses = potential_masses(pot)
+    v0 = SV.maxwell(s, masses, 300.0, np.random.default_rng(3))
+    p = gpu_pair(pot, g["elements"])
+    pool = RequestPool(p, 1000, 200000, 4)
+    with pytest.raises(MlipError):
+        pool.admit(s, nsteps=10, dt=0.001, skin=0.0, velocities=v0)
+    h = pool.admit(s, nsteps=200, dt=0.001, skin=0.3, velocities=v0)
+    res = drain(pool)[h]
+    nreb = pool.rebuilds()
+    o = O.Oracle(pot)
+    xo, vo, tho = MD.run_nve(o, s, masses, 0.001, 200, record=(50, 100, 200), v0=v0)
+    dx = np.abs(MD.periodic_diff(res["positions"], xo, s.cell)).max()
+    moved = np.linalg.norm(MD.periodic_diff(res["positions"], s.positions, s.cell), axis=1).max()
+    print("pool NVE 200 steps, skin 0.3 A: %d list rebuilds, largest move %.3f A, max |x - oracle| = %.2e A"
+          % (nreb, moved, dx), flush=True)
+    assert nreb >= 1
+    assert dx < 1e-9
+    assert np.abs(res["velocities"] - vo).max() < 1e-8
+    for k in (50, 100, 200):
+        assert abs(thermo_of(res["thermo"][k], s)["E_pair"] - tho[k]["E_pair"]) < 1e-9
