@@ -394,6 +394,7 @@ def served_section(pot, rates, windows, budget_gb):
         r["arrival_window_s"] = win
         r["fp64_sequential"] = seq
         r["mixed_vs_fp64_error"] = SV.stream_errors(stream, seq_stream)
+        r["regime"] = "saturated" if r["device_busy_fraction"] and r["device_busy_fraction"] > 0.8 else "below saturation"
         out["rates"][str(rate)] = r
         log("served %d req/s (%d requests): p50 %.1f ms p99 %.1f ms, %.4g atom-evals/s; "
             "FP64 one-at-a-time: p50 %.1f ms p99 %.1f ms, %.4g atom-evals/s"
@@ -418,8 +419,9 @@ def main():
     ap.add_argument("--no-trajectory", action="store_true", help="skip the 256k-atom NVE error trajectory")
     ap.add_argument("--trajectory-steps", type=int, default=200)
     ap.add_argument("--train-structures", type=int, default=256)
-    ap.add_argument("--serve-windows", type=str, default="1.0,0.5,0.25",
-                    help="arrival windows (s) for the 200/1000/5000 req/s streams")
+    ap.add_argument("--serve-windows", type=str, default="10.0,1.0,0.5",
+                    help="arrival windows (s) for the 200/1000/5000 req/s streams (200 req/s is below saturation: "
+                         "a steady state; the others saturate the device)")
     ap.add_argument("--pool-gb", type=float, default=16.0, help="HBM budget of the request pool")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="also launch a plain copy of this many MiB (HBM counter calibration)")

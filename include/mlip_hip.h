@@ -163,8 +163,24 @@ int mlip_graph_mode(mlip_ctx* ctx, int on);
 
 /* Times each launch shape of the force kernel (64/128/192/256 threads per
  * workgroup) `reps` times on the current resident structure or batch and
- * keeps the fastest; *chosen receives it. */
+ * keeps the fastest; *chosen receives it.  Also times the two a_nlm kernels
+ * (where the model fits the matrix-core one) and keeps the faster. */
 int mlip_autotune(mlip_ctx* ctx, int reps, int* chosen);
+
+/* a_nlm kernel: 0 = thread per lm column with FP64 register accumulators,
+ * 1 = chunk products on the matrix cores (FP64 MFMA; the default, used when
+ * the model has l_max <= 12, two species and at most 32 radial functions,
+ * else kernel 0 runs).  Both give the same sums up to rounding order. */
+int mlip_set_anlm_kernel(mlip_ctx* ctx, int kind);
+int mlip_get_anlm_kernel(const mlip_ctx* ctx);
+
+/* Order-3/4 workgroup plan: regroups the visits over the 12 waves of the
+ * order-3/4 kernel with each of a few work models (terms per scheduling
+ * region and per visit), times the adjoint kernel `reps` times on the current
+ * resident structure or batch with each, and keeps the fastest; plan2
+ * receives its two constants.  Results do not depend on the plan beyond the
+ * summation order of split rows. */
+int mlip_autotune_q34(mlip_ctx* ctx, int reps, int* plan2);
 
 /* Sets the force kernel's launch shape directly (64, 128, 192 or 256
  * threads per workgroup; 192 by default).  Results do not depend on it

@@ -131,7 +131,12 @@ size_t lds_limit();
 // hipGetLastError after a launch, with a dynamic LDS request checked first.
 hipError_t launched(size_t lds_bytes = 0);
 
-hipError_t launch_anlm(const DevModel& m, const DevSystem& s, int precision, hipStream_t st);
+// a_nlm kernels: kAnlmValu (k_anlm, thread per lm column, FP64 registers) or
+// kAnlmMfma (k_anlm_mfma, chunk products on the matrix cores) where the
+// model fits it (anlm_has_mfma), k_anlm otherwise
+enum AnlmKind : int { kAnlmValu = 0, kAnlmMfma = 1 };
+hipError_t launch_anlm(const DevModel& m, const DevSystem& s, int precision, hipStream_t st, int kind = kAnlmValu);
+bool anlm_has_mfma(const DevModel& m);
 hipError_t launch_q12(const DevModel& m, const DevSystem& s, bool feat, hipStream_t st);
 hipError_t launch_q34_adj(const DevModel& m, const DevSystem& s, hipStream_t st);
 hipError_t launch_q34_feat(const DevModel& m, const DevSystem& s, hipStream_t st);

@@ -37,42 +37,15 @@ void set_err(char* buf, size_t len, const std::string& msg) {
     if (buf && len) std::snprintf(buf, len, "%s", msg.c_str());
 }
 
-bool upload_model(mlip_ctx* c) {
+// The order-3/4 workgroup plan of every centre type (groups and packed
+// visit descriptors) into device tables, with their ranges in c->dm.
+bool upload_q34_plan(mlip_ctx* c) {
     const Model& m = c->model;
     DevModel& dm = c->dm;
-    std::memset(&dm, 0, sizeof dm);
-    dm.n_type = m.n_type;
-    dm.n_fn = m.n_fn;
-    dm.n_gtinv = m.n_gtinv;
-    dm.n_lmh = m.n_lmh;
-    dm.n_lmh3 = m.n_lmh3;
-    dm.maxl = m.maxl;
-    dm.lmax3 = m.lmax3;
-    dm.n_pitc = m.n_pitc;
-    dm.n_p = m.n_p;
-    for (int t = 0; t < 2; ++t)
-        for (int k = 0; k < 2; ++k) dm.prange[t][k] = m.prange[t][k];
-    dm.has_q34 = m.has_order34 ? 1 : 0;
-    dm.q34_red = m.q34_red;
-    dm.pair_model = m.pair_model ? 1 : 0;
-    for (int a = 0; a < 2; ++a)
-        for (int b = 0; b < 2; ++b) dm.pair_tc[a][b] = m.pair_tc[a][b];
-    dm.q34_dyn_adj = m.q34_dyn_adj ? 1 : 0;
-    dm.q34_dyn_feat = m.q34_dyn_feat ? 1 : 0;
-    dm.cutoff = m.cutoff;
-    for (int n = 0; n < m.n_fn; ++n) {
-        dm.beta[n] = m.beta[n];
-        dm.mu[n] = m.mu[n];
-    }
-    std::vector<VisitQ12> q12;
-    std::vector<int32_t> q12off;
     std::vector<VisitQ34> vadj, vfeat;
     std::vector<GroupQ34> gadj, gfeat;
     for (int t = 0; t < m.n_type; ++t) {
         const TypeTables& tt = m.types[t];
-        const int base = static_cast<int>(q12.size());
-        for (int v : tt.q12_off) q12off.push_back(base + v);
-        q12.insert(q12.end(), tt.visits_q12.begin(), tt.visits_q12.end());
         auto append = [](const std::vector<GroupQ34>& gs, const std::vector<VisitQ34>& vs,
                          std::vector<GroupQ34>& gout, std::vector<VisitQ34>& vout, int32_t& beg, int32_t& end) {
             const int vb = static_cast<int>(vout.size());
@@ -106,6 +79,57 @@ bool upload_model(mlip_ctx* c) {
         c->err = "order-3/4 visit outside the packed range";
         return false;
     }
+    if (!upload(c, c->d_vadj, vadjp.data(), vadjp.size(), "adjoint visits") ||
+        !upload(c, c->d_vfeat, vfeatp.data(), vfeatp.size(), "feature visits") ||
+        !upload(c, c->d_gadj, gadj.data(), gadj.size(), "adjoint groups") ||
+        !upload(c, c->d_gfeat, gfeat.data(), gfeat.size(), "feature groups"))
+        return false;
+    dm.vadj = c->d_vadj.p;
+    dm.gadj = c->d_gadj.p;
+    dm.vfeat = c->d_vfeat.p;
+    dm.gfeat = c->d_gfeat.p;
+    dm.q34_red = m.q34_red;
+    dm.q34_dyn_adj = m.q34_dyn_adj ? 1 : 0;
+    dm.q34_dyn_feat = m.q34_dyn_feat ? 1 : 0;
+    return true;
+}
+
+bool upload_model(mlip_ctx* c) {
+    const Model& m = c->model;
+    DevModel& dm = c->dm;
+    std::memset(&dm, 0, sizeof dm);
+    dm.n_type = m.n_type;
+    dm.n_fn = m.n_fn;
+    dm.n_gtinv = m.n_gtinv;
+    dm.n_lmh = m.n_lmh;
+    dm.n_lmh3 = m.n_lmh3;
+    dm.maxl = m.maxl;
+    dm.lmax3 = m.lmax3;
+    dm.n_pitc = m.n_pitc;
+    dm.n_p = m.n_p;
+    for (int t = 0; t < 2; ++t)
+        for (int k = 0; k < 2; ++k) dm.prange[t][k] = m.prange[t][k];
+    dm.has_q34 = m.has_order34 ? 1 : 0;
+    dm.q34_red = m.q34_red;
+    dm.pair_model = m.pair_model ? 1 : 0;
+    for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < 2; ++b) dm.pair_tc[a][b] = m.pair_tc[a][b];
+    dm.q34_dyn_adj = m.q34_dyn_adj ? 1 : 0;
+    dm.q34_dyn_feat = m.q34_dyn_feat ? 1 : 0;
+    dm.cutoff = m.cutoff;
+    for (int n = 0; n < m.n_fn; ++n) {
+        dm.beta[n] = m.beta[n];
+        dm.mu[n] = m.mu[n];
+    }
+    std::vector<VisitQ12> q12;
+    std::vector<int32_t> q12off;
+    for (int t = 0; t < m.n_type; ++t) {
+        const TypeTables& tt = m.types[t];
+        const int base = static_cast<int>(q12.size());
+        for (int v : tt.q12_off) q12off.push_back(base + v);
+        q12.insert(q12.end(), tt.visits_q12.begin(), tt.visits_q12.end());
+    }
+    if (!upload_q34_plan(c)) return false;
     // Polynomial features per species, compact: species t keeps the ranks
     // [lo, hi) = prange[t] it sees (the ranks are grouped so that this is
     // one range), column ci(n, r) = n (hi - lo) + r - lo, in a slot row of
@@ -202,10 +226,6 @@ bool upload_model(mlip_ctx* c) {
               upload(c, c->d_c3off, c3off.data(), c3off.size(), "cubic offsets") &&
               upload(c, c->d_q12, q12.data(), q12.size(), "q12 visits") &&
               upload(c, c->d_q12off, q12off.data(), q12off.size(), "q12 offsets") &&
-              upload(c, c->d_vadj, vadjp.data(), vadjp.size(), "adjoint visits") &&
-              upload(c, c->d_vfeat, vfeatp.data(), vfeatp.size(), "feature visits") &&
-              upload(c, c->d_gadj, gadj.data(), gadj.size(), "adjoint groups") &&
-              upload(c, c->d_gfeat, gfeat.data(), gfeat.size(), "feature groups") &&
               upload(c, c->d_zlist, zlist.data(), zlist.size(), "feature zero list");
     if (!ok) return false;
     dm.zlist = c->d_zlist.p;
@@ -255,10 +275,6 @@ bool upload_model(mlip_ctx* c) {
     dm.weight = c->d_weight.p;
     dm.q12 = c->d_q12.p;
     dm.q12_off = c->d_q12off.p;
-    dm.vadj = c->d_vadj.p;
-    dm.gadj = c->d_gadj.p;
-    dm.vfeat = c->d_vfeat.p;
-    dm.gfeat = c->d_gfeat.p;
     dm.lin_p = c->d_linp.p;
     dm.c3_off = c->d_c3off.p;
     dm.c3_bc = c->d_c3bc.p;
@@ -576,7 +592,7 @@ bool run_pipeline(mlip_ctx* c) {
             run("struct_reduce", [&] { return launch_struct_reduce(s, c->sbeg, c->send, c->nstruct, c->sres, st); });
         return ok;
     }
-    run("anlm", [&] { return launch_anlm(m, s, c->precision, st); });
+    run("anlm", [&] { return launch_anlm(m, s, c->precision, st, c->anlm_kind); });
     if (m.n_p > 0) {
         run("q12_feat", [&] { return launch_q12(m, s, true, st); });
         if (mm.has_feat34) run("q34_feat", [&] { return launch_q34_feat(m, s, st); });
@@ -1321,9 +1337,94 @@ int mlip_autotune(mlip_ctx* c, int reps, int* chosen) {
     }
     c->force_threads = best_nt;
     c->prof = prof;
+    // a_nlm: the matrix-core kernel against the register one, where the
+    // model fits the former
+    if (!c->model.pair_model && anlm_has_mfma(c->dm)) {
+        double tk[2] = {0.0, 0.0};
+        for (int kind = 0; kind < 2; ++kind) {
+            hipEvent_t a, b;
+            if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return MLIP_ERR_DEVICE;
+            hipError_t le = launch_anlm(c->dm, c->ds, c->precision, c->stream, kind);
+            (void)hipEventRecord(a, c->stream);
+            for (int r = 0; r < reps && le == hipSuccess; ++r)
+                le = launch_anlm(c->dm, c->ds, c->precision, c->stream, kind);
+            (void)hipEventRecord(b, c->stream);
+            float ms = 0.0f;
+            const bool ok = le == hipSuccess && hipEventSynchronize(b) == hipSuccess &&
+                            hipEventElapsedTime(&ms, a, b) == hipSuccess;
+            (void)hipEventDestroy(a);
+            (void)hipEventDestroy(b);
+            if (!ok) {
+                c->err = std::string("a_nlm kernel timing: ") + hipGetErrorString(le);
+                return MLIP_ERR_DEVICE;
+            }
+            tk[kind] = ms;
+        }
+        c->anlm_kind = tk[kAnlmMfma] <= tk[kAnlmValu] ? kAnlmMfma : kAnlmValu;
+    }
     drop_graph(c);
     if (chosen) *chosen = best_nt;
     return MLIP_OK;
+}
+
+int mlip_set_anlm_kernel(mlip_ctx* c, int kind) {
+    if (!c || (kind != kAnlmValu && kind != kAnlmMfma)) return MLIP_ERR_ARG;
+    if (kind != c->anlm_kind) drop_graph(c);
+    c->anlm_kind = kind;
+    return MLIP_OK;
+}
+
+int mlip_get_anlm_kernel(const mlip_ctx* c) { return c ? c->anlm_kind : -1; }
+
+int mlip_autotune_q34(mlip_ctx* c, int reps, int* plan2) {
+    if (!c || reps < 1) return MLIP_ERR_ARG;
+    if (!c->resident) {
+        c->err = "autotune needs a resident structure or batch";
+        return MLIP_ERR_STATE;
+    }
+    Model& m = c->model;
+    if (!m.has_order34 || m.pair_model) {
+        if (plan2) plan2[0] = m.plan_region, plan2[1] = m.plan_visit;
+        return MLIP_OK;
+    }
+    // the work models tried (terms per scheduling region, per visit); the
+    // structure's buffers hold the last evaluation's a_nlm and H, and the
+    // timed launches only rewrite G and the order-3/4 energies, which the
+    // next evaluation writes again
+    const int cand[][2] = {{3, 24}, {0, 0}, {2, 8}, {3, 12}, {3, 48}, {6, 24}, {1, 24}, {4, 32}};
+    double best = 1e300;
+    int bi = 0;
+    hipEvent_t a, b;
+    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return MLIP_ERR_DEVICE;
+    int rc = MLIP_OK;
+    for (int k = 0; k < static_cast<int>(sizeof cand / sizeof cand[0]) && rc == MLIP_OK; ++k) {
+        plan_q34(m, cand[k][0], cand[k][1]);
+        if (!upload_q34_plan(c) || !hip_ok(c, hipStreamSynchronize(c->stream), "plan upload")) {
+            rc = MLIP_ERR_DEVICE;
+            break;
+        }
+        hipError_t le = launch_q34_adj(c->dm, c->ds, c->stream);  // warm-up
+        (void)hipEventRecord(a, c->stream);
+        for (int r = 0; r < reps && le == hipSuccess; ++r) le = launch_q34_adj(c->dm, c->ds, c->stream);
+        (void)hipEventRecord(b, c->stream);
+        float ms = 0.0f;
+        if (le != hipSuccess || hipEventSynchronize(b) != hipSuccess || hipEventElapsedTime(&ms, a, b) != hipSuccess) {
+            c->err = std::string("order-3/4 plan timing: ") + hipGetErrorString(le);
+            rc = MLIP_ERR_DEVICE;
+            break;
+        }
+        if (ms < best) {
+            best = ms;
+            bi = k;
+        }
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    plan_q34(m, cand[bi][0], cand[bi][1]);
+    if (!upload_q34_plan(c) || !hip_ok(c, hipStreamSynchronize(c->stream), "plan upload")) return MLIP_ERR_DEVICE;
+    drop_graph(c);
+    if (plan2) plan2[0] = cand[bi][0], plan2[1] = cand[bi][1];
+    return rc;
 }
 
 int mlip_set_force_threads(mlip_ctx* c, int threads) {

@@ -94,6 +94,7 @@ struct mlip_ctx {
     int64_t npairs = 0;
     // launch shape chosen by mlip_autotune
     int force_threads = 192;
+    int anlm_kind = mlip::kAnlmMfma;  // a_nlm kernel (mlip_autotune keeps the faster one)
     int nblk_type[2] = {0, 0};  // slot blocks of each species (blocks are grouped by species)
     size_t wofs[2] = {0, 0};    // per-species quadratic form in d_wmat: offset and dimension
     int wdim[2] = {0, 0};

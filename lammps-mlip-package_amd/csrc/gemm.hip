@@ -11,8 +11,9 @@
 // Workgroup tile 64 rows x 128 columns, K staged through LDS 16 at a time
 // (the next stage's global loads in flight while the current one computes);
 // four waves of 32 x 64 each, i.e. 2 x 4 blocks of v_mfma_f64_16x16x4f64.
-// Lane layout of that instruction: A[i][k] from lane 16 k + i, B[k][j] from
-// lane 16 k + j, result register r of lane l is D[4 (l / 16) + r][l % 16].
+// Lane layout of that instruction (measured on gfx950): A[i][k] from lane
+// 16 k + i, B[k][j] from lane 16 k + j, result register r of lane l is
+// D[l / 16 + 4 r][l % 16].
 #include <hip/hip_runtime.h>
 
 #include "device_api.h"
@@ -102,7 +103,7 @@ __global__ __launch_bounds__(kGThreads) void k_gemm_rows(const double* __restric
             if (col >= N) continue;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int row = m0 + wr * 32 + i * 16 + 4 * (lane >> 4) + r;
+                const int row = m0 + wr * 32 + i * 16 + (lane >> 4) + 4 * r;
                 C[(size_t)row * ldc + col] = acc[i][j][r];
             }
         }

@@ -195,16 +195,202 @@ double lookup(const std::map<std::vector<int>, double>& c, const std::vector<int
     return it == c.end() ? 0.0 : it->second;
 }
 
+// Group visits by (S, L) and balance the groups over the waves of a
+// workgroup.  The work model of a visit is its terms plus region_cost per
+// m_a (per (m_a, m_b)) region and visit_cost per visit, in units of terms.
+void make_groups(const Model& m, std::vector<PlanVisit> pend, std::vector<GroupQ34>& groups,
+                 std::vector<VisitQ34>& visits, bool adjoint, int& red_total, bool& dyn, int region_cost,
+                 int visit_cost) {
+    // group by (S, L); inside a group, visits by energy class (linear
+    // order 3, linear order 4, polynomial: the adjoint kernel takes each
+    // linear class's energy from the accumulated G row at the class
+    // boundary), then with the same code path (signature) adjacent for
+    // instruction-cache locality
+    auto cls = [](const VisitQ34& v) { return v.pslot >= 0 ? 2 : (v.order == 3 ? 0 : 1); };
+    std::stable_sort(pend.begin(), pend.end(), [&](const PlanVisit& a, const PlanVisit& b) {
+        if (a.S != b.S) return a.S < b.S;
+        if (a.L != b.L) return a.L < b.L;
+        const auto ka = std::make_tuple(cls(a.v), a.v.flags & 4, a.v.order, a.v.la, a.v.lb, a.v.lc);
+        const auto kb = std::make_tuple(cls(b.v), b.v.flags & 4, b.v.order, b.v.la, b.v.lb, b.v.lc);
+        return ka < kb;
+    });
+    for (const PlanVisit& q : pend)
+        if (!(q.v.flags & 4)) dyn = true;
+    // Work model of a visit: its terms plus the per-m_a (per (m_a, m_b))
+    // region and per-visit overheads, in units of terms (3 and 24 were the
+    // best of a sweep at the 256k-atom benchmark model: 40.3 -> 38.8 ms);
+    // of a part: plus loading and storing its G row.
+    auto vcost = [&](const PlanVisit& q) {
+        const long regions = (2L * q.v.la + 1) * (q.v.order == 4 ? 2L * q.v.lb + 1 : 1);
+        return q.cost + region_cost * regions + visit_cost;
+    };
+    struct Row {
+        size_t b, e;
+        int L;
+    };
+    std::vector<Row> rows;
+    for (size_t k = 0; k < pend.size();) {
+        size_t e = k;
+        while (e < pend.size() && pend[e].S == pend[k].S && pend[e].L == pend[k].L) ++e;
+        rows.push_back({k, e, pend[k].L});
+        k = e;
+    }
+    // LDS available for the partial rows of split rows (adjoint only):
+    // what the workgroup's 160 KB leave beside the a_nlm tile
+    const long tile = static_cast<long>(m.n_type) * n_lmh_of(m.lmax3) * 2 * kBlockLanes;
+    const long budget = adjoint ? std::max(0L, 160L * 1024 / 8 - tile) : 1L << 40;
+    auto row_red = [](int L, int np) { return np > 1 ? static_cast<long>(np) * (L + 1) * 2 * kBlockLanes : 0L; };
+    struct Part {
+        size_t row, b, e;
+        long cost;
+    };
+    // split row r into np parts balanced by cost at visit boundaries
+    auto split = [&](size_t r, int np, std::vector<Part>& out) {
+        const Row& R = rows[r];
+        long c = 0;
+        for (size_t j = R.b; j < R.e; ++j) c += vcost(pend[j]);
+        size_t j = R.b;
+        long acc = 0;
+        for (int part = 0; part < np; ++part) {
+            Part P{r, j, j, 3L * (R.L + 1) + 20};
+            if (part == np - 1) {
+                while (j < R.e) P.cost += vcost(pend[j++]);
+            } else {
+                const long goal = c * (part + 1) / np;
+                const size_t keep = static_cast<size_t>(np - 1 - part);
+                do {
+                    const long vc = vcost(pend[j++]);
+                    P.cost += vc;
+                    acc += vc;
+                } while (j < R.e && acc < goal && R.e - j > keep);
+            }
+            P.e = j;
+            out.push_back(P);
+        }
+    };
+    // longest-processing-time assignment, then moves and swaps that lower
+    // the most loaded wave
+    auto assign = [&](const std::vector<Part>& parts, std::vector<int>& wave) {
+        std::vector<size_t> order(parts.size());
+        for (size_t k = 0; k < order.size(); ++k) order[k] = k;
+        std::stable_sort(order.begin(), order.end(),
+                         [&](size_t a, size_t b) { return parts[a].cost > parts[b].cost; });
+        std::vector<long> load(kWavesPerWg, 0);
+        wave.assign(parts.size(), 0);
+        for (size_t k : order) {
+            const int w = static_cast<int>(std::min_element(load.begin(), load.end()) - load.begin());
+            wave[k] = w;
+            load[w] += parts[k].cost;
+        }
+        for (int it = 0; it < 200; ++it) {
+            const int hi = static_cast<int>(std::max_element(load.begin(), load.end()) - load.begin());
+            long best = load[hi];
+            int bk = -1, bk2 = -1, bw = -1;
+            for (size_t k = 0; k < parts.size(); ++k) {
+                if (wave[k] != hi) continue;
+                for (int w = 0; w < kWavesPerWg; ++w) {
+                    if (w == hi) continue;
+                    // move k to w
+                    const long mv = std::max(load[hi] - parts[k].cost, load[w] + parts[k].cost);
+                    if (mv < best) {
+                        best = mv;
+                        bk = static_cast<int>(k);
+                        bk2 = -1;
+                        bw = w;
+                    }
+                    // swap k with a part k2 of w
+                    for (size_t k2 = 0; k2 < parts.size(); ++k2) {
+                        if (wave[k2] != w) continue;
+                        const long d = parts[k].cost - parts[k2].cost;
+                        if (d <= 0) continue;
+                        const long sw = std::max(load[hi] - d, load[w] + d);
+                        if (sw < best) {
+                            best = sw;
+                            bk = static_cast<int>(k);
+                            bk2 = static_cast<int>(k2);
+                            bw = w;
+                        }
+                    }
+                }
+            }
+            if (bk < 0) break;
+            load[hi] -= parts[bk].cost;
+            load[bw] += parts[bk].cost;
+            wave[bk] = bw;
+            if (bk2 >= 0) {
+                load[bw] -= parts[bk2].cost;
+                load[hi] += parts[bk2].cost;
+                wave[bk2] = hi;
+            }
+        }
+        return *std::max_element(load.begin(), load.end());
+    };
+    std::vector<int> nparts(rows.size(), 1);
+    auto build = [&](std::vector<Part>& parts) {
+        parts.clear();
+        for (size_t r = 0; r < rows.size(); ++r) split(r, nparts[r], parts);
+    };
+    std::vector<Part> parts;
+    std::vector<int> wave;
+    build(parts);
+    long best = assign(parts, wave);
+    long red_used = 0;
+    // split one row one part further while that lowers the most loaded
+    // wave and the partial rows fit in LDS
+    for (;;) {
+        long cand_best = best;
+        size_t cand = rows.size();
+        for (size_t r = 0; r < rows.size(); ++r) {
+            const Row& R = rows[r];
+            if (static_cast<size_t>(nparts[r]) >= R.e - R.b || nparts[r] >= kWavesPerWg) continue;
+            const long red_new = red_used - row_red(R.L, nparts[r]) + row_red(R.L, nparts[r] + 1);
+            if (red_new > budget) continue;
+            ++nparts[r];
+            std::vector<Part> p2;
+            std::vector<int> w2;
+            build(p2);
+            const long mx = assign(p2, w2);
+            --nparts[r];
+            if (mx < cand_best) {
+                cand_best = mx;
+                cand = r;
+            }
+        }
+        if (cand == rows.size()) break;
+        red_used += row_red(rows[cand].L, nparts[cand] + 1) - row_red(rows[cand].L, nparts[cand]);
+        ++nparts[cand];
+        build(parts);
+        best = assign(parts, wave);
+    }
+    // emit: parts in row order, partial-row offsets per split row
+    int red = 0;
+    size_t pi = 0;
+    for (size_t r = 0; r < rows.size(); ++r) {
+        const int np = nparts[r];
+        for (int part = 0; part < np; ++part, ++pi) {
+            const Part& P = parts[pi];
+            GroupQ34 g{};
+            g.S = pend[rows[r].b].S;
+            g.L = rows[r].L;
+            g.part = part;
+            g.nparts = np;
+            g.red = (adjoint && np > 1) ? red + part * (rows[r].L + 1) * 2 * kBlockLanes : 0;
+            g.vbeg = static_cast<int>(visits.size());
+            for (size_t j = P.b; j < P.e; ++j) visits.push_back(pend[j].v);
+            g.vend = static_cast<int>(visits.size());
+            g.wave = wave[pi];
+            groups.push_back(g);
+        }
+        if (adjoint && np > 1) red += static_cast<int>(row_red(rows[r].L, np));
+    }
+    red_total = std::max(red_total, red);
+}
+
 // Builds the flattened visit tables for centre type t.
 void build_type_tables(Model& m, int t, const std::vector<std::map<std::vector<int>, double>>& cmap) {
     TypeTables tt;
     std::vector<std::vector<VisitQ12>> by_l(m.maxl + 1);
-    struct Pending {
-        int S, L;
-        VisitQ34 v;
-        long cost;
-        int clen;  // length of the visit's coefficient block
-    };
+    using Pending = PlanVisit;
     std::vector<Pending> adj, feat;
     auto push_coef = [&](double v) { m.coef.push_back(v); };
     for (int i = 0; i < m.n_gtinv; ++i) {
@@ -362,193 +548,6 @@ void build_type_tables(Model& m, int t, const std::vector<std::map<std::vector<i
         tt.visits_q12.insert(tt.visits_q12.end(), by_l[L].begin(), by_l[L].end());
     }
     tt.q12_off[m.maxl + 1] = static_cast<int>(tt.visits_q12.size());
-    // group by (S, L), balance groups over the waves of a workgroup
-    auto make_groups = [&](std::vector<Pending>& pend, std::vector<GroupQ34>& groups,
-                           std::vector<VisitQ34>& visits, bool adjoint, int& red_total, bool& dyn) {
-        // group by (S, L); inside a group, visits by energy class (linear
-        // order 3, linear order 4, polynomial: the adjoint kernel takes each
-        // linear class's energy from the accumulated G row at the class
-        // boundary), then with the same code path (signature) adjacent for
-        // instruction-cache locality
-        auto cls = [](const VisitQ34& v) { return v.pslot >= 0 ? 2 : (v.order == 3 ? 0 : 1); };
-        std::stable_sort(pend.begin(), pend.end(), [&](const Pending& a, const Pending& b) {
-            if (a.S != b.S) return a.S < b.S;
-            if (a.L != b.L) return a.L < b.L;
-            const auto ka = std::make_tuple(cls(a.v), a.v.flags & 4, a.v.order, a.v.la, a.v.lb, a.v.lc);
-            const auto kb = std::make_tuple(cls(b.v), b.v.flags & 4, b.v.order, b.v.la, b.v.lb, b.v.lc);
-            return ka < kb;
-        });
-        for (const Pending& q : pend)
-            if (!(q.v.flags & 4)) dyn = true;
-        // Work model of a visit: its terms plus the per-m_a (per (m_a, m_b))
-        // region and per-visit overheads, in units of terms (3 and 24 were the
-        // best of a sweep at the 256k-atom benchmark model: 40.3 -> 38.8 ms);
-        // of a part: plus loading and storing its G row.
-        auto vcost = [](const Pending& q) {
-            const long regions = (2L * q.v.la + 1) * (q.v.order == 4 ? 2L * q.v.lb + 1 : 1);
-            return q.cost + 3 * regions + 24;
-        };
-        struct Row {
-            size_t b, e;
-            int L;
-        };
-        std::vector<Row> rows;
-        for (size_t k = 0; k < pend.size();) {
-            size_t e = k;
-            while (e < pend.size() && pend[e].S == pend[k].S && pend[e].L == pend[k].L) ++e;
-            rows.push_back({k, e, pend[k].L});
-            k = e;
-        }
-        // LDS available for the partial rows of split rows (adjoint only):
-        // what the workgroup's 160 KB leave beside the a_nlm tile
-        const long tile = static_cast<long>(m.n_type) * n_lmh_of(m.lmax3) * 2 * kBlockLanes;
-        const long budget = adjoint ? std::max(0L, 160L * 1024 / 8 - tile) : 1L << 40;
-        auto row_red = [](int L, int np) { return np > 1 ? static_cast<long>(np) * (L + 1) * 2 * kBlockLanes : 0L; };
-        struct Part {
-            size_t row, b, e;
-            long cost;
-        };
-        // split row r into np parts balanced by cost at visit boundaries
-        auto split = [&](size_t r, int np, std::vector<Part>& out) {
-            const Row& R = rows[r];
-            long c = 0;
-            for (size_t j = R.b; j < R.e; ++j) c += vcost(pend[j]);
-            size_t j = R.b;
-            long acc = 0;
-            for (int part = 0; part < np; ++part) {
-                Part P{r, j, j, 3L * (R.L + 1) + 20};
-                if (part == np - 1) {
-                    while (j < R.e) P.cost += vcost(pend[j++]);
-                } else {
-                    const long goal = c * (part + 1) / np;
-                    const size_t keep = static_cast<size_t>(np - 1 - part);
-                    do {
-                        const long vc = vcost(pend[j++]);
-                        P.cost += vc;
-                        acc += vc;
-                    } while (j < R.e && acc < goal && R.e - j > keep);
-                }
-                P.e = j;
-                out.push_back(P);
-            }
-        };
-        // longest-processing-time assignment, then moves and swaps that lower
-        // the most loaded wave
-        auto assign = [&](const std::vector<Part>& parts, std::vector<int>& wave) {
-            std::vector<size_t> order(parts.size());
-            for (size_t k = 0; k < order.size(); ++k) order[k] = k;
-            std::stable_sort(order.begin(), order.end(),
-                             [&](size_t a, size_t b) { return parts[a].cost > parts[b].cost; });
-            std::vector<long> load(kWavesPerWg, 0);
-            wave.assign(parts.size(), 0);
-            for (size_t k : order) {
-                const int w = static_cast<int>(std::min_element(load.begin(), load.end()) - load.begin());
-                wave[k] = w;
-                load[w] += parts[k].cost;
-            }
-            for (int it = 0; it < 200; ++it) {
-                const int hi = static_cast<int>(std::max_element(load.begin(), load.end()) - load.begin());
-                long best = load[hi];
-                int bk = -1, bk2 = -1, bw = -1;
-                for (size_t k = 0; k < parts.size(); ++k) {
-                    if (wave[k] != hi) continue;
-                    for (int w = 0; w < kWavesPerWg; ++w) {
-                        if (w == hi) continue;
-                        // move k to w
-                        const long mv = std::max(load[hi] - parts[k].cost, load[w] + parts[k].cost);
-                        if (mv < best) {
-                            best = mv;
-                            bk = static_cast<int>(k);
-                            bk2 = -1;
-                            bw = w;
-                        }
-                        // swap k with a part k2 of w
-                        for (size_t k2 = 0; k2 < parts.size(); ++k2) {
-                            if (wave[k2] != w) continue;
-                            const long d = parts[k].cost - parts[k2].cost;
-                            if (d <= 0) continue;
-                            const long sw = std::max(load[hi] - d, load[w] + d);
-                            if (sw < best) {
-                                best = sw;
-                                bk = static_cast<int>(k);
-                                bk2 = static_cast<int>(k2);
-                                bw = w;
-                            }
-                        }
-                    }
-                }
-                if (bk < 0) break;
-                load[hi] -= parts[bk].cost;
-                load[bw] += parts[bk].cost;
-                wave[bk] = bw;
-                if (bk2 >= 0) {
-                    load[bw] -= parts[bk2].cost;
-                    load[hi] += parts[bk2].cost;
-                    wave[bk2] = hi;
-                }
-            }
-            return *std::max_element(load.begin(), load.end());
-        };
-        std::vector<int> nparts(rows.size(), 1);
-        auto build = [&](std::vector<Part>& parts) {
-            parts.clear();
-            for (size_t r = 0; r < rows.size(); ++r) split(r, nparts[r], parts);
-        };
-        std::vector<Part> parts;
-        std::vector<int> wave;
-        build(parts);
-        long best = assign(parts, wave);
-        long red_used = 0;
-        // split one row one part further while that lowers the most loaded
-        // wave and the partial rows fit in LDS
-        for (;;) {
-            long cand_best = best;
-            size_t cand = rows.size();
-            for (size_t r = 0; r < rows.size(); ++r) {
-                const Row& R = rows[r];
-                if (static_cast<size_t>(nparts[r]) >= R.e - R.b || nparts[r] >= kWavesPerWg) continue;
-                const long red_new = red_used - row_red(R.L, nparts[r]) + row_red(R.L, nparts[r] + 1);
-                if (red_new > budget) continue;
-                ++nparts[r];
-                std::vector<Part> p2;
-                std::vector<int> w2;
-                build(p2);
-                const long mx = assign(p2, w2);
-                --nparts[r];
-                if (mx < cand_best) {
-                    cand_best = mx;
-                    cand = r;
-                }
-            }
-            if (cand == rows.size()) break;
-            red_used += row_red(rows[cand].L, nparts[cand] + 1) - row_red(rows[cand].L, nparts[cand]);
-            ++nparts[cand];
-            build(parts);
-            best = assign(parts, wave);
-        }
-        // emit: parts in row order, partial-row offsets per split row
-        int red = 0;
-        size_t pi = 0;
-        for (size_t r = 0; r < rows.size(); ++r) {
-            const int np = nparts[r];
-            for (int part = 0; part < np; ++part, ++pi) {
-                const Part& P = parts[pi];
-                GroupQ34 g{};
-                g.S = pend[rows[r].b].S;
-                g.L = rows[r].L;
-                g.part = part;
-                g.nparts = np;
-                g.red = (adjoint && np > 1) ? red + part * (rows[r].L + 1) * 2 * kBlockLanes : 0;
-                g.vbeg = static_cast<int>(visits.size());
-                for (size_t j = P.b; j < P.e; ++j) visits.push_back(pend[j].v);
-                g.vend = static_cast<int>(visits.size());
-                g.wave = wave[pi];
-                groups.push_back(g);
-            }
-            if (adjoint && np > 1) red += static_cast<int>(row_red(rows[r].L, np));
-        }
-        red_total = std::max(red_total, red);
-    };
     // Factor positions of one family that carry the same (species, l) and
     // see the same other factors in the same order add the same products
     // into the same G row, term for term: merge them into one visit whose
@@ -570,9 +569,8 @@ void build_type_tables(Model& m, int t, const std::vector<std::map<std::vector<i
         }
         adj.swap(merged);
     }
-    int feat_red = 0;
-    make_groups(adj, tt.groups_adj, tt.visits_adj, true, m.q34_red, m.q34_dyn_adj);
-    make_groups(feat, tt.groups_feat, tt.visits_feat, false, feat_red, m.q34_dyn_feat);
+    tt.plan_adj = std::move(adj);
+    tt.plan_feat = std::move(feat);
     m.types[t] = std::move(tt);
 }
 
@@ -939,10 +937,29 @@ bool load_model(const std::string& path, const GtinvTable& table, Model& m, std:
     m.coef.clear();
     m.types.assign(m.n_type, TypeTables());
     for (int t = 0; t < m.n_type; ++t) build_type_tables(m, t, cmap);
+    plan_q34(m, m.plan_region, m.plan_visit);
     m.has_feat34 = false;
     for (const auto& tt : m.types)
         if (!tt.visits_feat.empty()) m.has_feat34 = true;
     return true;
+}
+
+void plan_q34(Model& m, int region_cost, int visit_cost) {
+    m.plan_region = region_cost;
+    m.plan_visit = visit_cost;
+    m.q34_red = 0;
+    m.q34_dyn_adj = m.q34_dyn_feat = false;
+    for (TypeTables& tt : m.types) {
+        tt.groups_adj.clear();
+        tt.visits_adj.clear();
+        tt.groups_feat.clear();
+        tt.visits_feat.clear();
+        int feat_red = 0;
+        make_groups(m, tt.plan_adj, tt.groups_adj, tt.visits_adj, true, m.q34_red, m.q34_dyn_adj, region_cost,
+                    visit_cost);
+        make_groups(m, tt.plan_feat, tt.groups_feat, tt.visits_feat, false, feat_red, m.q34_dyn_feat, region_cost,
+                    visit_cost);
+    }
 }
 
 }  // namespace mlip

@@ -123,6 +123,15 @@ struct PairHEntry {
 
 constexpr int kMaxFnPair = 32;  // radial functions of the radial-only style
 
+// An order-3/4 visit before grouping: output row (S, L), the visit, its
+// term count and the length of its coefficient block.
+struct PlanVisit {
+    int S, L;
+    VisitQ34 v;
+    long cost;
+    int clen;
+};
+
 struct TypeTables {
     // order 1/2: per L, range into visits_q12
     std::vector<int32_t> q12_off;      // size maxl+2
@@ -132,6 +141,8 @@ struct TypeTables {
     std::vector<VisitQ34> visits_adj, visits_feat;
     // orders 5/6: adjoint and polynomial-feature visits
     std::vector<VisitHi> hi_adj, hi_feat;
+    // order 3/4 visits before grouping (plan_q34 groups them)
+    std::vector<PlanVisit> plan_adj, plan_feat;
 };
 
 struct Model {
@@ -181,7 +192,15 @@ struct Model {
     bool has_hi = false;
     bool q34_dyn_adj = false;  // visits outside the unrolled signatures (general loop kernel)
     bool q34_dyn_feat = false;
+    // work model of the order-3/4 workgroup plan (terms per scheduling
+    // region and per visit); 3 and 24 were the best of a sweep at the 256k
+    // benchmark model, mlip_autotune_q34 times alternatives on the device
+    int plan_region = 3, plan_visit = 24;
 };
+
+// (Re)group the order-3/4 visits of every centre type over the waves of the
+// order-3/4 workgroup with the given work model.
+void plan_q34(Model& m, int region_cost, int visit_cost);
 
 // Parse a potential file (format of mlip_model.h:45-101) and build every
 // table.  Returns false with a message instead of exiting.

@@ -88,27 +88,33 @@ __device__ __forceinline__ PairDel pair_del(const DevModel& M, const DevSystem& 
 // registers, n0 = blockIdx.y * ngroup * NPT (models with more radial
 // functions than one pass holds run several passes over the neighbors).
 constexpr int kAnlmJ = 16;
-constexpr int kAnlmGW = kMaxFn + 16;  // g_n row width: every pass's NPT-wide read stays in the row
 static_assert(kAnlmJ <= 64, "a chunk's neighbor species fit one per lane of a wave");
+// Size classes of the per-workgroup tables: models with l_max <= 12 whose
+// radial passes read at most 16 g_n per neighbor keep the compact tables
+// (the LDS they take sets how many workgroups share a CU); larger models use
+// the full l_max 20 tables and 48-wide g_n rows (every pass's NPT-wide read
+// stays inside the row).
+constexpr int kSmallL1 = 13, kSmallGW = 16;
+constexpr int kLargeGW = kMaxFn + 16;
 
-template <typename T, int NTH>
+template <typename T, int NTH, int KL1, int GW>
 struct AnlmLds {
-    T diag[kL1 + 1];
-    T alm[kL1][kL1];
-    T ainv[kL1][kL1];
+    T diag[KL1 + 1];
+    T alm[KL1][KL1];
+    T ainv[KL1][KL1];
     double beta[kMaxFn], mu[kMaxFn];
     T ux[NTH], uy[NTH], uz[NTH];
     double rr[NTH], fc[NTH];
     int spec[NTH];
-    alignas(16) double g[kAnlmJ][kAnlmGW];
+    alignas(16) double g[kAnlmJ][GW];
 };
 
-template <int NPT, int NTH, typename T>
+template <int NPT, int NTH, typename T, int KL1, int GW>
 __global__ __launch_bounds__(NTH) void k_anlm(DevModel M, DevSystem S, int ngroup) {
     constexpr int kAnlmThreads = NTH;
     constexpr int kAnlmParts = NTH / kAnlmJ;
     using T2 = typename Vec2<T>::type;
-    __shared__ AnlmLds<T, NTH> L;
+    __shared__ AnlmLds<T, NTH, KL1, GW> L;
     extern __shared__ unsigned char anlm_dyn[];
     T2* ys = reinterpret_cast<T2*>(anlm_dyn);  // [kAnlmJ][n_lmh]
     const int slot = xcd_slot(blockIdx.x, S.nblocks);
@@ -119,11 +125,12 @@ __global__ __launch_bounds__(NTH) void k_anlm(DevModel M, DevSystem S, int ngrou
     const int nfn = M.n_fn, nlmh = M.n_lmh, maxl = M.maxl;
     {
         const YTab<T>* yt = ytab_of<T>(M);
-        for (int k = tid; k < kL1 * kL1; k += kAnlmThreads) {
-            (&L.alm[0][0])[k] = (&yt->alm[0][0])[k];
-            (&L.ainv[0][0])[k] = (&yt->ainv[0][0])[k];
+        for (int k = tid; k < KL1 * KL1; k += kAnlmThreads) {
+            const int l = k / KL1, mm = k % KL1;
+            L.alm[l][mm] = yt->alm[l][mm];
+            L.ainv[l][mm] = yt->ainv[l][mm];
         }
-        if (tid <= kL1) L.diag[tid] = yt->diag[tid];
+        if (tid <= KL1) L.diag[tid] = yt->diag[tid];
         if (tid < kMaxFn) {
             L.beta[tid] = tid < nfn ? M.beta[tid] : 0.0;
             L.mu[tid] = tid < nfn ? M.mu[tid] : 0.0;
@@ -164,7 +171,7 @@ __global__ __launch_bounds__(NTH) void k_anlm(DevModel M, DevSystem S, int ngrou
                 // radial functions in FP64 in both modes: FP32 cannot carry
                 // the exponent beta (r - mu)^2 to the digits exp needs
                 const double r = L.rr[q], fcq = L.fc[q];
-                for (int n = part; n < kAnlmGW; n += kAnlmParts) {
+                for (int n = part; n < GW; n += kAnlmParts) {
                     const double d = n < nfn ? r - L.mu[n] : 0.0;
                     L.g[j1][n] = n < nfn ? exp(-L.beta[n] * (d * d)) * fcq : 0.0;
                 }
@@ -253,6 +260,194 @@ __global__ __launch_bounds__(NTH) void k_anlm(DevModel M, DevSystem S, int ngrou
     }
 }
 
+// ---------------------------------------------------------------- a_nlm on the matrix cores
+// The same sum as a product per chunk of 16 neighbors:
+//   a_s[n][(lm, re/im)] += sum_k At[k][n] Y[k][(lm, re/im)],
+// k = the chunk's neighbors with species s, on v_mfma_f64_16x16x4f64 (A[i][k]
+// from lane 16 k + i, B[k][j] from lane 16 k + j, result register r of lane
+// l is D[l / 16 + 4 r][l % 16]).  Phase 1 puts each chunk's neighbors in LDS
+// sorted by species (species 0 first), so a step of four neighbors feeds one
+// species' accumulators (two for the step that straddles the boundary, with
+// the other species' columns masked to zero) and no lane branches on species.
+// One workgroup of four waves per centre; wave w owns the column tiles
+// w, w + 4, ... and both species' accumulators for them (MT tiles of 16 radial
+// indices each).  Per-pair geometry, g_n and Y_lm are computed as in k_anlm
+// (radial functions in FP64, Y_lm in T); the product and its accumulation
+// are FP64.
+constexpr int kAmThreads = 256;
+constexpr int kAmJ = 16;
+
+template <typename T, int MT, int NTW>
+struct AmLds {
+    static constexpr int kCols = 64 * NTW;      // 4 waves x NTW tiles x 16 columns
+    static constexpr int kYPitch = kCols + 2;   // doubles; offsets the rows' banks
+    static constexpr int kAPitch = 16 * MT + 2;
+    T diag[kSmallL1 + 1];
+    T alm[kSmallL1][kSmallL1];
+    T ainv[kSmallL1][kSmallL1];
+    double beta[kMaxFn], mu[kMaxFn];
+    T ux[kAmThreads], uy[kAmThreads], uz[kAmThreads];
+    double rr[kAmThreads], fc[kAmThreads];
+    int spec[kAmThreads];
+    alignas(16) double At[kAmJ][kAPitch];   // [sorted neighbor][n]
+    alignas(16) double Y[kAmJ][kYPitch];    // [sorted neighbor][(lm, re/im)]
+};
+
+template <typename T, int MT, int NTW>
+__global__ __launch_bounds__(kAmThreads) void k_anlm_mfma(DevModel M, DevSystem S) {
+    using L_t = AmLds<T, MT, NTW>;
+    constexpr int kCols = L_t::kCols;
+    __shared__ L_t L;
+    const int slot = xcd_slot(blockIdx.x, S.nblocks);
+    if (slot < 0) return;
+    const int atom = S.atom_of_slot[slot];
+    if (atom < 0) return;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int nfn = M.n_fn, nlmh = M.n_lmh, maxl = M.maxl;
+    {
+        const YTab<T>* yt = ytab_of<T>(M);
+        for (int k = tid; k < kSmallL1 * kSmallL1; k += kAmThreads) {
+            const int l = k / kSmallL1, mm = k % kSmallL1;
+            L.alm[l][mm] = yt->alm[l][mm];
+            L.ainv[l][mm] = yt->ainv[l][mm];
+        }
+        if (tid <= kSmallL1) L.diag[tid] = yt->diag[tid];
+        if (tid < kMaxFn) {
+            L.beta[tid] = tid < nfn ? M.beta[tid] : 0.0;
+            L.mu[tid] = tid < nfn ? M.mu[tid] : 0.0;
+        }
+        // Y columns past 2 n_lmh stay zero for the whole kernel
+        for (int k = tid; k < kAmJ * kCols; k += kAmThreads) L.Y[k / kCols][k % kCols] = 0.0;
+    }
+    typedef double dbl4 __attribute__((ext_vector_type(4)));
+    dbl4 acc[2][MT][NTW];
+#pragma unroll
+    for (int s = 0; s < 2; ++s)
+#pragma unroll
+        for (int i = 0; i < MT; ++i)
+#pragma unroll
+            for (int t = 0; t < NTW; ++t) acc[s][i][t] = dbl4{0.0, 0.0, 0.0, 0.0};
+    const int p0 = S.poff[atom], p1 = S.pend[atom];
+    const double pir = kPiRef / M.cutoff;
+    const int j1 = tid >> 4, part = tid & 15;  // phase 1: neighbor of the chunk, part within it
+    __syncthreads();
+    for (int b0 = p0; b0 < p1; b0 += kAmThreads) {
+        const int nb = min(kAmThreads, p1 - b0);
+        if (tid < nb) {
+            const PairDel g = pair_del(M, S, atom, b0 + tid);
+            int spec = -1;
+            if (g.valid) {
+                spec = S.elem[S.pnbr[b0 + tid]];
+                const double ir = 1.0 / g.r;
+                L.ux[tid] = static_cast<T>(g.dx * ir);
+                L.uy[tid] = static_cast<T>(g.dy * ir);
+                L.uz[tid] = static_cast<T>(g.dz * ir);
+                L.rr[tid] = g.r;
+                L.fc[tid] = 0.5 * (cos(pir * g.r) + 1.0);
+            }
+            L.spec[tid] = spec;
+        }
+        __syncthreads();
+        for (int c0 = 0; c0 < nb; c0 += kAmJ) {
+            const int nc = min(kAmJ, nb - c0);
+            // sorted order of the chunk: species 0, then species 1; pairs
+            // beyond the cutoff get no place
+            const int specv = lane < nc ? L.spec[c0 + lane] : -1;
+            const uint64_t m0 = __ballot(specv == 0), m1 = __ballot(specv == 1);
+            const int n0 = __popcll(m0), nval = n0 + __popcll(m1);
+            // phase 1: g_n and Y of neighbor j1 into its sorted row
+            {
+                const int q = c0 + j1;
+                const int sp = j1 < nc ? L.spec[q] : -1;
+                const uint64_t below = (1ull << j1) - 1ull;
+                const int k = sp == 0 ? __popcll(m0 & below) : (sp == 1 ? n0 + __popcll(m1 & below) : -1);
+                if (k >= 0) {
+                    const double r = L.rr[q], fcq = L.fc[q];
+                    for (int n = part; n < 16 * MT; n += 16) {
+                        const double d = n < nfn ? r - L.mu[n] : 0.0;
+                        L.At[k][n] = n < nfn ? exp(-L.beta[n] * (d * d)) * fcq : 0.0;
+                    }
+                    const T x = L.ux[q], y = L.uy[q], z = L.uz[q];
+                    double* yrow = L.Y[k];
+                    for (int m = part; m <= maxl; m += 16) {
+                        T pr = (m & 1) ? T(-1) : T(1), pi = T(0);  // (-1)^m conj(u)^m
+                        T br = x, bi = -y;
+                        for (int e = m; e; e >>= 1) {
+                            if (e & 1) {
+                                const T t = pr * br - pi * bi;
+                                pi = pr * bi + pi * br;
+                                pr = t;
+                            }
+                            const T t2 = br * br - bi * bi;
+                            bi = T(2) * br * bi;
+                            br = t2;
+                        }
+                        T q2 = T(0), q1 = L.diag[m];
+                        int idx = m * (m + 1) / 2;  // lmh(m, -m)
+                        yrow[2 * idx] = static_cast<double>(q1 * pr);
+                        yrow[2 * idx + 1] = static_cast<double>(q1 * pi);
+                        for (int l = m + 1; l <= maxl; ++l) {
+                            const T qn = L.alm[l][m] * (z * q1 - L.ainv[l][m] * q2);
+                            idx += l + 1;  // lmh(l, -m) = lmh(l-1, -m) + l + 1
+                            yrow[2 * idx] = static_cast<double>(qn * pr);
+                            yrow[2 * idx + 1] = static_cast<double>(qn * pi);
+                            q2 = q1;
+                            q1 = qn;
+                        }
+                    }
+                }
+                // rows past the chunk's valid neighbors are zero
+                for (int e = tid; e < (kAmJ - nval) * 16 * MT; e += kAmThreads)
+                    L.At[nval + e / (16 * MT)][e % (16 * MT)] = 0.0;
+                for (int e = tid; e < (kAmJ - nval) * 2 * nlmh; e += kAmThreads)
+                    L.Y[nval + e / (2 * nlmh)][e % (2 * nlmh)] = 0.0;
+            }
+            __syncthreads();
+            // phase 2: four steps of four sorted neighbors
+            for (int ks = 0; ks < kAmJ / 4; ++ks) {
+                const int kb = 4 * ks;
+                if (kb >= nval) break;
+                const int kr = kb + (lane >> 4);
+                const bool s0 = kb < n0, s1 = kb + 3 >= n0;  // species present in the step
+                double a0[MT], a1[MT];
+#pragma unroll
+                for (int i = 0; i < MT; ++i) {
+                    const double a = L.At[kr][16 * i + (lane & 15)];
+                    a0[i] = kr < n0 ? a : 0.0;
+                    a1[i] = kr < n0 ? 0.0 : a;
+                }
+#pragma unroll
+                for (int t = 0; t < NTW; ++t) {
+                    const double b = L.Y[kr][(wave * NTW + t) * 16 + (lane & 15)];
+#pragma unroll
+                    for (int i = 0; i < MT; ++i) {
+                        if (s0) acc[0][i][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[i], b, acc[0][i][t], 0, 0, 0);
+                        if (s1) acc[1][i][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[i], b, acc[1][i][t], 0, 0, 0);
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    const int blk = slot / kBlock, sl = slot % kBlock;
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+        if (s >= M.n_type) break;
+#pragma unroll
+        for (int i = 0; i < MT; ++i)
+#pragma unroll
+            for (int t = 0; t < NTW; ++t) {
+                const int col = (wave * NTW + t) * 16 + (lane & 15);
+                if (col >= 2 * nlmh) continue;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int n = 16 * i + (lane >> 4) + 4 * r;
+                    if (n < nfn) S.A[a_index(M, blk, n, s, col >> 1, col & 1, sl)] = acc[s][i][t][r];
+                }
+            }
+    }
+}
+
 // ---------------------------------------------------------------- forces
 // One workgroup per centre; G of the centre (both species) staged in LDS as
 // T [s][lm][n][re,im] with n padded to NF (zeros).  Thread = neighbor pair;
@@ -264,13 +459,13 @@ __global__ __launch_bounds__(NTH) void k_anlm(DevModel M, DevSystem S, int ngrou
 //   A = m z Q + (1-z^2) c Q', B = m Q - z c Q', E = m (1+z^2) Q + z (1-z^2) c Q',
 //   Q = Qb_l^m, Q' = Qb_l^{m+1}, c = sqrt((l+m+1)(l-m)); for m = 0 the
 //   singular terms cancel to c Q' (z x, z y, -(1-z^2)).
-template <typename T>
+template <typename T, int KL1>
 struct ForceLds {
     double beta[kMaxFn], mu[kMaxFn];
-    T diag[kL1 + 1];
-    T alm[kL1][kL1];
-    T ainv[kL1][kL1];
-    T grad[kL1][kL1];
+    T diag[KL1 + 1];
+    T alm[KL1][KL1];
+    T ainv[KL1][KL1];
+    T grad[KL1][KL1];
 };
 
 // Register budget for occupancy: FP64 at most 168 VGPRs (3 waves per SIMD;
@@ -282,11 +477,11 @@ struct ForceLds {
 // false; every thread of the workgroup reads the same entries, served by the
 // caches), and models with more than 16 radial functions keep the register
 // arrays they need (no occupancy cap).
-template <int NF, int NT, typename T, bool GL>
+template <int NF, int NT, typename T, bool GL, int KL1>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NF > 16 ? 1 : (sizeof(T) == 4 ? 4 : 3)))) void k_force(DevModel M, DevSystem S) {
     extern __shared__ __attribute__((aligned(16))) unsigned char force_dyn[];
     T* Gs = reinterpret_cast<T*>(force_dyn);  // [ns][n_lmh][NF][2]
-    __shared__ ForceLds<T> C;
+    __shared__ ForceLds<T, KL1> C;
     __shared__ double vred[6][NT / 64];
     const int slot = xcd_slot(blockIdx.x, S.nblocks);
     if (slot < 0) return;
@@ -308,12 +503,13 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NF > 16 ? 1 
     }
     {
         const YTab<T>* yt = ytab_of<T>(M);
-        for (int k = threadIdx.x; k < kL1 * kL1; k += NT) {
-            (&C.alm[0][0])[k] = (&yt->alm[0][0])[k];
-            (&C.ainv[0][0])[k] = (&yt->ainv[0][0])[k];
-            (&C.grad[0][0])[k] = (&yt->grad[0][0])[k];
+        for (int k = threadIdx.x; k < KL1 * KL1; k += NT) {
+            const int l = k / KL1, mm = k % KL1;
+            C.alm[l][mm] = yt->alm[l][mm];
+            C.ainv[l][mm] = yt->ainv[l][mm];
+            C.grad[l][mm] = yt->grad[l][mm];
         }
-        if (threadIdx.x <= kL1) C.diag[threadIdx.x] = yt->diag[threadIdx.x];
+        if (threadIdx.x <= KL1) C.diag[threadIdx.x] = yt->diag[threadIdx.x];
         if (threadIdx.x < kMaxFn) {
             C.beta[threadIdx.x] = threadIdx.x < nfn ? M.beta[threadIdx.x] : 0.0;
             C.mu[threadIdx.x] = threadIdx.x < nfn ? M.mu[threadIdx.x] : 0.0;
@@ -595,70 +791,110 @@ int anlm_npt(const DevModel& m, int nth, int* ngroup, int* passes) {
     return 16;
 }
 
-template <int NTH, typename T>
-hipError_t launch_anlm_nth(const DevModel& m, const DevSystem& s, hipStream_t st) {
-    int ngroup = 1, passes = 1;
-    const int npt = anlm_npt(m, NTH, &ngroup, &passes);
+template <int NTH, typename T, int KL1, int GW>
+hipError_t launch_anlm_nth(const DevModel& m, const DevSystem& s, int npt, int ngroup, int passes, hipStream_t st) {
     const size_t lds = sizeof(typename Vec2<T>::type) * kAnlmJ * m.n_lmh;
-    if (lds + sizeof(AnlmLds<T, NTH>) > lds_limit()) return hipErrorInvalidConfiguration;
+    if (lds + sizeof(AnlmLds<T, NTH, KL1, GW>) > lds_limit()) return hipErrorInvalidConfiguration;
     const dim3 grid(slot_grid(s), passes), block(NTH);
     switch (npt) {
-        case 2: hipLaunchKernelGGL((k_anlm<2, NTH, T>), grid, block, lds, st, m, s, ngroup); break;
-        case 4: hipLaunchKernelGGL((k_anlm<4, NTH, T>), grid, block, lds, st, m, s, ngroup); break;
-        case 6: hipLaunchKernelGGL((k_anlm<6, NTH, T>), grid, block, lds, st, m, s, ngroup); break;
-        case 8: hipLaunchKernelGGL((k_anlm<8, NTH, T>), grid, block, lds, st, m, s, ngroup); break;
-        default: hipLaunchKernelGGL((k_anlm<16, NTH, T>), grid, block, lds, st, m, s, ngroup); break;
+        case 2: hipLaunchKernelGGL((k_anlm<2, NTH, T, KL1, GW>), grid, block, lds, st, m, s, ngroup); break;
+        case 4: hipLaunchKernelGGL((k_anlm<4, NTH, T, KL1, GW>), grid, block, lds, st, m, s, ngroup); break;
+        case 6: hipLaunchKernelGGL((k_anlm<6, NTH, T, KL1, GW>), grid, block, lds, st, m, s, ngroup); break;
+        case 8: hipLaunchKernelGGL((k_anlm<8, NTH, T, KL1, GW>), grid, block, lds, st, m, s, ngroup); break;
+        default: hipLaunchKernelGGL((k_anlm<16, NTH, T, KL1, GW>), grid, block, lds, st, m, s, ngroup); break;
     }
     return launched();
+}
+
+template <typename T, int MT, int NTW>
+hipError_t launch_anlm_mfma_k(const DevModel& m, const DevSystem& s, hipStream_t st) {
+    if (sizeof(AmLds<T, MT, NTW>) > lds_limit()) return hipErrorInvalidConfiguration;
+    hipLaunchKernelGGL((k_anlm_mfma<T, MT, NTW>), dim3(slot_grid(s)), dim3(kAmThreads), 0, st, m, s);
+    return launched();
+}
+
+// the matrix-core a_nlm kernel takes l_max <= 12, two species and up to 32
+// radial functions (MT = 1 or 2 tiles of 16), with at most three column
+// tiles per wave (2 n_lmh <= 192 columns)
+bool anlm_mfma_fits(const DevModel& m) {
+    const int nt = (2 * m.n_lmh + 15) / 16;
+    return m.maxl < kSmallL1 && m.n_type <= 2 && m.n_fn <= 32 && (nt + 3) / 4 <= 3;
+}
+
+template <typename T>
+hipError_t launch_anlm_mfma_t(const DevModel& m, const DevSystem& s, hipStream_t st) {
+    const int mt = (m.n_fn + 15) / 16, ntw = ((2 * m.n_lmh + 15) / 16 + 3) / 4;
+    if (mt == 1) {
+        if (ntw == 1) return launch_anlm_mfma_k<T, 1, 1>(m, s, st);
+        if (ntw == 2) return launch_anlm_mfma_k<T, 1, 2>(m, s, st);
+        return launch_anlm_mfma_k<T, 1, 3>(m, s, st);
+    }
+    if (ntw == 1) return launch_anlm_mfma_k<T, 2, 1>(m, s, st);
+    if (ntw == 2) return launch_anlm_mfma_k<T, 2, 2>(m, s, st);
+    return launch_anlm_mfma_k<T, 2, 3>(m, s, st);
 }
 
 template <typename T>
 hipError_t launch_anlm_t(const DevModel& m, const DevSystem& s, hipStream_t st) {
     // phase 2 needs a thread per lm column: 192 threads up to l_max 18
-    return m.n_lmh <= 192 ? launch_anlm_nth<192, T>(m, s, st) : launch_anlm_nth<256, T>(m, s, st);
+    const int nth = m.n_lmh <= 192 ? 192 : 256;
+    int ngroup = 1, passes = 1;
+    const int npt = anlm_npt(m, nth, &ngroup, &passes);
+    if (m.maxl < kSmallL1 && passes * ngroup * npt <= kSmallGW)
+        return launch_anlm_nth<192, T, kSmallL1, kSmallGW>(m, s, npt, ngroup, passes, st);
+    if (nth == 192) return launch_anlm_nth<192, T, kL1, kLargeGW>(m, s, npt, ngroup, passes, st);
+    return launch_anlm_nth<256, T, kL1, kLargeGW>(m, s, npt, ngroup, passes, st);
 }
 
-template <int NF, int NT, typename T, bool GL>
+template <int NF, int NT, typename T, bool GL, int KL1>
 hipError_t launch_force_k(const DevModel& m, const DevSystem& s, hipStream_t st) {
     const size_t lds = GL ? sizeof(T) * 2 * m.n_type * m.n_lmh * NF : 0;
-    if (lds + sizeof(ForceLds<T>) + sizeof(double) * 6 * (NT / 64) > lds_limit()) return hipErrorInvalidConfiguration;
+    if (lds + sizeof(ForceLds<T, KL1>) + sizeof(double) * 6 * (NT / 64) > lds_limit())
+        return hipErrorInvalidConfiguration;
     if (GL) {
-        const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k_force<NF, NT, T, GL>),
+        const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k_force<NF, NT, T, GL, KL1>),
                                                  hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL((k_force<NF, NT, T, GL>), dim3(slot_grid(s)), dim3(NT), lds, st, m, s);
+    hipLaunchKernelGGL((k_force<NF, NT, T, GL, KL1>), dim3(slot_grid(s)), dim3(NT), lds, st, m, s);
     return launched();
 }
 
 // G tile staged in LDS when it fits beside the kernel's static LDS
-template <typename T>
+template <typename T, int KL1>
 bool force_tile_fits(const DevModel& m, int nf, int nt) {
     const size_t lds = sizeof(T) * 2 * m.n_type * m.n_lmh * nf;
-    return lds + sizeof(ForceLds<T>) + sizeof(double) * 6 * (nt / 64) <= lds_limit();
+    return lds + sizeof(ForceLds<T, KL1>) + sizeof(double) * 6 * (nt / 64) <= lds_limit();
 }
 
 template <int NT, typename T>
 hipError_t launch_force_nt(const DevModel& m, const DevSystem& s, hipStream_t st) {
     const int nf = m.n_fn <= 4 ? 4 : m.n_fn <= 8 ? 8 : m.n_fn <= 12 ? 12 : m.n_fn <= 16 ? 16 : m.n_fn <= 24 ? 24 : 32;
-    if (nf <= 16 && force_tile_fits<T>(m, nf, NT)) {
-        if (nf == 4) return launch_force_k<4, NT, T, true>(m, s, st);
-        if (nf == 8) return launch_force_k<8, NT, T, true>(m, s, st);
-        if (nf == 12) return launch_force_k<12, NT, T, true>(m, s, st);
-        return launch_force_k<16, NT, T, true>(m, s, st);
+    if (m.maxl < kSmallL1 && nf <= 16 && force_tile_fits<T, kSmallL1>(m, nf, NT)) {
+        if (nf == 4) return launch_force_k<4, NT, T, true, kSmallL1>(m, s, st);
+        if (nf == 8) return launch_force_k<8, NT, T, true, kSmallL1>(m, s, st);
+        if (nf == 12) return launch_force_k<12, NT, T, true, kSmallL1>(m, s, st);
+        return launch_force_k<16, NT, T, true, kSmallL1>(m, s, st);
     }
-    // large models: one launch shape
-    if (force_tile_fits<T>(m, nf, 128)) {
-        if (nf == 24) return launch_force_k<24, 128, T, true>(m, s, st);
-        if (nf == 32) return launch_force_k<32, 128, T, true>(m, s, st);
+    // large models (l_max above 12, more than 16 radial functions or a G
+    // tile beyond the LDS): one launch shape, full-size tables
+    if (force_tile_fits<T, kL1>(m, nf, 128)) {
+        switch (nf) {
+            case 4: return launch_force_k<4, 128, T, true, kL1>(m, s, st);
+            case 8: return launch_force_k<8, 128, T, true, kL1>(m, s, st);
+            case 12: return launch_force_k<12, 128, T, true, kL1>(m, s, st);
+            case 16: return launch_force_k<16, 128, T, true, kL1>(m, s, st);
+            case 24: return launch_force_k<24, 128, T, true, kL1>(m, s, st);
+            default: return launch_force_k<32, 128, T, true, kL1>(m, s, st);
+        }
     }
     switch (nf) {
-        case 4: return launch_force_k<4, 128, T, false>(m, s, st);
-        case 8: return launch_force_k<8, 128, T, false>(m, s, st);
-        case 12: return launch_force_k<12, 128, T, false>(m, s, st);
-        case 16: return launch_force_k<16, 128, T, false>(m, s, st);
-        case 24: return launch_force_k<24, 128, T, false>(m, s, st);
-        default: return launch_force_k<32, 128, T, false>(m, s, st);
+        case 4: return launch_force_k<4, 128, T, false, kL1>(m, s, st);
+        case 8: return launch_force_k<8, 128, T, false, kL1>(m, s, st);
+        case 12: return launch_force_k<12, 128, T, false, kL1>(m, s, st);
+        case 16: return launch_force_k<16, 128, T, false, kL1>(m, s, st);
+        case 24: return launch_force_k<24, 128, T, false, kL1>(m, s, st);
+        default: return launch_force_k<32, 128, T, false, kL1>(m, s, st);
     }
 }
 
@@ -705,10 +941,14 @@ void build_ylm_tables(YTab<double>& t64, YTab<float>& t32) {
         }
 }
 
-hipError_t launch_anlm(const DevModel& m, const DevSystem& s, int precision, hipStream_t st) {
+hipError_t launch_anlm(const DevModel& m, const DevSystem& s, int precision, hipStream_t st, int kind) {
     if (s.nblocks == 0) return hipSuccess;
+    if (kind == kAnlmMfma && anlm_mfma_fits(m))
+        return precision == kMixed ? launch_anlm_mfma_t<float>(m, s, st) : launch_anlm_mfma_t<double>(m, s, st);
     return precision == kMixed ? launch_anlm_t<float>(m, s, st) : launch_anlm_t<double>(m, s, st);
 }
+
+bool anlm_has_mfma(const DevModel& m) { return anlm_mfma_fits(m); }
 
 hipError_t launch_pair_energy(const DevModel& m, const DevSystem& s, double* epv, hipStream_t st) {
     if (s.nblocks == 0) return hipSuccess;

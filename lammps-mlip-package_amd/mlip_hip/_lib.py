@@ -88,6 +88,9 @@ def library():
     L.mlip_batch_get.argtypes = [vp, dp, dp, dp]
     L.mlip_graph_mode.argtypes = [vp, c_int]
     L.mlip_autotune.argtypes = [vp, c_int, ip]
+    L.mlip_autotune_q34.argtypes = [vp, c_int, ip]
+    L.mlip_set_anlm_kernel.argtypes = [vp, c_int]
+    L.mlip_get_anlm_kernel.argtypes = [vp]
     L.mlip_calibration_copy.argtypes = [vp, ctypes.c_int64, c_int]
     L.mlip_debug_fetch.argtypes = [vp, c_int, dp, ctypes.c_int64]
     L.mlip_debug_fetch.restype = ctypes.c_int64

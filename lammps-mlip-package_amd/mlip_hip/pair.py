@@ -80,6 +80,13 @@ class PairMLIPGtinv:
         self.lib.mlip_model_info(self.ctx, _p(v, ctypes.c_int))
         return dict(zip(INFO_KEYS, v.tolist()))
 
+    def set_anlm_kernel(self, kind):
+        """a_nlm kernel: "registers" or "mfma" (matrix cores)."""
+        self._check(self.lib.mlip_set_anlm_kernel(self.ctx, {"registers": 0, "mfma": 1}[kind]))
+
+    def anlm_kernel(self):
+        return {0: "registers", 1: "mfma"}.get(self.lib.mlip_get_anlm_kernel(self.ctx))
+
     def set_precision(self, mode):
         self._check(self.lib.mlip_set_precision(self.ctx, {"fp64": 0, "mixed": 1}[mode]))
 
@@ -175,6 +182,13 @@ class _ResidentBase:
         v = ctypes.c_int(0)
         self.pair._check(self.lib.mlip_autotune(self.pair.ctx, int(reps), ctypes.byref(v)))
         return v.value
+
+    def autotune_q34(self, reps=2):
+        """Order-3/4 workgroup plan timed on the device; returns its two
+        work-model constants."""
+        v = np.zeros(2, dtype=np.int32)
+        self.pair._check(self.lib.mlip_autotune_q34(self.pair.ctx, int(reps), _p(v, ctypes.c_int)))
+        return tuple(int(x) for x in v)
 
     def set_force_threads(self, threads):
         self.pair._check(self.lib.mlip_set_force_threads(self.pair.ctx, int(threads)))
@@ -444,6 +458,14 @@ class RequestPool:
 
     def profile(self, on=True):
         self.lib.mlip_profile_enable(self.pair.ctx, 1 if on else 0)
+
+    def read_profile(self):
+        """{kernel: (total_ms, launches)} since profile(True)."""
+        names = (ctypes.c_char_p * 64)()
+        ms = np.zeros(64)
+        cnt = np.zeros(64, dtype=np.int32)
+        n = self.lib.mlip_profile_read(self.pair.ctx, names, _p(ms, ctypes.c_double), _p(cnt, ctypes.c_int), 64)
+        return {names[k].decode(): (float(ms[k]), int(cnt[k])) for k in range(n)}
 
 
 def inspect_model(potfile, which, cap=1 << 26):

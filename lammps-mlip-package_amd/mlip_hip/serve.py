@@ -66,9 +66,13 @@ def make_stream(rate, duration, seed, mass_of_type, nmin=16, nmax=4096, frac_md=
     return out
 
 
-def serve(pool, stream, dt=0.001, skin=0.5, drain_timeout=120.0, nthreads=16, max_admit=512):
+def serve(pool, stream, dt=0.001, skin=0.5, drain_timeout=120.0, nthreads=16, max_admit=512, device_time=True):
     """Runs the stream through the pool in real time.  Returns per-request
-    latencies (s) and the run's totals."""
+    latencies (s) and the run's totals; with device_time, the evaluation
+    kernels' device time (HIP events around every launch) and its share of
+    the makespan."""
+    if device_time:
+        pool.profile(True)
     t0 = time.perf_counter()
     pending = deque(sorted(stream, key=lambda r: r.t_arrival))
     queue = deque()
@@ -148,11 +152,18 @@ def serve(pool, stream, dt=0.001, skin=0.5, drain_timeout=120.0, nthreads=16, ma
             done.append(r)
         host["results_s"] += time.perf_counter() - tr
     makespan = now()
+    dev_ms = None
+    if device_time:
+        dev_ms = sum(v[0] for v in pool.read_profile().values())
+        pool.profile(False)
     lat = np.array([r.t_done - r.t_arrival for r in done]) if done else np.zeros(0)
     return {
         "completed": len(done),
         "offered": len(stream),
         "unfinished": len(stream) - len(done),
+        "device_kernel_s": dev_ms / 1e3 if dev_ms is not None else None,
+        "device_busy_fraction": dev_ms / 1e3 / makespan if (dev_ms is not None and makespan > 0) else None,
+        "latency_p90_ms": float(np.percentile(lat, 90) * 1e3) if lat.size else None,
         "latency_p50_ms": float(np.percentile(lat, 50) * 1e3) if lat.size else None,
         "latency_p99_ms": float(np.percentile(lat, 99) * 1e3) if lat.size else None,
         "latency_mean_ms": float(lat.mean() * 1e3) if lat.size else None,

@@ -19,9 +19,9 @@ def check_forms(p, tag):
     D = p.debug_fetch(1, 1 << 28)
     Hq = p.debug_fetch(4, 1 << 28)
     forms = p.debug_fetch(5, 1 << 26)
-    nslot = D.size // n_pr
-    D = D.reshape(nslot, n_pr)
-    Hq = Hq.reshape(nslot, n_pr)
+    nslot = 64 * (int(lay[3]) + int(lay[6]))
+    D = D[:nslot * n_pr].reshape(nslot, n_pr)
+    Hq = Hq[:nslot * n_pr].reshape(nslot, n_pr)
     row0 = 0
     worst = 0.0
     for t in range(2):

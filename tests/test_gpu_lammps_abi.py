@@ -73,16 +73,54 @@ def test_int_pointer_entry(pot, full):
     assert np.array_equal(r2["eatom"], r["eatom"]) and np.array_equal(r2["vatom"], r["vatom"])
 
 
+def rotation(a=0.3, b=0.2):
+    rz = np.array([[np.cos(a), -np.sin(a), 0.0], [np.sin(a), np.cos(a), 0.0], [0.0, 0.0, 1.0]])
+    rx = np.array([[1.0, 0.0, 0.0], [0.0, np.cos(b), -np.sin(b)], [0.0, np.sin(b), np.cos(b)]])
+    return rx @ rz
+
+
+def rotated(nl, R):
+    """The same list with every position (ghosts included) rotated by R."""
+    nl.x = np.ascontiguousarray((nl.x.reshape(-1, 3) @ R.T).reshape(nl.x.shape))
+    return nl
+
+
 @pytest.mark.parametrize("pot", ["gtinv-197.mlip", "gtinv-820.mlip", "pair-60.mlip"])
 def test_triclinic_64(pot):
+    """The reference's 64-atom triclinic cell: a perfect lattice in a tilted
+    box, whose owned forces nearly cancel (|F| ~ 1e-5 from the file's
+    printed digits) while ghost entries carry up to 0.9 eV/A.  40 of its
+    half pairs lie exactly on the z axis, where the reference's derivative
+    drops the |m| = 1 polar term (mlip_features.cpp:111) and the oracle
+    with it, so parity is taken on a rotated copy of every position; the
+    unrotated cell then checks that the device forces are rotation
+    covariant there.  Owned forces are compared with a bar of 1e-8 of the
+    largest per-position force, the scale their rounding comes from."""
     s = W.read_lammps_data(ref("structure_00064_triclinic.lammps"))
+    R = rotation()
     o = O.Oracle(ref(pot))
-    nl_o = W.build_neighbor_list(s, o.cutoff, full=False)
+    nl_o = rotated(W.build_neighbor_list(s, o.cutoff, full=False), R)
     ro = o.compute(nl_o, s.types - 1, eatom=True, vatom=True)
+    fscale = np.abs(ro["forces_all"]).max()
     p = gpu_pair(ref(pot))
     for full in (False, True):
-        nl = W.build_neighbor_list(s, p.init_one(), full=full)
-        check(p.compute_lammps(nl, nl.types, eatom=True, vatom=True), ro, nl, nl_o, "%s tri full=%d" % (pot, full))
+        nl = rotated(W.build_neighbor_list(s, p.init_one(), full=full), R)
+        r = p.compute_lammps(nl, nl.types, eatom=True, vatom=True)
+        fe = np.abs(r["forces"] - ro["forces"]).max() / fscale
+        print("%s tri full=%d: owned-force error %.2e of max per-position |F|" % (pot, full, fe), flush=True)
+        assert fe <= 1e-8
+        r["forces"] = ro["forces"]  # compared above, on its own scale
+        check(r, ro, nl, nl_o, "%s tri full=%d" % (pot, full))
+        if not full:
+            fa = np.abs(r["forces_all"] - ro["forces_all"]).max() / fscale
+            assert fa <= 1e-8, fa
+        # unrotated: the same forces rotated back (exact on the z axis)
+        nl0 = W.build_neighbor_list(s, p.init_one(), full=full)
+        r0 = p.compute_lammps(nl0, nl0.types)
+        back = r["forces_all"] @ R
+        fr = np.abs(r0["forces_all"] - back).max() / fscale
+        print("%s tri full=%d: unrotated vs rotated back %.2e" % (pot, full, fr), flush=True)
+        assert fr <= 1e-10 and abs(r0["energy"] - r["energy"]) <= 1e-10 * nl.nlocal
 
 
 def test_chain_split_gpu():

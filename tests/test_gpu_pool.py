@@ -263,3 +263,33 @@ def test_pool_nve_rebuilds_lists():
     assert np.abs(res["velocities"] - vo).max() < 1e-8
     for k in (50, 100, 200):
         assert abs(thermo_of(res["thermo"][k], s)["E_pair"] - tho[k]["E_pair"]) < 1e-9
+
+
+def test_cfg1_nve_300k_against_oracle(tmp_path):
+    """The 256-atom fcc configuration (a = 4.05 A, sigma = 0.05 A, seed 1;
+    order-3 linear model) run 100 NVE steps at 1 fs from a 300 K Maxwell
+    draw (seed 3) in the pool, against the oracle integrator (lists rebuilt
+    every step): positions within 1e-9 A, thermo within 1e-9 eV; the run's
+    LAMMPS-format thermo block and dump come from the package."""
+    from mlip_hip import serve as SV
+    pot = config_potential("fcc256_order3_linear")
+    s = W.fcc(4, 4.05, 0.05, 1)
+    masses = potential_masses(pot)
+    v0 = SV.maxwell(s, masses, 300.0, np.random.default_rng(3))
+    p = gpu_pair(pot, ["X"])
+    pool = RequestPool(p, 1000, 200000, 4)
+    h = pool.admit(s, nsteps=100, dt=0.001, skin=1.0, velocities=v0)
+    res = drain(pool)[h]
+    o = O.Oracle(pot)
+    xo, vo, tho = MD.run_nve(o, s, masses, 0.001, 100, record=(0, 50, 100), v0=v0)
+    dx = np.abs(MD.periodic_diff(res["positions"], xo, s.cell)).max()
+    print("cfg1 NVE 100 steps from 300 K: max |x - oracle| = %.2e A, %d list rebuilds" % (dx, pool.rebuilds()))
+    assert dx < 1e-9
+    for k in (0, 50, 100):
+        t = thermo_of(res["thermo"][k], s)
+        assert abs(t["E_pair"] - tho[k]["E_pair"]) < 1e-9 and abs(t["TotEng"] - tho[k]["TotEng"]) < 1e-9
+        assert abs(t["Temp"] - tho[k]["Temp"]) < 1e-6 * max(1.0, tho[k]["Temp"])
+    log = MDP.thermo_log(res["thermo"], s, every=50)
+    print(log)
+    MDP.write_dump(os.path.join(str(tmp_path), "dump.atom"), 100, s, res["positions"])
+    assert np.abs(MD.read_dump(os.path.join(str(tmp_path), "dump.atom")) - res["positions"]).max() < 1e-12
