@@ -90,6 +90,24 @@ int mlip_compute_flat(mlip_ctx* ctx, int nlocal, int nghost, const double* x, co
                       const int* neigh_off, const int* neigh_idx, int neigh_full, int eflag, int vflag,
                       double* f, double* eng_vdwl, double* virial6, double* eatom, double* vatom6);
 
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Same arguments and results as mlip_compute_flat, bit
+ * for bit, but every device buffer of the call (positions, lists, a_nlm,
+ * adjoints, features, pair forces, reductions) is carved from one static
+ * device array of 128 MiB per GPU, held until the process ends; the call
+ * itself allocates nothing.  Structures needing more return MLIP_ERR_ARG
+ * ("scratch" in the message): about 1,400 atoms of the order-4 benchmark
+ * model at 137 neighbors per atom (95 KB per atom).
+ * The array is shared by every context and call on a GPU and keeps the last
+ * call's contents: calls that overlap on one GPU (two host threads, two
+ * contexts) race. */
+int mlip_compute_flat_noalloc(mlip_ctx* ctx, int nlocal, int nghost, const double* x, const int* type,
+                              const int64_t* tag, int inum, const int* ilist, const int* numneigh,
+                              const int* neigh_off, const int* neigh_idx, int neigh_full, int eflag,
+                              int vflag, double* f, double* eng_vdwl, double* virial6, double* eatom,
+                              double* vatom6);
+
 /* ---- Device-resident evaluation (no counterpart in the reference; the
  * Kokkos style re-copies its inputs every step, pair_mlip_gtinv_kokkos_impl.h
  * :411-634).  Positions, types and a full neighbor list of owned atoms

@@ -53,6 +53,7 @@ def library():
                                c_int, dp, dp, dp, dp, dp]
     L.mlip_compute_flat.argtypes = [vp, c_int, c_int, dp, ip, i64p, c_int, ip, ip, ip, ip, c_int, c_int, c_int,
                                     dp, dp, dp, dp, dp]
+    L.mlip_compute_flat_noalloc.argtypes = L.mlip_compute_flat.argtypes
     L.mlip_resident_set.argtypes = [vp, c_int, dp, ip, ctypes.c_int64, i64p, ip, dp]
     L.mlip_resident_update_positions.argtypes = [vp, dp]
     L.mlip_resident_compute.argtypes = [vp]

@@ -130,11 +130,12 @@ class PairMLIPGtinv:
             out["vatom"] = va
         return out
 
-    def compute(self, nl, types, eflag=1, vflag=1, eatom=False, vatom=False):
+    def compute(self, nl, types, eflag=1, vflag=1, eatom=False, vatom=False, noalloc=False):
         """Evaluate on a ``workloads.NeighborList``; ``types`` are the
         1-based LAMMPS types of all nall positions.  Returns a dict with the
         energy, forces on all positions (ghost entries unreduced), forces
-        folded onto owned atoms, and the virial."""
+        folded onto owned atoms, and the virial.  ``noalloc`` selects the
+        entry point that allocates no device memory (mlip_compute_flat_noalloc)."""
         f = np.zeros((nl.nall, 3))
         eng = np.zeros(1)
         vir = np.zeros(6)
@@ -142,7 +143,8 @@ class PairMLIPGtinv:
         va = np.zeros((nl.nall, 6)) if vatom else None
         typ = np.ascontiguousarray(types, dtype=np.int32)
         tag = np.ascontiguousarray(nl.tag, dtype=np.int64)
-        rc = self.lib.mlip_compute_flat(
+        fn = self.lib.mlip_compute_flat_noalloc if noalloc else self.lib.mlip_compute_flat
+        rc = fn(
             self.ctx, nl.nlocal, nl.nall - nl.nlocal, _p(nl.x, ctypes.c_double), _p(typ, ctypes.c_int),
             _p(tag, ctypes.c_int64), nl.inum, _p(nl.ilist, ctypes.c_int), _p(nl.numneigh, ctypes.c_int),
             _p(nl.off, ctypes.c_int), _p(nl.idx, ctypes.c_int), 1 if nl.full else 0, eflag, vflag,
