@@ -135,15 +135,19 @@ def drift_section(pot, nsteps=200):
 
 
 def timed_steps(rs, warmup, steps, tag):
-    """Warm up, pick the force kernel's launch shape with the autotuner, then
-    time `steps` evaluations (HIP events per kernel on the engine's stream)."""
+    """Warm up, pick the force kernel's launch shape, the a_nlm kernel and the
+    order-3/4 workgroup plan with the autotuners, then time `steps`
+    evaluations (HIP events per kernel on the engine's stream)."""
     for _ in range(max(warmup, 1)):
         rs.compute()
     rs.sync()
     nt = rs.autotune(reps=2)
+    plan = rs.autotune_q34(reps=2)
     rs.compute()
     rs.sync()
-    log("%s: warmup done, force kernel %d threads per workgroup" % (tag, nt))
+    rs.tuning = {"force_threads": nt, "anlm_kernel": rs.pair.anlm_kernel(), "q34_plan_work_model": list(plan)}
+    log("%s: warmup done, force kernel %d threads per workgroup, a_nlm kernel %s, order-3/4 plan %s"
+        % (tag, nt, rs.pair.anlm_kernel(), plan))
     rs.profile(True)
     rs.sync()
     t0 = time.perf_counter()
@@ -461,6 +465,7 @@ def main():
     log("neighbor list: %d atoms, %d pairs (%.1f per atom): device %.3fs (first %.2fs), host %.2fs"
         % (s.natoms, rs.npairs, rs.npairs / s.natoms, t_dev, t_dev_first, t_host))
     dt, prof, res, nt = timed_steps(rs, args.warmup, args.steps, args.precision)
+    tuning = dict(rs.tuning)
     if not np.isfinite(res["energy"]) or not np.all(np.isfinite(res["forces"])):
         raise SystemExit("non-finite result")
     flops = rs.flops()
@@ -472,7 +477,7 @@ def main():
         pair.set_precision("fp64")
         df = np.abs(resm["forces"] - res["forces"])
         mixed = {"value": s.natoms * args.steps / dtm, "ms_per_step": 1e3 * dtm / args.steps,
-                 "speedup_vs_fp64": dt / dtm, "force_threads": ntm,
+                 "speedup_vs_fp64": dt / dtm, "tuning": dict(rs.tuning),
                  "energy_err_per_atom": abs(resm["energy"] - res["energy"]) / s.natoms,
                  "force_err_max": float(df.max()), "force_mae": float(df.mean()),
                  "reference": "FP64 GPU result of the same step (itself within 1e-10 eV/atom of the CPU oracle)",
@@ -555,7 +560,7 @@ def main():
                                "one E+F+virial evaluation per step, inputs resident in HBM"
                                % (args.ncell, args.ncell, args.ncell, natoms),
                    "natoms": natoms, "pairs_per_atom": npairs / natoms, "precision": args.precision,
-                   "force_threads": nt},
+                   "tuning": tuning},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "mixed": mixed,
