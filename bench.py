@@ -98,6 +98,65 @@ def cpu_baseline(pot, budget_s):
                                         % (n1, n1a, t1)}}
 
 
+def cfg1_section(tmpdir, nsteps=100):
+    """The 256-atom fcc configuration (4x4x4 cells, a = 4.05 A, sigma = 0.05 A,
+    seed 1; order-3 [8,4] linear model, 10 Gaussians, r_c 6 A): one E/F/virial
+    evaluation and a 100-step NVE run at 1 fs from a 300 K Maxwell draw
+    (seed 3), on the host's cores with the oracle (neighbor list rebuilt each
+    step, as the oracle's own integrator does) and on the device (request
+    pool, skin 1 A, lists rebuilt when an atom moves half the skin); final
+    positions and step-100 energies compared."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import md_oracle as MD
+    import oracle_lib
+    pot = os.path.join(tmpdir, "fcc256_order3_linear.mlip")
+    W.config_potential(pot, "fcc256_order3_linear", lambda probe: inspect_model(probe, 4))
+    s = W.fcc(4, 4.05, 0.05, 1)
+    mass = [26.98]
+    v0 = SV.maxwell(s, mass, 300.0, np.random.default_rng(3))
+    o = oracle_lib.Oracle(pot)
+    nl = W.build_neighbor_list(s, o.cutoff, full=False)
+    ts = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        o.compute(nl, s.types - 1)
+        ts.append(time.perf_counter() - t0)
+    t_eval_cpu = float(np.median(ts))
+    t0 = time.perf_counter()
+    xo, _, tho = MD.run_nve(o, s, mass, 0.001, nsteps, record=(nsteps,), v0=v0)
+    t_nve_cpu = time.perf_counter() - t0
+    p = PairMLIPGtinv()
+    p.settings()
+    p.coeff("*", "*", pot, *W.CONFIG_MODELS["fcc256_order3_linear"]["elements"])
+    rs = ResidentSystem.from_cell(p, s.cell, s.positions, s.types)
+    for _ in range(3):
+        rs.compute()
+    rs.sync()
+    t0 = time.perf_counter()
+    for _ in range(20):
+        rs.compute()
+    rs.sync()
+    t_eval_gpu = (time.perf_counter() - t0) / 20
+    pool = SV.RequestPool(p, 1000, 200000, 4)
+    t0 = time.perf_counter()
+    h = pool.admit(s, nsteps=nsteps, dt=0.001, skin=1.0, velocities=v0)
+    while pool.tick():
+        pool.sync()
+    t_nve_gpu = time.perf_counter() - t0
+    res = pool.result(h, max_evals=nsteps + 8)
+    dx = float(np.abs(MD.periodic_diff(res["positions"], xo, s.cell)).max())
+    th = res["thermo"][nsteps]
+    return {"workload": "256-atom fcc (4x4x4 cells, a=4.05 A, sigma=0.05 A, seed 1), order-3 [8,4] linear model, "
+                        "10 Gaussians, r_c 6 A; 100 NVE steps at 1 fs from 300 K (seed 3)",
+            "cpu_oracle": {"eval_s": t_eval_cpu, "atom_evals_per_s": s.natoms / t_eval_cpu, "nve_100_s": t_nve_cpu,
+                           "cores": int(oracle_lib.lib().oracle_num_threads()),
+                           "note": "median of 5 evaluations; the NVE time includes a host neighbor build per step"},
+            "gpu": {"eval_s": t_eval_gpu, "atom_evals_per_s": s.natoms / t_eval_gpu, "nve_100_s": t_nve_gpu,
+                    "list_rebuilds": pool.rebuilds(), "note": "a 256-atom evaluation is launch-latency bound"},
+            "final_positions_max_diff_A": dx,
+            "epair_step100_diff_per_atom": abs(th[0] - tho[nsteps]["E_pair"]) / s.natoms}
+
+
 def nve_drift(pot, precision, nsteps=200, ncell=10):
     """Total-energy drift of a device-resident NVE run (4000-atom binary fcc
     of the benchmark generator, 300 K Maxwell start, 1 fs) in the request
@@ -398,7 +457,10 @@ def served_section(pot, rates, windows, budget_gb):
         r["arrival_window_s"] = win
         r["fp64_sequential"] = seq
         r["mixed_vs_fp64_error"] = SV.stream_errors(stream, seq_stream)
-        r["regime"] = "saturated" if r["device_busy_fraction"] and r["device_busy_fraction"] > 0.8 else "below saturation"
+        # offered load: the stream's atom-evaluations over its arrival window;
+        # a makespan well past the window means the queue grew (saturation)
+        r["offered_atom_evals_per_s"] = r["atom_evals"] / win
+        r["regime"] = "saturated" if r["makespan_s"] > 1.2 * win else "below saturation"
         out["rates"][str(rate)] = r
         log("served %d req/s (%d requests): p50 %.1f ms p99 %.1f ms, %.4g atom-evals/s; "
             "FP64 one-at-a-time: p50 %.1f ms p99 %.1f ms, %.4g atom-evals/s"
@@ -542,6 +604,15 @@ def main():
     if not args.no_cpu_baseline:
         log("timing the CPU oracle (bounded sample)")
         cpu = cpu_baseline(pot, args.cpu_budget)
+    cfg1 = None
+    if not args.no_cpu_baseline:
+        log("256-atom order-3 configuration: CPU oracle and device, one evaluation and 100 NVE steps")
+        cfg1 = cfg1_section(tmp)
+        log("  oracle %.3g atom-evals/s (%d threads), NVE 100 steps %.1f s; device %.3g atom-evals/s, NVE %.2f s "
+            "(%d list rebuilds); final positions differ by %.2e A"
+            % (cfg1["cpu_oracle"]["atom_evals_per_s"], cfg1["cpu_oracle"]["cores"], cfg1["cpu_oracle"]["nve_100_s"],
+               cfg1["gpu"]["atom_evals_per_s"], cfg1["gpu"]["nve_100_s"], cfg1["gpu"]["list_rebuilds"],
+               cfg1["final_positions_max_diff_A"]))
     line = {
         "metric": "atom-evals/s (E+F) FP64 vs mixed at 256k atoms; p50/p99 latency served",
         "value": value,
@@ -563,6 +634,7 @@ def main():
                    "tuning": tuning},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "fcc256_order3": cfg1,
         "mixed": mixed,
         "batched": batched,
         "served": served,

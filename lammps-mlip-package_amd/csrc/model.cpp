@@ -21,6 +21,38 @@ extern const char mlip_gtinv_table_end[];
 
 namespace mlip {
 
+#include "q34_signatures.inc"
+static_assert(MLIP_CAP3 == kUnrollCap3 && MLIP_CAP4 == kUnrollCap4,
+              "unroll caps of the model tables and of the order-3/4 kernel's signature list");
+
+namespace {
+
+// (order, output L, l_a, l_b[, l_c]) is one of the signatures the order-3/4
+// kernel has unrolled code for (the list q34_signatures.inc compiles into it)
+bool unrolled_signature(int q, int L, int la, int lb, int lc) {
+    static const std::set<std::array<int, 5>> sigs = [] {
+        std::set<std::array<int, 5>> s;
+#define MLIP_HOST_SIG3(A, B) s.insert({3, LCUR, A, B, 0});
+#define MLIP_HOST_SIG4(A, B, C) s.insert({4, LCUR, A, B, C});
+        int LCUR = 0;
+        LCUR = 0; MLIP_SIG3_L0(MLIP_HOST_SIG3) MLIP_SIG4_L0(MLIP_HOST_SIG4)
+        LCUR = 1; MLIP_SIG3_L1(MLIP_HOST_SIG3) MLIP_SIG4_L1(MLIP_HOST_SIG4)
+        LCUR = 2; MLIP_SIG3_L2(MLIP_HOST_SIG3) MLIP_SIG4_L2(MLIP_HOST_SIG4)
+        LCUR = 3; MLIP_SIG3_L3(MLIP_HOST_SIG3)
+        LCUR = 4; MLIP_SIG3_L4(MLIP_HOST_SIG3)
+        LCUR = 5; MLIP_SIG3_L5(MLIP_HOST_SIG3)
+        LCUR = 6; MLIP_SIG3_L6(MLIP_HOST_SIG3)
+        LCUR = 7; MLIP_SIG3_L7(MLIP_HOST_SIG3)
+        LCUR = 8; MLIP_SIG3_L8(MLIP_HOST_SIG3)
+#undef MLIP_HOST_SIG3
+#undef MLIP_HOST_SIG4
+        return s;
+    }();
+    return sigs.count({q, L, la, lb, q == 4 ? lc : 0}) > 0;
+}
+
+}  // namespace
+
 // ------------------------------------------------------------ gtinv table
 bool parse_gtinv_text(const char* text, size_t len, GtinvTable& out, std::string& err) {
     std::istringstream in(std::string(text, len));
@@ -483,7 +515,10 @@ void build_type_tables(Model& m, int t, const std::vector<std::map<std::vector<i
             const bool is_poly = pslot >= 0;
             int lmax_v = std::max(L, std::max(v.la, v.lb));
             if (q == 4) lmax_v = std::max(lmax_v, v.lc);
-            const bool fast = (q == 3 && lmax_v <= kUnrollCap3) || (q == 4 && lmax_v <= kUnrollCap4);
+            // a visit runs unrolled only if the order-3/4 kernel was compiled
+            // with its signature; any other goes to the general loop
+            const bool fast = ((q == 3 && lmax_v <= kUnrollCap3) || (q == 4 && lmax_v <= kUnrollCap4)) &&
+                              unrolled_signature(q, L, v.la, v.lb, v.lc);
             // adjoint visits accumulate G only; the energy of linear-only
             // families follows from G by Euler's theorem in the kernel
             v.flags = 2 | (fast ? 4 : 0);
