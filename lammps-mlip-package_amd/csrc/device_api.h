@@ -28,6 +28,7 @@ struct YTab {
 struct DevModel {
     int n_type, n_fn, n_gtinv, n_lmh, n_lmh3, maxl, lmax3, n_pitc, n_p;
     int has_q34;            // order-3/4 invariants present (k_q34 runs)
+    int q12_fuse;           // order-1/2 adjoint rows L <= q12_fuse are the order-3/4 kernel's (-1: none)
     double cutoff;
     double beta[kMaxFnPair], mu[kMaxFnPair];
     const double* coef;     // dense coefficient blocks

@@ -110,6 +110,7 @@ bool upload_model(mlip_ctx* c) {
     for (int t = 0; t < 2; ++t)
         for (int k = 0; k < 2; ++k) dm.prange[t][k] = m.prange[t][k];
     dm.has_q34 = m.has_order34 ? 1 : 0;
+    dm.q12_fuse = m.q12_fuse;
     dm.q34_red = m.q34_red;
     dm.pair_model = m.pair_model ? 1 : 0;
     for (int a = 0; a < 2; ++a)
@@ -1484,12 +1485,16 @@ int mlip_flops(const mlip_ctx* c, double* out7) {
     for (int t = 0; t < m.n_type && t < static_cast<int>(c->owners_per_type.size()); ++t) {
         const double na = static_cast<double>(c->owners_per_type[t]) * m.n_fn;
         const TypeTables& tt = m.types[t];
+        // orders 1/2: feature pass 6 (L + 1) per polynomial family (2 for
+        // order 1), adjoint 14 (L + 1) (2); adjoint rows L <= q12_fuse are
+        // counted with the order-3/4 kernel that runs them
         double q12 = 0.0;
         for (const VisitQ12& v : tt.visits_q12) {
             int L = 0;
             for (int l = 0; l <= m.maxl; ++l)
                 if (&v - tt.visits_q12.data() >= tt.q12_off[l]) L = l;
-            q12 += v.order == 1 ? 4.0 : 20.0 * (L + 1);
+            if (v.pslot >= 0) q12 += v.order == 1 ? 2.0 : 6.0 * (L + 1);
+            if (L > m.q12_fuse) q12 += v.order == 1 ? 2.0 : 14.0 * (L + 1);
         }
         out7[1] += na * q12;
         auto terms = [&](const std::vector<GroupQ34>& gs, const std::vector<VisitQ34>& vs, bool adj) {
@@ -1497,6 +1502,12 @@ int mlip_flops(const mlip_ctx* c, double* out7) {
             for (const GroupQ34& g : gs)
                 for (int vi = g.vbeg; vi < g.vend; ++vi) {
                     const VisitQ34& v = vs[vi];
+                    if (v.order <= 2) {
+                        // fused order-1/2 adjoint: H C (1) and a complex
+                        // FMA pair (4) per term; order 1 one FMA
+                        f += v.order == 0 ? 0.0 : v.order == 1 ? 2.0 : 5.0 * (g.L + 1);
+                        continue;
+                    }
                     long cnt = 0;
                     for (int ma = -v.la; ma <= v.la; ++ma) {
                         if (v.order == 3) {
@@ -1516,9 +1527,14 @@ int mlip_flops(const mlip_ctx* c, double* out7) {
                     f += (adj ? 10.0 : 11.0) * cnt + (v.order == 4 ? 6.0 * na_ * nb_ : 0.0) + (adj ? 2.0 * na_ : 0.0);
                 }
             // adjoint: the energy of linear families from the G rows (Euler),
-            // at most two class boundaries per group
+            // one row dot product per linear class present in the group
             if (adj)
-                for (const GroupQ34& g : gs) f += 2.0 * 5.0 * (g.L + 1);
+                for (const GroupQ34& g : gs) {
+                    bool seen[4] = {false, false, false, false};
+                    for (int vi = g.vbeg; vi < g.vend; ++vi)
+                        if (vs[vi].pslot < 0 && vs[vi].order >= 1 && vs[vi].order <= 4) seen[vs[vi].order - 1] = true;
+                    for (bool b : seen) f += b ? 5.0 * (g.L + 1) : 0.0;
+                }
             return f;
         };
         out7[2] += na * terms(tt.groups_feat, tt.visits_feat, false);

@@ -234,11 +234,11 @@ void make_groups(const Model& m, std::vector<PlanVisit> pend, std::vector<GroupQ
                  std::vector<VisitQ34>& visits, bool adjoint, int& red_total, bool& dyn, int region_cost,
                  int visit_cost) {
     // group by (S, L); inside a group, visits by energy class (linear
-    // order 3, linear order 4, polynomial: the adjoint kernel takes each
+    // order 1, 2, 3, 4, polynomial: the adjoint kernel takes each
     // linear class's energy from the accumulated G row at the class
     // boundary), then with the same code path (signature) adjacent for
     // instruction-cache locality
-    auto cls = [](const VisitQ34& v) { return v.pslot >= 0 ? 2 : (v.order == 3 ? 0 : 1); };
+    auto cls = [](const VisitQ34& v) { return v.pslot >= 0 ? 4 : std::max(v.order - 1, 0); };
     std::stable_sort(pend.begin(), pend.end(), [&](const PlanVisit& a, const PlanVisit& b) {
         if (a.S != b.S) return a.S < b.S;
         if (a.L != b.L) return a.L < b.L;
@@ -253,7 +253,7 @@ void make_groups(const Model& m, std::vector<PlanVisit> pend, std::vector<GroupQ
     // best of a sweep at the 256k-atom benchmark model: 40.3 -> 38.8 ms);
     // of a part: plus loading and storing its G row.
     auto vcost = [&](const PlanVisit& q) {
-        const long regions = (2L * q.v.la + 1) * (q.v.order == 4 ? 2L * q.v.lb + 1 : 1);
+        const long regions = q.v.order <= 2 ? 0 : (2L * q.v.la + 1) * (q.v.order == 4 ? 2L * q.v.lb + 1 : 1);
         return q.cost + region_cost * regions + visit_cost;
     };
     struct Row {
@@ -449,6 +449,30 @@ void build_type_tables(Model& m, int t, const std::vector<std::map<std::vector<i
                 for (int m0 = -L; m0 <= L; ++m0) push_coef(lookup(cm, {m0, -m0}));
             }
             by_l[L].push_back(v);
+            if (L <= m.q12_fuse) {
+                // the same family's adjoint in the order-3/4 kernel, one visit
+                // per factor position p: dd/da_{sp[p],L,-k} = C a_{other,L,+k}
+                // with C = C(-k, k) (p = 0) or C(k, -k) (p = 1)
+                for (int pp = 0; pp < q; ++pp) {
+                    VisitQ34 f{};
+                    f.order = q;
+                    f.itc = i;
+                    f.la = q == 2 ? L : 0;
+                    f.sa = q == 2 ? sp[1 - pp] : 0;
+                    f.pslot = pslot;
+                    f.flags = 2 | 4;
+                    f.coef = static_cast<int>(m.coef.size());
+                    f.dcoef = -1;
+                    if (q == 1) {
+                        push_coef(lookup(cm, {0}));
+                    } else {
+                        for (int k = 0; k <= L; ++k) push_coef(lookup(cm, pp == 0 ? std::vector<int>{-k, k}
+                                                                                   : std::vector<int>{k, -k}));
+                    }
+                    const int nc = q == 1 ? 1 : L + 1;
+                    adj.push_back({sp[pp], L, f, nc, nc});
+                }
+            }
             continue;
         }
         if (q >= 5) {
@@ -604,6 +628,22 @@ void build_type_tables(Model& m, int t, const std::vector<std::map<std::vector<i
         }
         adj.swap(merged);
     }
+    // every fused row is written by the order-3/4 kernel (the order-1/2
+    // kernel skips it): rows no family adds to get an empty marker visit,
+    // so their G is written as zero
+    for (int S = 0; S < m.n_type; ++S)
+        for (int L = 0; L <= m.q12_fuse; ++L) {
+            bool any = false;
+            for (const Pending& q : adj) any = any || (q.S == S && q.L == L);
+            if (!any) {
+                VisitQ34 z{};
+                z.order = 0;
+                z.flags = 2 | 4;
+                z.pslot = -1;
+                z.dcoef = -1;
+                adj.push_back({S, L, z, 0, 0});
+            }
+        }
     tt.plan_adj = std::move(adj);
     tt.plan_feat = std::move(feat);
     m.types[t] = std::move(tt);
@@ -751,6 +791,7 @@ bool load_pair_model(std::istream& in, Model& m, std::string& err, bool strict_c
     m.lmax3 = 0;
     m.has_order34 = false;
     m.has_feat34 = false;
+    m.q12_fuse = -1;
     m.types.assign(m.n_type, TypeTables());
     return true;
 }
@@ -960,6 +1001,7 @@ bool load_model(const std::string& path, const GtinvTable& table, Model& m, std:
             m.lmax3 = std::max(m.lmax3, ir.l.back());
         }
     m.n_lmh3 = n_lmh_of(m.lmax3);
+    m.q12_fuse = m.has_order34 ? std::min(m.lmax3, kUnrollCap3) : -1;
     m.lmaxh = -1;
     for (const Irrep& ir : m.irreps)
         if (ir.l.size() >= 5) m.lmaxh = std::max(m.lmaxh, ir.l.back());

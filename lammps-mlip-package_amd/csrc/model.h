@@ -62,7 +62,7 @@ struct VisitQ12 {
 // Order-3/4 visit: contributions of one (itc, factor position) to the
 // adjoint G_{S,L,m<=0} and, for the designated position, the feature d.
 struct VisitQ34 {
-    int32_t order;     // 3 or 4
+    int32_t order;     // 3 or 4; 1 or 2 for fused order-1/2 adjoint rows; 0: empty row marker
     int32_t itc;
     int32_t la, sa;    // other factors in irrep order
     int32_t lb, sb;
@@ -179,6 +179,9 @@ struct Model {
     std::vector<TypeTables> types;
     bool has_order34 = false;
     bool has_feat34 = false;
+    // order-1/2 adjoint rows L <= q12_fuse run inside the order-3/4 adjoint
+    // kernel, on its LDS tile (-1: none; the order-1/2 kernel takes the rest)
+    int q12_fuse = -1;
     // radial-only style (des_type pair): features f = tc * n_fn + n
     bool pair_model = false;
     std::array<std::array<int, 2>, 2> pair_tc{};   // [centre type][neighbor type] -> tc

@@ -358,13 +358,14 @@ __device__ __forceinline__ void q34_group(const Tile& T, const DevModel& M, cons
     cptr<VisitPack> vis = as_const(FEAT ? M.vfeat : M.vadj);
     const int vb = uniform(g.vbeg), ve = uniform(g.vend);
     // energy classes of the visits (sorted in this order within a row):
-    // 0 linear order 3, 1 linear order 4, 2 polynomial features
+    // linear order 1, 2, 3, 4 (class q - 1), 4 polynomial features; a linear
+    // class's energy is 1/q of the row's dot product gained across it
     int cls_cur = -1;
     double dprev = 0.0;
     auto close_class = [&](int c) {
-        if (c == 0 || c == 1) {
+        if (c >= 0 && c <= 3) {
             const double D = row_dot<L>(T, Sx, gr, gi);
-            eacc = fma(c == 0 ? (1.0 / 3.0) : 0.25, D - dprev, eacc);
+            eacc = fma(c == 0 ? 1.0 : c == 1 ? 0.5 : c == 2 ? (1.0 / 3.0) : 0.25, D - dprev, eacc);
             dprev = D;
         }
     };
@@ -378,14 +379,36 @@ __device__ __forceinline__ void q34_group(const Tile& T, const DevModel& M, cons
         const int order = v.order, flags = v.flags, pslot = v.pslot;
         const bool fast = (flags & 4) != 0;
         if (fast == DYN) continue;  // the other kernel's visit
+        if (order == 0) continue;   // marker of a fused row no family adds to
         const int itc = uniform(v.itc);
         double d = 0.0;
         if (!FEAT) {
-            const int c = pslot >= 0 ? 2 : (order == 3 ? 0 : 1);
+            const int c = pslot >= 0 ? 4 : order - 1;
             if (c != cls_cur) {
                 close_class(cls_cur);
                 cls_cur = c;
             }
+        }
+        if (!FEAT && order <= 2) {
+            // fused order-1/2 adjoint: G_{S,L,-k} += H C_k a_{s_a,L,+k}
+            // (order 1: G_{S,0,0} += H C)
+            const double H =
+                pslot >= 0 ? S.H[prow + pslot] : as_const(M.weight)[n * M.n_gtinv + itc];
+            cptr<double> c1 = as_const(M.coef) + uniform(v.coef);
+            if (order == 1) {
+                gr[0] = fma(H, c1[0], gr[0]);
+            } else {
+                const int sa = uniform(v.sa);
+#pragma unroll
+                for (int k = 0; k <= L; ++k) {
+                    double yr, yi;
+                    T.get(sa, L, k, yr, yi);
+                    const double hc = H * c1[k];
+                    gr[k] = fma(hc, yr, gr[k]);
+                    gi[k] = fma(hc, yi, gi[k]);
+                }
+            }
+            continue;
         }
         const int la = uniform(v.la), lb = uniform(v.lb), lc = uniform(v.lc);
         cptr<double> cg = as_const(M.coef) + uniform(v.coef);
@@ -416,12 +439,19 @@ __device__ __forceinline__ void q34_group(const Tile& T, const DevModel& M, cons
     if (!FEAT) {
         close_class(cls_cur);
         if (uniform(g.nparts) == 1) {
+            // fused rows are this kernel's alone: the first kernel writes them
+            const bool first = !DYN && L <= M.q12_fuse;
 #pragma unroll
             for (int k = 0; k <= L; ++k) {
                 const size_t ir = a_index(M, blk, n, Sx, lmh(L, -k), 0, T.lane);
                 const size_t ii = a_index(M, blk, n, Sx, lmh(L, -k), 1, T.lane);
-                S.G[ir] += gr[k];
-                S.G[ii] += gi[k];
+                if (first) {
+                    S.G[ir] = gr[k];
+                    S.G[ii] = gi[k];
+                } else {
+                    S.G[ir] += gr[k];
+                    S.G[ii] += gi[k];
+                }
             }
         } else {
             double* pr = red + uniform(g.red);
@@ -436,8 +466,9 @@ __device__ __forceinline__ void q34_group(const Tile& T, const DevModel& M, cons
 
 // Sum of the partial rows of a split (S, L) row, in part order, into G.
 __device__ __forceinline__ void q34_combine(const DevModel& M, const DevSystem& S, const GroupQ34& g, int blk, int n,
-                                            int lane, const double* red) {
+                                            int lane, const double* red, bool dyn) {
     const int L = uniform(g.L), Sx = uniform(g.S), np = uniform(g.nparts);
+    const bool first = !dyn && L <= M.q12_fuse;  // a fused row: written, not added to
     const double* base = red + uniform(g.red);
     const int stride = (L + 1) * 2 * kBlock;
     for (int k = 0; k <= L; ++k) {
@@ -446,8 +477,15 @@ __device__ __forceinline__ void q34_combine(const DevModel& M, const DevSystem& 
             sr += base[j * stride + (2 * k) * kBlock + lane];
             si += base[j * stride + (2 * k + 1) * kBlock + lane];
         }
-        S.G[a_index(M, blk, n, Sx, lmh(L, -k), 0, lane)] += sr;
-        S.G[a_index(M, blk, n, Sx, lmh(L, -k), 1, lane)] += si;
+        double& gre = S.G[a_index(M, blk, n, Sx, lmh(L, -k), 0, lane)];
+        double& gim = S.G[a_index(M, blk, n, Sx, lmh(L, -k), 1, lane)];
+        if (first) {
+            gre = sr;
+            gim = si;
+        } else {
+            gre += sr;
+            gim += si;
+        }
     }
 }
 
@@ -509,7 +547,7 @@ __global__ __launch_bounds__(kQ34Threads) void k_q34(DevModel M, DevSystem S) {
                 const GroupQ34 g = groups[gi];
                 if (uniform(g.wave) != wave || uniform(g.nparts) == 1 || uniform(g.part) != 0) continue;
                 if (!DYN && uniform(g.L) > kMaxLUnrolled) continue;
-                q34_combine(M, S, g, blk, n, T.lane, red);
+                q34_combine(M, S, g, blk, n, T.lane, red, DYN);
             }
         }
         double* e = &S.Eq34[((size_t)(blk * kBlock + T.lane) * M.n_fn + n) * kWavesPerWg + wave];
