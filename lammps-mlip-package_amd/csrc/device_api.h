@@ -45,6 +45,10 @@ struct DevModel {
     // zlist[zoff[t] .. zoff[t+1]) are set to zero by the order-1/2 feature pass
     const int32_t* zlist;
     int32_t zoff[3];
+    // order-1/2 polynomial features per centre species, fq12[fq12_off[t] ..
+    // fq12_off[t+1]) (formed in the a_nlm kernel's epilogue)
+    const FeatQ12* fq12;
+    int32_t fq12_off[3];
     // Polynomial features on the device are per species and compact: a
     // centre of species t keeps the ranks [lo, hi) = prange[t] it sees, as
     // the row [n][rank - lo] of n_fn * (hi - lo) doubles in a slot row of
@@ -136,8 +140,13 @@ hipError_t launched(size_t lds_bytes = 0);
 // kAnlmMfma (k_anlm_mfma, chunk products on the matrix cores) where the
 // model fits it (anlm_has_mfma), k_anlm otherwise
 enum AnlmKind : int { kAnlmValu = 0, kAnlmMfma = 1 };
-hipError_t launch_anlm(const DevModel& m, const DevSystem& s, int precision, hipStream_t st, int kind = kAnlmValu);
+// feat: the register kernel also forms the order-1/2 polynomial features
+// (what k_q12<true> does) from the a_nlm it holds; only where
+// anlm_forms_features says the launch can
+hipError_t launch_anlm(const DevModel& m, const DevSystem& s, int precision, hipStream_t st, int kind = kAnlmValu,
+                       bool feat = false);
 bool anlm_has_mfma(const DevModel& m);
+bool anlm_forms_features(const DevModel& m, int precision, int kind);
 hipError_t launch_q12(const DevModel& m, const DevSystem& s, bool feat, hipStream_t st);
 hipError_t launch_q34_adj(const DevModel& m, const DevSystem& s, hipStream_t st);
 hipError_t launch_q34_feat(const DevModel& m, const DevSystem& s, hipStream_t st);

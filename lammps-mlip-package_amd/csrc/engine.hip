@@ -124,12 +124,22 @@ bool upload_model(mlip_ctx* c) {
     }
     std::vector<VisitQ12> q12;
     std::vector<int32_t> q12off;
+    std::vector<FeatQ12> fq12;
     for (int t = 0; t < m.n_type; ++t) {
         const TypeTables& tt = m.types[t];
         const int base = static_cast<int>(q12.size());
         for (int v : tt.q12_off) q12off.push_back(base + v);
         q12.insert(q12.end(), tt.visits_q12.begin(), tt.visits_q12.end());
+        dm.fq12_off[t] = static_cast<int32_t>(fq12.size());
+        // (the radial-only style has no order-1/2 tables: q12_off is empty)
+        const int nl = static_cast<int>(tt.q12_off.size()) - 1;
+        for (int L = 0; L < nl && L <= m.maxl; ++L)
+            for (int k = tt.q12_off[L]; k < tt.q12_off[L + 1]; ++k) {
+                const VisitQ12& v = tt.visits_q12[k];
+                if (v.pslot >= 0) fq12.push_back({v.order, L, v.s0, v.s1, v.pslot - m.prange[t][0], v.coef});
+            }
     }
+    for (int t = m.n_type; t < 3; ++t) dm.fq12_off[t] = static_cast<int32_t>(fq12.size());
     if (!upload_q34_plan(c)) return false;
     // Polynomial features per species, compact: species t keeps the ranks
     // [lo, hi) = prange[t] it sees (the ranks are grouped so that this is
@@ -227,9 +237,11 @@ bool upload_model(mlip_ctx* c) {
               upload(c, c->d_c3off, c3off.data(), c3off.size(), "cubic offsets") &&
               upload(c, c->d_q12, q12.data(), q12.size(), "q12 visits") &&
               upload(c, c->d_q12off, q12off.data(), q12off.size(), "q12 offsets") &&
+              upload(c, c->d_fq12, fq12.data(), fq12.size(), "order-1/2 feature list") &&
               upload(c, c->d_zlist, zlist.data(), zlist.size(), "feature zero list");
     if (!ok) return false;
     dm.zlist = c->d_zlist.p;
+    dm.fq12 = c->d_fq12.p;
     if (m.pair_model) {
         if (!upload(c, c->d_pairw, m.weight.data(), static_cast<size_t>(m.n_des), "pair weights") ||
             !upload(c, c->d_phoff, m.ph_off.data(), m.ph_off.size(), "pair derivative offsets") ||
@@ -593,9 +605,11 @@ bool run_pipeline(mlip_ctx* c) {
             run("struct_reduce", [&] { return launch_struct_reduce(s, c->sbeg, c->send, c->nstruct, c->sres, st); });
         return ok;
     }
-    run("anlm", [&] { return launch_anlm(m, s, c->precision, st, c->anlm_kind); });
+    // the register a_nlm kernel forms the order-1/2 features itself where it can
+    const bool ff = m.n_p > 0 && anlm_forms_features(m, c->precision, c->anlm_kind);
+    run("anlm", [&] { return launch_anlm(m, s, c->precision, st, c->anlm_kind, ff); });
     if (m.n_p > 0) {
-        run("q12_feat", [&] { return launch_q12(m, s, true, st); });
+        if (!ff) run("q12_feat", [&] { return launch_q12(m, s, true, st); });
         if (mm.has_feat34) run("q34_feat", [&] { return launch_q34_feat(m, s, st); });
         if (mm.has_hi) run("qhigh_feat", [&] { return launch_qhigh(m, s, true, st); });
         if (!ok) return false;
@@ -968,6 +982,7 @@ void mlip_destroy(mlip_ctx* c) {
                             &c->soff, &c->d_zlist};
     for (auto* b : ibs) b->release();
     c->d_q12.release();
+    c->d_fq12.release();
     c->d_vadj.release();
     c->d_vfeat.release();
     c->d_gadj.release();
@@ -1345,10 +1360,16 @@ int mlip_autotune(mlip_ctx* c, int reps, int* chosen) {
         for (int kind = 0; kind < 2; ++kind) {
             hipEvent_t a, b;
             if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return MLIP_ERR_DEVICE;
-            hipError_t le = launch_anlm(c->dm, c->ds, c->precision, c->stream, kind);
+            // each with what it leaves to the order-1/2 feature pass
+            const bool ff = c->dm.n_p > 0 && anlm_forms_features(c->dm, c->precision, kind);
+            auto once = [&] {
+                hipError_t e = launch_anlm(c->dm, c->ds, c->precision, c->stream, kind, ff);
+                if (e == hipSuccess && c->dm.n_p > 0 && !ff) e = launch_q12(c->dm, c->ds, true, c->stream);
+                return e;
+            };
+            hipError_t le = once();
             (void)hipEventRecord(a, c->stream);
-            for (int r = 0; r < reps && le == hipSuccess; ++r)
-                le = launch_anlm(c->dm, c->ds, c->precision, c->stream, kind);
+            for (int r = 0; r < reps && le == hipSuccess; ++r) le = once();
             (void)hipEventRecord(b, c->stream);
             float ms = 0.0f;
             const bool ok = le == hipSuccess && hipEventSynchronize(b) == hipSuccess &&

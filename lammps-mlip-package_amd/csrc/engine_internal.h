@@ -59,6 +59,7 @@ struct mlip_ctx {
     // model tables
     mlip::DBuf<double> d_coef, d_weight, d_wmat, d_linp, d_c3w, d_weight_e;
     mlip::DBuf<mlip::VisitQ12> d_q12;
+    mlip::DBuf<mlip::FeatQ12> d_fq12;
     mlip::DBuf<int32_t> d_q12off, d_c3off, d_c3bc, d_pdeg;
     mlip::DBuf<mlip::VisitPack> d_vadj, d_vfeat;
     mlip::DBuf<mlip::GroupQ34> d_gadj, d_gfeat;

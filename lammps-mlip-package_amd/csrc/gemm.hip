@@ -29,7 +29,9 @@ constexpr int kGM = 64;        // rows per workgroup
 constexpr int kGN = 128;       // columns per workgroup
 constexpr int kGK = 16;        // K per LDS stage
 constexpr int kGThreads = 256;
-constexpr int kGPad = 1;       // LDS row padding (doubles)
+// LDS row padding (doubles): rows of the K slabs 80 and 144 doubles apart
+// (measured 2.48 -> 2.35 ms per evaluation at 256k against a pad of 1)
+constexpr int kGPad = 16;
 
 __global__ __launch_bounds__(kGThreads) void k_gemm_rows(const double* __restrict__ A, int lda,
                                                          const double* __restrict__ B, int ldb,

@@ -59,6 +59,14 @@ struct VisitQ12 {
     int32_t coef;      // offset into coefficient array: order 2 -> 2L+1 values (m0=-L..L)
 };
 
+// Order-1/2 polynomial feature of one centre species, formed in the a_nlm
+// kernel's epilogue: d = C a_{s0,0,0} (order 1) or
+// sum_{m0} C(m0) a_{s0,L,m0} a_{s1,L,-m0} (order 2), stored at rank offset
+// col (rank - lo of the species) of each radial index's row segment.
+struct FeatQ12 {
+    int32_t order, L, s0, s1, col, coef;
+};
+
 // Order-3/4 visit: contributions of one (itc, factor position) to the
 // adjoint G_{S,L,m<=0} and, for the designated position, the feature d.
 struct VisitQ34 {

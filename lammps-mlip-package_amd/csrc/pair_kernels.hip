@@ -109,13 +109,21 @@ struct AnlmLds {
     alignas(16) double g[kAnlmJ][GW];
 };
 
+// Radial indices per round of the feature epilogue: a of both species
+// (FP64 pairs) and the round's feature segments must fit the Y_lm chunk
+// buffer (kAnlmJ rows of n_lmh values of T2) they reuse.
+__host__ __device__ inline int anlm_feature_round(int n_lmh, int w, int t2_bytes) {
+    const int buf = kAnlmJ * n_lmh * t2_bytes;
+    return buf / (2 * n_lmh * 16 + w * 8);
+}
+
 template <int NPT, int NTH, typename T, int KL1, int GW>
-__global__ __launch_bounds__(NTH) void k_anlm(DevModel M, DevSystem S, int ngroup) {
+__global__ __launch_bounds__(NTH) void k_anlm(DevModel M, DevSystem S, int ngroup, int feat) {
     constexpr int kAnlmThreads = NTH;
     constexpr int kAnlmParts = NTH / kAnlmJ;
     using T2 = typename Vec2<T>::type;
     __shared__ AnlmLds<T, NTH, KL1, GW> L;
-    extern __shared__ unsigned char anlm_dyn[];
+    extern __shared__ __attribute__((aligned(16))) unsigned char anlm_dyn[];
     T2* ys = reinterpret_cast<T2*>(anlm_dyn);  // [kAnlmJ][n_lmh]
     const int slot = xcd_slot(blockIdx.x, S.nblocks);
     if (slot < 0) return;
@@ -257,6 +265,64 @@ __global__ __launch_bounds__(NTH) void k_anlm(DevModel M, DevSystem S, int ngrou
                 }
             }
         }
+    }
+    if (!feat) return;
+    // Order-1/2 polynomial features of this centre (the order-1/2 feature
+    // pass's d, formed here from the a_nlm the workgroup holds): per round
+    // of R radial indices, a of both species staged in LDS [n][s][lm] over
+    // the Y_lm chunk buffer, thread per (feature, n), features gathered in
+    // an LDS row segment [n][rank - lo] (zero where no order-1/2 family
+    // writes) and stored as one contiguous piece of the slot's compact row.
+    const int blk = slot / kBlock;
+    const int t = S.block_type[blk];
+    const int lo = M.prange[t][0], w = M.prange[t][1] - lo;
+    const int f0 = M.fq12_off[t], nfe = M.fq12_off[t + 1] - f0;
+    const int R = anlm_feature_round(nlmh, w, static_cast<int>(sizeof(T2)));
+    const int nw0 = blockIdx.y * ngroup * NPT, nw1 = min(nw0 + ngroup * NPT, nfn);
+    double2* as = reinterpret_cast<double2*>(anlm_dyn);
+    for (int r0 = nw0; r0 < nw1; r0 += R) {
+        const int rn = min(R, nw1 - r0);
+        double* tile = reinterpret_cast<double*>(as + static_cast<size_t>(rn) * 2 * nlmh);
+        __syncthreads();  // the chunk buffer (or the last round) is free
+        if (act2) {
+#pragma unroll
+            for (int k = 0; k < NPT; ++k) {
+                const int n = nbase + k;
+                if (n >= r0 && n < r0 + rn) {
+                    as[((n - r0) * 2 + 0) * nlmh + lm2] = double2{a0r[k], a0i[k]};
+                    as[((n - r0) * 2 + 1) * nlmh + lm2] = double2{a1r[k], a1i[k]};
+                }
+            }
+        }
+        for (int e = tid; e < rn * w; e += kAnlmThreads) tile[e] = 0.0;
+        __syncthreads();
+        for (int it = tid; it < nfe * rn; it += kAnlmThreads) {
+            const int nl = it / nfe;
+            const FeatQ12 f = M.fq12[f0 + it - nl * nfe];
+            const double* cp = M.coef + f.coef;
+            const double2* x = as + (nl * 2 + f.s0) * nlmh;
+            double d;
+            if (f.order == 1) {
+                d = cp[0] * x[0].x;
+            } else {
+                // the order-1/2 pass's pair-summed form, same order of terms
+                const double2* y = as + (nl * 2 + f.s1) * nlmh;
+                const int L = f.L, b = L * (L + 1) / 2 + L;  // lmh(L, 0); lmh(L, -k) = b - k
+                d = 0.0;
+                for (int k = 0; k <= L; ++k) {
+                    const double sg = (k & 1) ? -1.0 : 1.0;
+                    const double2 xv = x[b - k], yv = y[b - k];
+                    const double cm = cp[L - k], cpk = cp[L + k];
+                    const double re = sg * (xv.x * yv.x + xv.y * yv.y);
+                    d = fma(k == 0 ? cm : cm + cpk, re, d);
+                }
+            }
+            tile[nl * w + f.col] = d;
+        }
+        __syncthreads();
+        // columns r0 w .. (r0 + rn) w of the slot's compact row
+        double* drow = S.D + static_cast<size_t>(slot) * M.n_pr + static_cast<size_t>(r0) * w;
+        for (int e = tid; e < rn * w; e += kAnlmThreads) drow[e] = tile[e];
     }
 }
 
@@ -792,16 +858,17 @@ int anlm_npt(const DevModel& m, int nth, int* ngroup, int* passes) {
 }
 
 template <int NTH, typename T, int KL1, int GW>
-hipError_t launch_anlm_nth(const DevModel& m, const DevSystem& s, int npt, int ngroup, int passes, hipStream_t st) {
+hipError_t launch_anlm_nth(const DevModel& m, const DevSystem& s, int npt, int ngroup, int passes, int feat,
+                           hipStream_t st) {
     const size_t lds = sizeof(typename Vec2<T>::type) * kAnlmJ * m.n_lmh;
     if (lds + sizeof(AnlmLds<T, NTH, KL1, GW>) > lds_limit()) return hipErrorInvalidConfiguration;
     const dim3 grid(slot_grid(s), passes), block(NTH);
     switch (npt) {
-        case 2: hipLaunchKernelGGL((k_anlm<2, NTH, T, KL1, GW>), grid, block, lds, st, m, s, ngroup); break;
-        case 4: hipLaunchKernelGGL((k_anlm<4, NTH, T, KL1, GW>), grid, block, lds, st, m, s, ngroup); break;
-        case 6: hipLaunchKernelGGL((k_anlm<6, NTH, T, KL1, GW>), grid, block, lds, st, m, s, ngroup); break;
-        case 8: hipLaunchKernelGGL((k_anlm<8, NTH, T, KL1, GW>), grid, block, lds, st, m, s, ngroup); break;
-        default: hipLaunchKernelGGL((k_anlm<16, NTH, T, KL1, GW>), grid, block, lds, st, m, s, ngroup); break;
+        case 2: hipLaunchKernelGGL((k_anlm<2, NTH, T, KL1, GW>), grid, block, lds, st, m, s, ngroup, feat); break;
+        case 4: hipLaunchKernelGGL((k_anlm<4, NTH, T, KL1, GW>), grid, block, lds, st, m, s, ngroup, feat); break;
+        case 6: hipLaunchKernelGGL((k_anlm<6, NTH, T, KL1, GW>), grid, block, lds, st, m, s, ngroup, feat); break;
+        case 8: hipLaunchKernelGGL((k_anlm<8, NTH, T, KL1, GW>), grid, block, lds, st, m, s, ngroup, feat); break;
+        default: hipLaunchKernelGGL((k_anlm<16, NTH, T, KL1, GW>), grid, block, lds, st, m, s, ngroup, feat); break;
     }
     return launched();
 }
@@ -835,15 +902,15 @@ hipError_t launch_anlm_mfma_t(const DevModel& m, const DevSystem& s, hipStream_t
 }
 
 template <typename T>
-hipError_t launch_anlm_t(const DevModel& m, const DevSystem& s, hipStream_t st) {
+hipError_t launch_anlm_t(const DevModel& m, const DevSystem& s, int feat, hipStream_t st) {
     // phase 2 needs a thread per lm column: 192 threads up to l_max 18
     const int nth = m.n_lmh <= 192 ? 192 : 256;
     int ngroup = 1, passes = 1;
     const int npt = anlm_npt(m, nth, &ngroup, &passes);
     if (m.maxl < kSmallL1 && passes * ngroup * npt <= kSmallGW)
-        return launch_anlm_nth<192, T, kSmallL1, kSmallGW>(m, s, npt, ngroup, passes, st);
-    if (nth == 192) return launch_anlm_nth<192, T, kL1, kLargeGW>(m, s, npt, ngroup, passes, st);
-    return launch_anlm_nth<256, T, kL1, kLargeGW>(m, s, npt, ngroup, passes, st);
+        return launch_anlm_nth<192, T, kSmallL1, kSmallGW>(m, s, npt, ngroup, passes, feat, st);
+    if (nth == 192) return launch_anlm_nth<192, T, kL1, kLargeGW>(m, s, npt, ngroup, passes, feat, st);
+    return launch_anlm_nth<256, T, kL1, kLargeGW>(m, s, npt, ngroup, passes, feat, st);
 }
 
 template <int NF, int NT, typename T, bool GL, int KL1>
@@ -941,14 +1008,27 @@ void build_ylm_tables(YTab<double>& t64, YTab<float>& t32) {
         }
 }
 
-hipError_t launch_anlm(const DevModel& m, const DevSystem& s, int precision, hipStream_t st, int kind) {
+hipError_t launch_anlm(const DevModel& m, const DevSystem& s, int precision, hipStream_t st, int kind, bool feat) {
     if (s.nblocks == 0) return hipSuccess;
+    if (feat && !anlm_forms_features(m, precision, kind)) return hipErrorInvalidValue;
     if (kind == kAnlmMfma && anlm_mfma_fits(m))
         return precision == kMixed ? launch_anlm_mfma_t<float>(m, s, st) : launch_anlm_mfma_t<double>(m, s, st);
-    return precision == kMixed ? launch_anlm_t<float>(m, s, st) : launch_anlm_t<double>(m, s, st);
+    const int f = feat ? 1 : 0;
+    return precision == kMixed ? launch_anlm_t<float>(m, s, f, st) : launch_anlm_t<double>(m, s, f, st);
 }
 
 bool anlm_has_mfma(const DevModel& m) { return anlm_mfma_fits(m); }
+
+// The register kernel forms the order-1/2 features when a round of at least
+// one radial index fits its chunk buffer for every species' row segment.
+bool anlm_forms_features(const DevModel& m, int precision, int kind) {
+    if (kind == kAnlmMfma && anlm_mfma_fits(m)) return false;
+    if (m.pair_model || m.n_p <= 0) return false;
+    const int t2 = precision == kMixed ? 8 : 16;
+    for (int t = 0; t < m.n_type && t < 2; ++t)
+        if (anlm_feature_round(m.n_lmh, m.prange[t][1] - m.prange[t][0], t2) < 1) return false;
+    return true;
+}
 
 hipError_t launch_pair_energy(const DevModel& m, const DevSystem& s, double* epv, hipStream_t st) {
     if (s.nblocks == 0) return hipSuccess;
