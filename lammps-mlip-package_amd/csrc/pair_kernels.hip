@@ -296,9 +296,11 @@ __global__ __launch_bounds__(NTH) void k_anlm(DevModel M, DevSystem S, int ngrou
         }
         for (int e = tid; e < rn * w; e += kAnlmThreads) tile[e] = 0.0;
         __syncthreads();
+        // item = (feature, n) with n fastest: the features are listed by L,
+        // so a wave's items have nearly the same number of terms
         for (int it = tid; it < nfe * rn; it += kAnlmThreads) {
-            const int nl = it / nfe;
-            const FeatQ12 f = M.fq12[f0 + it - nl * nfe];
+            const int fi = it / rn, nl = it - fi * rn;
+            const FeatQ12 f = M.fq12[f0 + fi];
             const double* cp = M.coef + f.coef;
             const double2* x = as + (nl * 2 + f.s0) * nlmh;
             double d;
