@@ -290,7 +290,9 @@ int mlip_pool_admit(mlip_ctx* ctx, int natoms, const double* cell, const double*
                     int nsteps, double dt, double skin, int* handle);
 /* Admits requests 0..count-1 in order (FIFO) until one does not fit;
  * structures concatenated (natoms[k] atoms each, cells 9 per request, v
- * NULL or concatenated), neighbor lists built on nthreads host threads.
+ * NULL or concatenated).  The neighbor lists of all of them are built in
+ * one batch on the device, on a stream of the pool's own that runs beside a
+ * tick in flight (nthreads is accepted for compatibility and not used).
  * Returns the number admitted (negative MLIP_ERR_* on error). */
 int mlip_pool_admit_many(mlip_ctx* ctx, int count, const int* natoms, const double* cells, const double* x,
                          const int* type, const double* v, const int* nsteps, double dt, const double* skin,

@@ -189,6 +189,21 @@ void pool_free(mlip_ctx* c);
 // build_gather.  Synchronises once to size the pair arrays.
 bool device_lists(mlip_ctx* c, int nstruct, const int32_t* natoms, const double* cells, double rc, int64_t& npairs);
 void nb_free(mlip_ctx* c);
+// The same build with explicit outputs, a cutoff per structure and the
+// work on stream st: positions (concatenated) in x; lists into poff, pnbr,
+// pshift, goff, gent; spairs (nstruct + 1, optional) receives the first pair
+// of each structure.  Synchronises st once.
+struct NbOut {
+    DBuf<int32_t>* poff;
+    DBuf<int32_t>* pnbr;
+    DBuf<double>* pshift;
+    DBuf<int32_t>* goff;
+    DBuf<int32_t>* gent;
+};
+bool device_lists_into(mlip_ctx* c, NbWork*& w, hipStream_t st, int nstruct, const int32_t* natoms,
+                       const double* cells, const double* rcs, const double* x, const NbOut& out, int64_t& npairs,
+                       std::vector<int64_t>* spairs);
+void nb_work_free(NbWork*& w);
 // Training outputs of the resident structure or batch (see
 // mlip_train_features); returns an MLIP_* code.
 int train_features(mlip_ctx* c, double* X, double* P, double* FF, double* SF);

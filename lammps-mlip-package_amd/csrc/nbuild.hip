@@ -38,6 +38,7 @@ namespace mlip {
 
 struct NbCell {
     double cell[9], g[9];
+    double rc2;  // squared cutoff of the structure
     int32_t nb[3], reach[3];
     int32_t bin0, natoms;
 };
@@ -47,9 +48,11 @@ struct NbWork {
     DBuf<int32_t> sid, val, val2, head, pv, pv2, ihead;
     DBuf<uint32_t> key, key2, pk, pk2;
     DBuf<double> xw, wrap;
-    DBuf<int64_t> cnt, off;
+    DBuf<int64_t> cnt, off, first, soff;
     DBuf<char> tmp;
     void release() {
+        first.release();
+        soff.release();
         cells.release();
         sid.release();
         val.release();
@@ -128,12 +131,13 @@ __global__ __launch_bounds__(kNbThreads) void k_nb_pairs(const NbCell* __restric
                                                          const int32_t* __restrict__ head,
                                                          const int32_t* __restrict__ member,
                                                          const double* __restrict__ xw,
-                                                         const double* __restrict__ wrap, double rc2,
+                                                         const double* __restrict__ wrap,
                                                          int64_t* __restrict__ cnt, const int64_t* __restrict__ off,
                                                          int32_t* __restrict__ pnbr, double* __restrict__ pshift) {
     const int a = blockIdx.x * kNbThreads + threadIdx.x;
     if (a >= n) return;
     const NbCell& C = cells[sid[a]];
+    const double rc2 = C.rc2;
     const int nb0 = C.nb[0], nb1 = C.nb[1], nb2 = C.nb[2];
     const int bl = static_cast<int>(key[a]) - C.bin0;
     const int b0 = bl / (nb1 * nb2), b1 = (bl / nb2) % nb1, b2 = bl % nb2;
@@ -203,6 +207,14 @@ __global__ __launch_bounds__(kNbThreads) void k_nb_pair_keys(const int32_t* __re
     pv[p] = p;
 }
 
+// first pair of each structure: off at the structure's first atom
+__global__ __launch_bounds__(kNbThreads) void k_nb_struct_off(const int64_t* __restrict__ off,
+                                                              const int64_t* __restrict__ first_atom, int nstruct,
+                                                              int64_t* __restrict__ out) {
+    const int s = blockIdx.x * kNbThreads + threadIdx.x;
+    if (s <= nstruct) out[s] = off[first_atom[s]];
+}
+
 // Gather list of position k: its own pairs (centre k, entry -(p+1)) and the
 // pairs that have k as neighbor (entry p+1), in increasing p; for a
 // self-image pair (both), the centre entry first.
@@ -241,16 +253,28 @@ bool cub_ok(mlip_ctx* c, hipError_t e, const char* what) { return hip_ok(c, e, w
 
 }  // namespace
 
-bool device_lists(mlip_ctx* c, int nstruct, const int32_t* natoms, const double* cells, double rc,
-                  int64_t& npairs) {
-    if (!c->nbw) c->nbw = new NbWork();
-    NbWork& w = *c->nbw;
-    hipStream_t st = c->stream;
+bool device_lists_into(mlip_ctx* c, NbWork*& wp, hipStream_t st, int nstruct, const int32_t* natoms,
+                       const double* cells, const double* rcs, const double* x, const NbOut& out, int64_t& npairs,
+                       std::vector<int64_t>* spairs) {
+    if (!wp) wp = new NbWork();
+    NbWork& w = *wp;
+    auto up = [&](auto& b, const auto* h, size_t n, const char* what) {
+        if (!b.ensure(n)) {
+            c->err = std::string("device allocation failed for ") + what;
+            return false;
+        }
+        return n == 0 || hip_ok(c, hipMemcpyAsync(b.p, h, n * sizeof(*h), hipMemcpyHostToDevice, st), what);
+    };
     npairs = 0;
     std::vector<NbCell> hc(static_cast<size_t>(nstruct));
     std::vector<int32_t> sid;
+    std::vector<int64_t> first(static_cast<size_t>(nstruct) + 1, 0);
     int64_t bins = 0, n64 = 0;
-    for (int s = 0; s < nstruct; ++s) n64 += natoms[s];
+    for (int s = 0; s < nstruct; ++s) {
+        first[s] = n64;
+        n64 += natoms[s];
+    }
+    first[nstruct] = n64;
     if (n64 > INT_MAX - 1) {
         c->err = "too many atoms";
         return false;
@@ -261,10 +285,11 @@ bool device_lists(mlip_ctx* c, int nstruct, const int32_t* natoms, const double*
         std::memset(&h, 0, sizeof h);
         h.bin0 = static_cast<int32_t>(bins);
         h.natoms = natoms[s];
+        h.rc2 = rcs[s] * rcs[s];
         if (natoms[s] == 0) continue;
         CellBins cb;
         std::string e;
-        if (!cell_bins(cells + 9 * static_cast<size_t>(s), rc, cb, e)) {
+        if (!cell_bins(cells + 9 * static_cast<size_t>(s), rcs[s], cb, e)) {
             c->err = "structure " + std::to_string(s) + ": " + e;
             return false;
         }
@@ -284,21 +309,24 @@ bool device_lists(mlip_ctx* c, int nstruct, const int32_t* natoms, const double*
         sid.insert(sid.end(), natoms[s], s);
     }
     const int n = static_cast<int>(n64);
-    if (!c->poff.ensure(static_cast<size_t>(n) + 1) || !c->goff.ensure(static_cast<size_t>(n) + 1)) {
+    if (!out.poff->ensure(static_cast<size_t>(n) + 1) || !out.goff->ensure(static_cast<size_t>(n) + 1)) {
         c->err = "device allocation failed for neighbor offsets";
         return false;
     }
     if (n == 0) {
-        return hip_ok(c, hipMemsetAsync(c->poff.p, 0, sizeof(int32_t), st), "neighbor offsets") &&
-               hip_ok(c, hipMemsetAsync(c->goff.p, 0, sizeof(int32_t), st), "gather offsets");
+        if (spairs) spairs->assign(static_cast<size_t>(nstruct) + 1, 0);
+        return hip_ok(c, hipMemsetAsync(out.poff->p, 0, sizeof(int32_t), st), "neighbor offsets") &&
+               hip_ok(c, hipMemsetAsync(out.goff->p, 0, sizeof(int32_t), st), "gather offsets") &&
+               hip_ok(c, hipStreamSynchronize(st), "neighbor offsets");
     }
     const int nbins = static_cast<int>(bins);
-    if (!upload(c, w.cells, hc.data(), hc.size(), "bin grids") || !upload(c, w.sid, sid.data(), sid.size(), "atoms"))
+    if (!up(w.cells, hc.data(), hc.size(), "bin grids") || !up(w.sid, sid.data(), sid.size(), "atoms") ||
+        !up(w.first, first.data(), first.size(), "structure starts"))
         return false;
     if (!w.xw.ensure(3 * static_cast<size_t>(n)) || !w.wrap.ensure(3 * static_cast<size_t>(n)) ||
         !w.key.ensure(n) || !w.key2.ensure(n) || !w.val.ensure(n) || !w.val2.ensure(n) ||
         !w.head.ensure(static_cast<size_t>(nbins) + 1) || !w.cnt.ensure(static_cast<size_t>(n) + 1) ||
-        !w.off.ensure(static_cast<size_t>(n) + 1)) {
+        !w.off.ensure(static_cast<size_t>(n) + 1) || !w.soff.ensure(static_cast<size_t>(nstruct) + 1)) {
         c->err = "device allocation failed for the neighbor build";
         return false;
     }
@@ -313,24 +341,28 @@ bool device_lists(mlip_ctx* c, int nstruct, const int32_t* natoms, const double*
         c->err = "device allocation failed for the neighbor build";
         return false;
     }
-    k_nb_bin<<<grid_of(n), kNbThreads, 0, st>>>(w.cells.p, w.sid.p, n, c->x.p, w.xw.p, w.wrap.p, w.key.p, w.val.p);
+    k_nb_bin<<<grid_of(n), kNbThreads, 0, st>>>(w.cells.p, w.sid.p, n, x, w.xw.p, w.wrap.p, w.key.p, w.val.p);
     size_t tb = w.tmp.cap;
     if (!cub_ok(c, hipcub::DeviceRadixSort::SortPairs(w.tmp.p, tb, w.key.p, w.key2.p, w.val.p, w.val2.p, n, 0,
                                                       key_bits(nbins), st),
                 "bin sort"))
         return false;
     k_nb_heads<<<grid_of(static_cast<int64_t>(nbins) + 1), kNbThreads, 0, st>>>(w.key2.p, n, nbins, w.head.p);
-    const double rc2 = rc * rc;
     k_nb_pairs<false><<<grid_of(n), kNbThreads, 0, st>>>(w.cells.p, w.sid.p, n, w.key.p, w.head.p, w.val2.p, w.xw.p,
-                                                         w.wrap.p, rc2, w.cnt.p, nullptr, nullptr, nullptr);
+                                                         w.wrap.p, w.cnt.p, nullptr, nullptr, nullptr);
     if (!hip_ok(c, hipMemsetAsync(w.cnt.p + n, 0, sizeof(int64_t), st), "neighbor counts")) return false;
     tb = w.tmp.cap;
     if (!cub_ok(c, hipcub::DeviceScan::ExclusiveSum(w.tmp.p, tb, w.cnt.p, w.off.p, n + 1, st), "neighbor scan"))
         return false;
-    int64_t total = 0;
-    if (!hip_ok(c, hipMemcpyAsync(&total, w.off.p + n, sizeof total, hipMemcpyDeviceToHost, st), "pair count") ||
+    std::vector<int64_t> soff(static_cast<size_t>(nstruct) + 1, 0);
+    k_nb_struct_off<<<grid_of(static_cast<int64_t>(nstruct) + 1), kNbThreads, 0, st>>>(w.off.p, w.first.p, nstruct,
+                                                                                       w.soff.p);
+    if (!hip_ok(c, hipMemcpyAsync(soff.data(), w.soff.p, soff.size() * sizeof(int64_t), hipMemcpyDeviceToHost, st),
+                "pair count") ||
         !hip_ok(c, hipStreamSynchronize(st), "pair count"))
         return false;
+    const int64_t total = soff[nstruct];
+    if (spairs) *spairs = soff;
     // gather offsets reach 2 * npairs and are int32
     if (total > (INT_MAX - 1) / 2) {
         c->err = "more than 2^30 pairs";
@@ -338,16 +370,16 @@ bool device_lists(mlip_ctx* c, int nstruct, const int32_t* natoms, const double*
     }
     const int np = static_cast<int>(total);
     npairs = total;
-    if (!c->pnbr.ensure(np) || !c->pshift.ensure(3 * static_cast<size_t>(np)) ||
-        !c->gent.ensure(2 * static_cast<size_t>(np)) || !w.ihead.ensure(static_cast<size_t>(n) + 1)) {
+    if (!out.pnbr->ensure(np) || !out.pshift->ensure(3 * static_cast<size_t>(np)) ||
+        !out.gent->ensure(2 * static_cast<size_t>(np)) || !w.ihead.ensure(static_cast<size_t>(n) + 1)) {
         c->err = "device allocation failed for the neighbor list";
         return false;
     }
     k_nb_pairs<true><<<grid_of(n), kNbThreads, 0, st>>>(w.cells.p, w.sid.p, n, w.key.p, w.head.p, w.val2.p, w.xw.p,
-                                                        w.wrap.p, rc2, nullptr, w.off.p, c->pnbr.p, c->pshift.p);
-    k_nb_off32<<<grid_of(static_cast<int64_t>(n) + 1), kNbThreads, 0, st>>>(w.off.p, n, c->poff.p);
+                                                        w.wrap.p, nullptr, w.off.p, out.pnbr->p, out.pshift->p);
+    k_nb_off32<<<grid_of(static_cast<int64_t>(n) + 1), kNbThreads, 0, st>>>(w.off.p, n, out.poff->p);
     if (np == 0) {
-        return hip_ok(c, hipMemsetAsync(c->goff.p, 0, (static_cast<size_t>(n) + 1) * sizeof(int32_t), st),
+        return hip_ok(c, hipMemsetAsync(out.goff->p, 0, (static_cast<size_t>(n) + 1) * sizeof(int32_t), st),
                       "gather offsets");
     }
     // incoming pairs of every position, in pair order
@@ -364,23 +396,32 @@ bool device_lists(mlip_ctx* c, int nstruct, const int32_t* natoms, const double*
         c->err = "device allocation failed for the gather build";
         return false;
     }
-    k_nb_pair_keys<<<grid_of(np), kNbThreads, 0, st>>>(c->pnbr.p, np, w.pk.p, w.pv.p);
+    k_nb_pair_keys<<<grid_of(np), kNbThreads, 0, st>>>(out.pnbr->p, np, w.pk.p, w.pv.p);
     tb = w.tmp.cap;
     if (!cub_ok(c, hipcub::DeviceRadixSort::SortPairs(w.tmp.p, tb, w.pk.p, w.pk2.p, w.pv.p, w.pv2.p, np, 0,
                                                       key_bits(n), st),
                 "pair sort"))
         return false;
     k_nb_heads<<<grid_of(static_cast<int64_t>(n) + 1), kNbThreads, 0, st>>>(w.pk2.p, np, n, w.ihead.p);
-    k_nb_gather<<<grid_of(static_cast<int64_t>(n) + 1), kNbThreads, 0, st>>>(c->poff.p, w.ihead.p, w.pv2.p, n,
-                                                                             c->goff.p, c->gent.p);
+    k_nb_gather<<<grid_of(static_cast<int64_t>(n) + 1), kNbThreads, 0, st>>>(out.poff->p, w.ihead.p, w.pv2.p, n,
+                                                                             out.goff->p, out.gent->p);
     return hip_ok(c, hipGetLastError(), "neighbor build");
 }
 
-void nb_free(mlip_ctx* c) {
-    if (!c->nbw) return;
-    c->nbw->release();
-    delete c->nbw;
-    c->nbw = nullptr;
+bool device_lists(mlip_ctx* c, int nstruct, const int32_t* natoms, const double* cells, double rc,
+                  int64_t& npairs) {
+    const std::vector<double> rcs(static_cast<size_t>(nstruct), rc);
+    const NbOut out{&c->poff, &c->pnbr, &c->pshift, &c->goff, &c->gent};
+    return device_lists_into(c, c->nbw, c->stream, nstruct, natoms, cells, rcs.data(), c->x.p, out, npairs, nullptr);
 }
+
+void nb_work_free(NbWork*& w) {
+    if (!w) return;
+    w->release();
+    delete w;
+    w = nullptr;
+}
+
+void nb_free(mlip_ctx* c) { nb_work_free(c->nbw); }
 
 }  // namespace mlip

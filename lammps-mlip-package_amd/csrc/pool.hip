@@ -3,8 +3,9 @@
 //
 // Every admitted request owns an atom range and a pair range of the pool's
 // device arrays (positions, velocities, element, per-centre pair ranges with
-// image shifts, force-gather lists), filled once at admission from a native
-// neighbor list built with a skin.  A tick evaluates all resident requests
+// image shifts, force-gather lists), filled once at admission from a
+// neighbor list built with a skin on the device (the batch builder of
+// nbuild.hip, on a stream of its own so that it runs beside a tick).  A tick evaluates all resident requests
 // as one batch: the slot blocks of the pipeline mix atoms of different
 // requests (same species per block), pairs and gather entries stay in each
 // request's own ranges, and per-request energy, virial and kinetic-energy
@@ -28,10 +29,8 @@
 
 #include <algorithm>
 #include <cstring>
-#include <atomic>
 #include <map>
 #include <memory>
-#include <thread>
 #include <string>
 #include <vector>
 
@@ -149,6 +148,47 @@ __global__ void k_scatter(const ScatterDesc* d, const unsigned char* stage) {
         dst[w] = src[w];
 }
 
+// A request's lists built in an admission batch (batch atom start A0, first
+// pair P0) and its pool ranges (atoms a0.., pairs p0..).
+struct AdmDesc {
+    int64_t A0, P0, np, a0, p0;
+    int32_t n, pad;
+};
+
+// Places admitted requests' device-built lists into their pool ranges:
+// neighbor indices and signed gather entries (+-(p + 1)) move by the range
+// offsets, image shifts are copied, per-atom pair and gather ranges come
+// from the batch offsets.
+__global__ void k_adm_place(const AdmDesc* d, const int32_t* __restrict__ bpoff, const int32_t* __restrict__ bgoff,
+                            const int32_t* __restrict__ bpnbr, const double* __restrict__ bshift,
+                            const int32_t* __restrict__ bgent, int32_t* pbeg, int32_t* pend, int32_t* gbeg,
+                            int32_t* gend, int32_t* pnbr, double* pshift, int32_t* gent) {
+    const AdmDesc q = d[blockIdx.y];
+    const int64_t dp = q.p0 - q.P0;
+    const int64_t m = q.n > 2 * q.np ? static_cast<int64_t>(q.n) : 2 * q.np;
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < m; i += stride) {
+        if (i < q.n) {
+            const int64_t a = q.A0 + i, o = q.a0 + i;
+            pbeg[o] = static_cast<int32_t>(bpoff[a] + dp);
+            pend[o] = static_cast<int32_t>(bpoff[a + 1] + dp);
+            gbeg[o] = static_cast<int32_t>(bgoff[a] + 2 * dp);
+            gend[o] = static_cast<int32_t>(bgoff[a + 1] + 2 * dp);
+        }
+        if (i < q.np) {
+            const int64_t e = q.P0 + i, o = q.p0 + i;
+            pnbr[o] = static_cast<int32_t>(bpnbr[e] - q.A0 + q.a0);
+            pshift[3 * o] = bshift[3 * e];
+            pshift[3 * o + 1] = bshift[3 * e + 1];
+            pshift[3 * o + 2] = bshift[3 * e + 2];
+        }
+        if (i < 2 * q.np) {
+            const int32_t en = bgent[2 * q.P0 + i];
+            gent[2 * q.p0 + i] = static_cast<int32_t>(en > 0 ? en + dp : en - dp);
+        }
+    }
+}
+
 struct HostArena {
     unsigned char* h = nullptr;
     size_t cap = 0;
@@ -251,6 +291,16 @@ struct Pool {
     std::vector<ScatterDesc> descs;
     DBuf<unsigned char> dstage;
     DBuf<ScatterDesc> ddesc;
+    // admissions: lists built on the device on their own stream (while a
+    // tick runs), then placed into the requests' ranges on the pool stream
+    NbWork* nbw = nullptr;
+    hipStream_t ast = nullptr;
+    hipEvent_t built = nullptr;   // admission lists complete on the admission stream
+    hipEvent_t placed = nullptr;  // last placement done (its inputs may be reused)
+    bool placed_pending = false;
+    DBuf<double> ax, apshift;
+    DBuf<int32_t> apoff, apnbr, agoff, agent;
+    DBuf<AdmDesc> adesc;
 };
 
 void pool_free(mlip_ctx* c) {
@@ -271,6 +321,18 @@ void pool_free(mlip_ctx* c) {
     p->dnext.release();
     p->dstage.release();
     p->ddesc.release();
+    if (p->ast) (void)hipStreamSynchronize(p->ast);
+    nb_work_free(p->nbw);
+    p->ax.release();
+    p->apshift.release();
+    p->apoff.release();
+    p->apnbr.release();
+    p->agoff.release();
+    p->agent.release();
+    p->adesc.release();
+    if (p->placed) (void)hipEventDestroy(p->placed);
+    if (p->built) (void)hipEventDestroy(p->built);
+    if (p->ast) (void)hipStreamDestroy(p->ast);
     for (auto& a : p->arena) {
         if (a.h) (void)hipHostFree(a.h);
         if (a.ev) (void)hipEventDestroy(a.ev);
@@ -428,7 +490,10 @@ int mlip_pool_init(mlip_ctx* c, int64_t max_atoms, int64_t max_pairs, int max_re
                     p->out.ensure(8) && p->sout.ensure(7 * nr) && p->ke.ensure(6 * nr) && p->sbeg.ensure(nr) &&
                     p->send.ensure(nr) && p->first.ensure(nr) && p->final_.ensure(nr) && p->disp.ensure(nr) &&
                     p->dnext.ensure(nr) && p->xref.ensure(3 * na) && p->mass.ensure(static_cast<size_t>(m.n_type));
-    if (!ok || hipHostMalloc(reinterpret_cast<void**>(&p->h_out), 14 * nr * sizeof(double)) != hipSuccess) {
+    if (!ok || hipHostMalloc(reinterpret_cast<void**>(&p->h_out), 14 * nr * sizeof(double)) != hipSuccess ||
+        hipStreamCreateWithFlags(&p->ast, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&p->placed, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&p->built, hipEventDisableTiming) != hipSuccess) {
         c->err = "device allocation failed for the request pool";
         pool_free(c);
         return MLIP_ERR_DEVICE;
@@ -458,172 +523,139 @@ int mlip_pool_init(mlip_ctx* c, int64_t max_atoms, int64_t max_pairs, int max_re
 namespace mlip {
 namespace {
 
-// Host half of an admission that needs no pool state (runs on worker
-// threads): element mapping, the neighbor list at cutoff + skin and the
-// request-local force-gather lists.
-struct Prepared {
-    int rc = MLIP_OK;
-    std::string err;
-    std::vector<int32_t> elem;
-    NeighborList nl;
-    std::vector<int32_t> loff, goff, gent;
-    int64_t a0 = -1, p0 = -1;        // ranges, once placed
-    size_t off[11] = {};             // arena offsets of the eleven pieces
-};
-
-void prepare(const mlip_ctx* c, int natoms, const double* cell, const double* x, const int* type, double skin,
-             Prepared& out, int inner_threads = 1) {
-    out.elem.resize(natoms);
-    for (int i = 0; i < natoms; ++i) {
-        const int t = type[i];
-        if (t < 1 || t > static_cast<int>(c->elem_of_type.size()) || c->elem_of_type[t - 1] < 0) {
-            out.rc = MLIP_ERR_ARG;
-            out.err = "atom type without an element mapping";
-            return;
+// Admission with lists built on the device.  The requests 0..count-1 are
+// checked in order (element mapping, finite positions, cell at cutoff +
+// skin); their lists are built in one batch on the admission stream, which
+// runs beside a tick in flight and is waited for alone to learn each
+// request's pair count; ranges are then taken in FIFO order until one does
+// not fit.  Positions, velocities, elements and reference positions go up
+// through the pinned staging arena, and one kernel on the pool stream
+// places the lists into the requests' ranges, ahead of the next tick.
+// Returns the number admitted, or -MLIP_ERR_*.
+int admit_device(mlip_ctx* c, Pool* p, int count, const int* natoms, const int64_t* at, const double* cells,
+                 const double* x, const int* type, const double* v, const int* nsteps, double dt,
+                 const double* skins, int* handles) {
+    const double rcut = c->model.cutoff;
+    std::vector<std::vector<int32_t>> elem(static_cast<size_t>(count));
+    std::vector<double> rcs(static_cast<size_t>(count));
+    int nc = 0;
+    std::string first_err;
+    for (; nc < count; ++nc) {
+        std::string e;
+        bool ok = true;
+        elem[nc].resize(natoms[nc]);
+        for (int i = 0; i < natoms[nc] && ok; ++i) {
+            const int t = type[at[nc] + i];
+            if (t < 1 || t > static_cast<int>(c->elem_of_type.size()) || c->elem_of_type[t - 1] < 0) {
+                e = "atom type without an element mapping";
+                ok = false;
+            } else {
+                elem[nc][i] = c->elem_of_type[t - 1];
+            }
         }
-        out.elem[i] = c->elem_of_type[t - 1];
+        for (int64_t i = 3 * at[nc]; ok && i < 3 * at[nc + 1]; ++i)
+            if (!std::isfinite(x[i])) {
+                e = "non-finite position";
+                ok = false;
+            }
+        rcs[nc] = rcut + skins[nc];
+        CellBins cb;
+        if (ok && !cell_bins(cells + 9 * static_cast<size_t>(nc), rcs[nc], cb, e)) ok = false;
+        if (!ok) {
+            first_err = "request structure: " + e;
+            break;
+        }
     }
-    std::string e;
-    if (!build_full_list(cell, natoms, x, c->model.cutoff + skin, out.nl, e, inner_threads)) {
-        out.rc = MLIP_ERR_ARG;
-        out.err = "request structure: " + e;
-        return;
+    if (nc == 0) {
+        c->err = first_err;
+        return -MLIP_ERR_ARG;
     }
-    out.loff.resize(natoms + 1);
-    for (int i = 0; i <= natoms; ++i) out.loff[i] = static_cast<int32_t>(out.nl.off[i]);
-    build_gather(natoms, out.loff, out.nl.idx, nullptr, out.goff, out.gent);
-}
-
-template <typename F>
-void parallel_for(int count, int nthreads, F&& f) {
-    const int nt = std::max(1, std::min(nthreads, count));
-    std::atomic<int> next{0};
-    auto work = [&]() {
-        for (int k = next++; k < count; k = next++) f(k);
-    };
-    std::vector<std::thread> th;
-    for (int t = 1; t < nt; ++t) th.emplace_back(work);
-    work();
-    for (auto& t : th) t.join();
-}
-
-// Admits prepared requests 0..count-1 in order until one does not fit:
-// ranges are taken sequentially (FIFO), then every admitted request's
-// pieces are written into the pinned arena by worker threads and one packed
-// copy moves them.  Returns the number admitted, or -MLIP_ERR_*.
-int commit_many(mlip_ctx* c, Pool* p, std::vector<Prepared>& pr, int count, const int* natoms, const int64_t* at,
-                const double* cells, const double* x, const double* v, const int* nsteps, double dt,
-                const double* skins, int nthreads, int* handles) {
+    nc = std::min<int>(nc, std::max(0, p->cap_req - static_cast<int>(p->req.size())));
+    if (nc == 0) {
+        c->err = "request pool: capacity exhausted";
+        return 0;
+    }
+    // the admission buffers are free once the last placement has run
+    if (p->placed_pending && !hip_ok(c, hipEventSynchronize(p->placed), "admission buffers")) return -MLIP_ERR_DEVICE;
+    p->placed_pending = false;
+    const size_t n3 = 3 * static_cast<size_t>(at[nc]);
+    if (!p->ax.ensure(std::max<size_t>(n3, 1))) {
+        c->err = "device allocation failed for admissions";
+        return -MLIP_ERR_DEVICE;
+    }
+    if (!hip_ok(c, hipMemcpyAsync(p->ax.p, x, n3 * sizeof(double), hipMemcpyHostToDevice, p->ast), "admission positions"))
+        return -MLIP_ERR_DEVICE;
+    const NbOut out{&p->apoff, &p->apnbr, &p->apshift, &p->agoff, &p->agent};
+    int64_t total = 0;
+    std::vector<int64_t> spairs;
+    if (!device_lists_into(c, p->nbw, p->ast, nc, natoms, cells, rcs.data(), p->ax.p, out, total, &spairs) ||
+        !hip_ok(c, hipEventRecord(p->built, p->ast), "admission lists"))
+        return -MLIP_ERR_DEVICE;
+    // ranges in FIFO order
+    std::vector<AdmDesc> dd;
     int n = 0;
-    for (; n < count; ++n) {
-        Prepared& q = pr[n];
-        if (q.rc != MLIP_OK) {
-            c->err = q.err;
-            if (n == 0) return -q.rc;
+    for (; n < nc; ++n) {
+        const int64_t np = spairs[n + 1] - spairs[n];
+        const int64_t a0 = p->atoms.take(natoms[n]);
+        if (a0 < 0) break;
+        const int64_t p0 = p->pairs.take(std::max<int64_t>(np, 1));
+        if (p0 < 0) {
+            p->atoms.give(a0, natoms[n]);
             break;
         }
-        if (static_cast<int>(p->req.size()) + n >= p->cap_req) break;
-        const int64_t np = static_cast<int64_t>(q.nl.idx.size());
-        q.a0 = p->atoms.take(natoms[n]);
-        if (q.a0 < 0) break;
-        q.p0 = p->pairs.take(std::max<int64_t>(np, 1));
-        if (q.p0 < 0) {
-            p->atoms.give(q.a0, natoms[n]);
-            q.a0 = -1;
-            break;
-        }
+        dd.push_back({at[n], spairs[n], np, a0, p0, natoms[n], 0});
     }
     if (n == 0) {
         c->err = "request pool: capacity exhausted";
         return 0;
     }
-    // arena layout: eleven 16-byte-aligned pieces per request
+    // host data through the staging arena; the descriptors too
     if (!stage_begin(c, p)) return -MLIP_ERR_DEVICE;
-    HostArena& ar = p->arena[p->cur];
-    size_t fill = 0;
-    auto place = [&](size_t bytes) {
-        const size_t o = (fill + 15) & ~static_cast<size_t>(15);
-        fill = o + bytes;
-        return o;
-    };
+    std::vector<double> zeros;
     for (int k = 0; k < n; ++k) {
-        Prepared& q = pr[k];
-        const size_t na = static_cast<size_t>(natoms[k]), np = q.nl.idx.size();
-        const size_t bytes[11] = {24 * na, 24 * na, 4 * na, 4 * na, 4 * na, 4 * na, 4 * na, 4 * np, 24 * np, 8 * np,
-                                  24 * na};
-        double* dx = p->x.p + 3 * q.a0;
-        void* dst[11] = {dx,
-                         p->v.p + 3 * q.a0,
-                         p->elem.p + q.a0,
-                         p->pbeg.p + q.a0,
-                         p->pend.p + q.a0,
-                         p->gbeg.p + q.a0,
-                         p->gend.p + q.a0,
-                         p->pnbr.p + q.p0,
-                         p->pshift.p + 3 * q.p0,
-                         p->gent.p + 2 * q.p0,
-                         p->xref.p + 3 * q.a0};
-        for (int j = 0; j < 11; ++j) {
-            q.off[j] = place(bytes[j]);
-            if (bytes[j]) p->descs.push_back({q.off[j], reinterpret_cast<uint64_t>(dst[j]), bytes[j] / 4});
-        }
+        const size_t na = static_cast<size_t>(natoms[k]);
+        const double* xk = x + 3 * at[k];
+        const int64_t a0 = dd[k].a0;
+        if (v == nullptr && zeros.size() < 3 * na) zeros.assign(3 * na, 0.0);
+        if (!stage_upload(c, p, p->x.p + 3 * a0, xk, 3 * na, "positions") ||
+            !stage_upload(c, p, p->xref.p + 3 * a0, xk, 3 * na, "reference positions") ||
+            !stage_upload(c, p, p->v.p + 3 * a0, v ? v + 3 * at[k] : zeros.data(), 3 * na, "velocities") ||
+            !stage_upload(c, p, p->elem.p + a0, elem[k].data(), na, "elements"))
+            return -MLIP_ERR_DEVICE;
     }
-    if (!arena_reserve(c, ar, fill, 0)) return -MLIP_ERR_DEVICE;
-    parallel_for(n, nthreads, [&](int k) {
-        Prepared& q = pr[k];
-        const int na = natoms[k];
-        const int64_t np = static_cast<int64_t>(q.nl.idx.size());
-        unsigned char* h = ar.h;
-        std::memcpy(h + q.off[0], x + 3 * at[k], 24 * static_cast<size_t>(na));
-        std::memcpy(h + q.off[10], x + 3 * at[k], 24 * static_cast<size_t>(na));
-        if (v)
-            std::memcpy(h + q.off[1], v + 3 * at[k], 24 * static_cast<size_t>(na));
-        else
-            std::memset(h + q.off[1], 0, 24 * static_cast<size_t>(na));
-        std::memcpy(h + q.off[2], q.elem.data(), 4 * static_cast<size_t>(na));
-        int32_t* pb = reinterpret_cast<int32_t*>(h + q.off[3]);
-        int32_t* pe = reinterpret_cast<int32_t*>(h + q.off[4]);
-        int32_t* gb = reinterpret_cast<int32_t*>(h + q.off[5]);
-        int32_t* ge = reinterpret_cast<int32_t*>(h + q.off[6]);
-        const int64_t g0 = 2 * q.p0;
-        for (int i = 0; i < na; ++i) {
-            pb[i] = static_cast<int32_t>(q.p0 + q.loff[i]);
-            pe[i] = static_cast<int32_t>(q.p0 + q.loff[i + 1]);
-            gb[i] = static_cast<int32_t>(g0 + q.goff[i]);
-            ge[i] = static_cast<int32_t>(g0 + q.goff[i + 1]);
-        }
-        int32_t* nb = reinterpret_cast<int32_t*>(h + q.off[7]);
-        for (int64_t e = 0; e < np; ++e) nb[e] = static_cast<int32_t>(q.a0 + q.nl.idx[e]);
-        std::memcpy(h + q.off[8], q.nl.shift.data(), 24 * static_cast<size_t>(np));
-        int32_t* gt = reinterpret_cast<int32_t*>(h + q.off[9]);
-        for (size_t e = 0; e < q.gent.size(); ++e) {
-            const int32_t en = q.gent[e];
-            gt[e] = en > 0 ? static_cast<int32_t>(en + q.p0) : static_cast<int32_t>(en - q.p0);
-        }
-    });
-    p->fill = fill;
-    if (!stage_flush(c, p)) return -MLIP_ERR_DEVICE;
+    if (!p->adesc.ensure(dd.size())) {
+        c->err = "device allocation failed for admissions";
+        return -MLIP_ERR_DEVICE;
+    }
+    if (!stage_upload(c, p, p->adesc.p, dd.data(), dd.size(), "admission descriptors") || !stage_flush(c, p))
+        return -MLIP_ERR_DEVICE;
+    int64_t most = 1;
+    for (const AdmDesc& q : dd) most = std::max<int64_t>(most, std::max<int64_t>(q.n, 2 * q.np));
+    const unsigned gx = static_cast<unsigned>(std::min<int64_t>((most + 255) / 256, 64));
+    if (!hip_ok(c, hipStreamWaitEvent(c->stream, p->built, 0), "admission lists")) return -MLIP_ERR_DEVICE;
+    hipLaunchKernelGGL(k_adm_place, dim3(gx, static_cast<unsigned>(n)), dim3(256), 0, c->stream, p->adesc.p,
+                       p->apoff.p, p->agoff.p, p->apnbr.p, p->apshift.p, p->agent.p, p->pbeg.p, p->pend.p,
+                       p->gbeg.p, p->gend.p, p->pnbr.p, p->pshift.p, p->gent.p);
+    if (!hip_ok(c, hipGetLastError(), "admission placement") ||
+        !hip_ok(c, hipEventRecord(p->placed, c->stream), "admission placement"))
+        return -MLIP_ERR_DEVICE;
+    p->placed_pending = true;
     for (int k = 0; k < n; ++k) {
-        Prepared& q = pr[k];
         Request r;
         r.natoms = natoms[k];
         r.nsteps = nsteps[k];
         r.dt = dt;
         r.skin = skins[k];
         std::copy(cells + 9 * k, cells + 9 * k + 9, r.cell);
-        r.a0 = q.a0;
-        r.p0 = q.p0;
-        r.np = static_cast<int64_t>(q.nl.idx.size());
-        r.elem = std::move(q.elem);
+        r.a0 = dd[k].a0;
+        r.p0 = dd[k].p0;
+        r.np = dd[k].np;
+        r.elem = std::move(elem[k]);
         r.id = p->next_id++;
         handles[k] = r.id;
         p->req.emplace(r.id, std::move(r));
     }
     return n;
-}
-
-int default_threads() {
-    return static_cast<int>(std::min(16u, std::max(1u, std::thread::hardware_concurrency())));
 }
 
 // Frees a request's device ranges (its atoms own no pairs or gather entries
@@ -642,45 +674,50 @@ bool release_request(mlip_ctx* c, Pool* p, Request& r) {
 
 // New neighbor lists (cutoff + skin, from the current positions) for NVE
 // requests whose atoms would pass skin / 2 in their next step; runs between
-// ticks (the stream is idle).  A request whose new list does not fit the
-// pair capacity fails with an error (reported by mlip_pool_result).
+// ticks (the pool stream is idle), on the device: the requests' positions
+// are gathered into the admission buffer, built as one batch, and placed
+// into fresh pair ranges like an admission; the positions become the new
+// reference positions.  A request whose new list does not fit the pair
+// capacity fails with an error (reported by mlip_pool_result).
 int rebuild_lists(mlip_ctx* c, Pool* p, const std::vector<int>& ids, int* ndone) {
-    struct Job {
-        Request* r;
-        std::vector<double> x;
-        NeighborList nl;
-        std::vector<int32_t> loff, goff, gent;
-        std::string err;
-        bool ok = true;
-    };
-    std::vector<Job> jobs(ids.size());
-    for (size_t k = 0; k < ids.size(); ++k) {
-        Request& r = p->req[ids[k]];
-        jobs[k].r = &r;
-        jobs[k].x.resize(3 * static_cast<size_t>(r.natoms));
-        if (!hip_ok(c, hipMemcpy(jobs[k].x.data(), p->x.p + 3 * r.a0, jobs[k].x.size() * 8, hipMemcpyDeviceToHost),
-                    "pool positions for a list rebuild"))
-            return MLIP_ERR_DEVICE;
+    const int nr = static_cast<int>(ids.size());
+    std::vector<Request*> rq(nr);
+    std::vector<int32_t> natoms(nr);
+    std::vector<int64_t> at(static_cast<size_t>(nr) + 1, 0);
+    std::vector<double> cells(9 * static_cast<size_t>(nr)), rcs(nr);
+    for (int k = 0; k < nr; ++k) {
+        rq[k] = &p->req[ids[k]];
+        natoms[k] = rq[k]->natoms;
+        at[k + 1] = at[k] + natoms[k];
+        std::copy(rq[k]->cell, rq[k]->cell + 9, cells.begin() + 9 * k);
+        rcs[k] = c->model.cutoff + rq[k]->skin;
     }
-    const double rc = c->model.cutoff;
-    parallel_for(static_cast<int>(jobs.size()), default_threads(), [&](int k) {
-        Job& j = jobs[k];
-        j.ok = build_full_list(j.r->cell, j.r->natoms, j.x.data(), rc + j.r->skin, j.nl, j.err, 1);
-        if (!j.ok) return;
-        j.loff.resize(j.r->natoms + 1);
-        for (int i = 0; i <= j.r->natoms; ++i) j.loff[i] = static_cast<int32_t>(j.nl.off[i]);
-        build_gather(j.r->natoms, j.loff, j.nl.idx, nullptr, j.goff, j.gent);
-    });
-    if (!stage_begin(c, p)) return MLIP_ERR_DEVICE;
-    for (Job& j : jobs) {
-        Request& r = *j.r;
-        const int64_t np = j.ok ? static_cast<int64_t>(j.nl.idx.size()) : 0;
+    if (p->placed_pending && !hip_ok(c, hipEventSynchronize(p->placed), "rebuild buffers")) return MLIP_ERR_DEVICE;
+    p->placed_pending = false;
+    if (!p->ax.ensure(std::max<size_t>(3 * static_cast<size_t>(at[nr]), 1))) {
+        c->err = "device allocation failed for list rebuilds";
+        return MLIP_ERR_DEVICE;
+    }
+    for (int k = 0; k < nr; ++k)
+        if (!hip_ok(c, hipMemcpyAsync(p->ax.p + 3 * at[k], p->x.p + 3 * rq[k]->a0, 24 * static_cast<size_t>(natoms[k]),
+                                      hipMemcpyDeviceToDevice, c->stream),
+                    "rebuild positions"))
+            return MLIP_ERR_DEVICE;
+    const NbOut out{&p->apoff, &p->apnbr, &p->apshift, &p->agoff, &p->agent};
+    int64_t total = 0;
+    std::vector<int64_t> spairs;
+    if (!device_lists_into(c, p->nbw, c->stream, nr, natoms.data(), cells.data(), rcs.data(), p->ax.p, out, total,
+                           &spairs))
+        return MLIP_ERR_DEVICE;
+    std::vector<AdmDesc> dd;
+    for (int k = 0; k < nr; ++k) {
+        Request& r = *rq[k];
+        const int64_t np = spairs[k + 1] - spairs[k];
         p->pairs.give(r.p0, std::max<int64_t>(r.np, 1));
-        const int64_t p0 = j.ok ? p->pairs.take(std::max<int64_t>(np, 1)) : -1;
+        const int64_t p0 = p->pairs.take(std::max<int64_t>(np, 1));
         if (p0 < 0) {
             // no list for the request: it ends here with an error
-            r.error = j.ok ? "request pool: no pair capacity for an NVE neighbor-list rebuild"
-                           : "NVE neighbor-list rebuild: " + j.err;
+            r.error = "request pool: no pair capacity for an NVE neighbor-list rebuild";
             r.np = 0;
             r.p0 = p->pairs.take(1);
             r.done = true;
@@ -693,32 +730,31 @@ int rebuild_lists(mlip_ctx* c, Pool* p, const std::vector<int>& ids, int* ndone)
         r.np = np;
         ++r.rebuilds;
         ++p->rebuilds;
-        const int na = r.natoms;
-        std::vector<int32_t> pb(na), pe(na), gb(na), ge(na), nb(static_cast<size_t>(np)), gt(j.gent.size());
-        const int64_t g0 = 2 * p0;
-        for (int i = 0; i < na; ++i) {
-            pb[i] = static_cast<int32_t>(p0 + j.loff[i]);
-            pe[i] = static_cast<int32_t>(p0 + j.loff[i + 1]);
-            gb[i] = static_cast<int32_t>(g0 + j.goff[i]);
-            ge[i] = static_cast<int32_t>(g0 + j.goff[i + 1]);
-        }
-        for (int64_t e = 0; e < np; ++e) nb[e] = static_cast<int32_t>(r.a0 + j.nl.idx[e]);
-        for (size_t e = 0; e < j.gent.size(); ++e) {
-            const int32_t en = j.gent[e];
-            gt[e] = en > 0 ? static_cast<int32_t>(en + p0) : static_cast<int32_t>(en - p0);
-        }
-        const size_t a3 = 3 * static_cast<size_t>(r.a0);
-        if (!stage_upload(c, p, p->pbeg.p + r.a0, pb.data(), pb.size(), "rebuild") ||
-            !stage_upload(c, p, p->pend.p + r.a0, pe.data(), pe.size(), "rebuild") ||
-            !stage_upload(c, p, p->gbeg.p + r.a0, gb.data(), gb.size(), "rebuild") ||
-            !stage_upload(c, p, p->gend.p + r.a0, ge.data(), ge.size(), "rebuild") ||
-            !stage_upload(c, p, p->pnbr.p + p0, nb.data(), nb.size(), "rebuild") ||
-            !stage_upload(c, p, p->pshift.p + 3 * p0, j.nl.shift.data(), j.nl.shift.size(), "rebuild") ||
-            !stage_upload(c, p, p->gent.p + 2 * p0, gt.data(), gt.size(), "rebuild") ||
-            !stage_upload(c, p, p->xref.p + a3, j.x.data(), j.x.size(), "rebuild"))
+        dd.push_back({at[k], spairs[k], np, r.a0, p0, r.natoms, 0});
+        if (!hip_ok(c, hipMemcpyAsync(p->xref.p + 3 * r.a0, p->x.p + 3 * r.a0, 24 * static_cast<size_t>(r.natoms),
+                                      hipMemcpyDeviceToDevice, c->stream),
+                    "reference positions"))
             return MLIP_ERR_DEVICE;
     }
-    return stage_flush(c, p) ? MLIP_OK : MLIP_ERR_DEVICE;
+    if (dd.empty()) return MLIP_OK;
+    if (!p->adesc.ensure(dd.size())) {
+        c->err = "device allocation failed for list rebuilds";
+        return MLIP_ERR_DEVICE;
+    }
+    if (!stage_begin(c, p) || !stage_upload(c, p, p->adesc.p, dd.data(), dd.size(), "rebuild descriptors") ||
+        !stage_flush(c, p))
+        return MLIP_ERR_DEVICE;
+    int64_t most = 1;
+    for (const AdmDesc& q : dd) most = std::max<int64_t>(most, std::max<int64_t>(q.n, 2 * q.np));
+    const unsigned gx = static_cast<unsigned>(std::min<int64_t>((most + 255) / 256, 64));
+    hipLaunchKernelGGL(k_adm_place, dim3(gx, static_cast<unsigned>(dd.size())), dim3(256), 0, c->stream, p->adesc.p,
+                       p->apoff.p, p->agoff.p, p->apnbr.p, p->apshift.p, p->agent.p, p->pbeg.p, p->pend.p,
+                       p->gbeg.p, p->gend.p, p->pnbr.p, p->pshift.p, p->gent.p);
+    if (!hip_ok(c, hipGetLastError(), "rebuild placement") ||
+        !hip_ok(c, hipEventRecord(p->placed, c->stream), "rebuild placement"))
+        return MLIP_ERR_DEVICE;
+    p->placed_pending = true;
+    return MLIP_OK;
 }
 
 }  // namespace
@@ -735,10 +771,8 @@ int mlip_pool_admit(mlip_ctx* c, int natoms, const double* cell, const double* x
         c->err = "NVE requests need a neighbor skin > 0 (the list is rebuilt when an atom moves skin / 2)";
         return MLIP_ERR_ARG;
     }
-    std::vector<Prepared> pr(1);
-    prepare(c, natoms, cell, x, type, skin, pr[0], default_threads());
     const int64_t at[2] = {0, natoms};
-    const int k = commit_many(c, c->pool, pr, 1, &natoms, at, cell, x, v, &nsteps, dt, &skin, 1, handle);
+    const int k = admit_device(c, c->pool, 1, &natoms, at, cell, x, type, v, &nsteps, dt, &skin, handle);
     if (k < 0) return -k;
     return k == 1 ? MLIP_OK : MLIP_ERR_FULL;
 }
@@ -759,14 +793,8 @@ int mlip_pool_admit_many(mlip_ctx* c, int count, const int* natoms, const double
         }
         at[k + 1] = at[k] + natoms[k];
     }
-    const int nt = nthreads > 0 ? nthreads : default_threads();
-    std::vector<Prepared> pr(count);
-    // few requests: threads inside each neighbor build; many: one per request
-    const int inner = std::max(1, nt / count);
-    parallel_for(count, std::min(nt, count), [&](int k) {
-        prepare(c, natoms[k], cells + 9 * k, x + 3 * at[k], type + at[k], skin[k], pr[k], inner);
-    });
-    return commit_many(c, c->pool, pr, count, natoms, at.data(), cells, x, v, nsteps, dt, skin, nt, handles);
+    (void)nthreads;  // the lists are built on the device
+    return admit_device(c, c->pool, count, natoms, at.data(), cells, x, type, v, nsteps, dt, skin, handles);
 }
 
 int mlip_pool_tick(mlip_ctx* c, int* nbatch) {
