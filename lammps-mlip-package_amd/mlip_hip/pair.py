@@ -384,9 +384,10 @@ class RequestPool:
         return h.value
 
     def admit_many(self, structures, nsteps, dt=0.001, skins=None, velocities=None, nthreads=16):
-        """FIFO admission of several structures (neighbor lists built on
-        `nthreads` host threads); returns the handles of those admitted (a
-        prefix of the list)."""
+        """FIFO admission of several structures (their neighbor lists are
+        built in one batch on the device; `nthreads` is kept for callers of
+        the earlier host builder and not used); returns the handles of those
+        admitted (a prefix of the list)."""
         n = len(structures)
         if n == 0:
             return []
