@@ -62,20 +62,32 @@ bool upload_q34_plan(mlip_ctx* c) {
         append(tt.groups_feat, tt.visits_feat, gfeat, vfeat, dm.gfeat_beg[t], dm.gfeat_end[t]);
     }
     // device form of the order-3/4 visits
-    auto pack = [&](const std::vector<VisitQ34>& vs, std::vector<VisitPack>& out) {
+    // adjoint visits: H sources h = pslot >= 0 ? pslot : -1 - itc, the row
+    // of a paired group in flags bit0 (row 1) and bit3 (both rows)
+    auto hsrc = [](int pslot, int itc) { return pslot >= 0 ? pslot : -1 - itc; };
+    auto pack = [&](const std::vector<VisitQ34>& vs, std::vector<VisitPack>& out, bool adj) {
         for (const VisitQ34& v : vs) {
-            const int f[8] = {v.order, v.la, v.lb, v.lc, v.sa, v.sb, v.sc, v.flags};
+            int flags = v.flags;
+            if (adj) {
+                if (flags & (1 | 8)) return false;
+                flags |= v.row == 1 ? 1 : v.row == 2 ? 8 : 0;
+            }
+            const int f[8] = {v.order, v.la, v.lb, v.lc, v.sa, v.sb, v.sc, flags};
             uint32_t sig = 0;
             for (int k = 0; k < 8; ++k) {
                 if (f[k] < 0 || f[k] > 15) return false;
                 sig |= static_cast<uint32_t>(f[k]) << (4 * k);
             }
-            out.push_back({static_cast<int32_t>(sig), v.coef, v.pslot, v.itc});
+            if (adj)
+                out.push_back({static_cast<int32_t>(sig), v.coef, hsrc(v.pslot, v.itc),
+                               v.row == 2 ? hsrc(v.pslot1, v.itc1) : 0});
+            else
+                out.push_back({static_cast<int32_t>(sig), v.coef, v.pslot, v.itc});
         }
         return true;
     };
     std::vector<VisitPack> vadjp, vfeatp;
-    if (!pack(vadj, vadjp) || !pack(vfeat, vfeatp)) {
+    if (!pack(vadj, vadjp, true) || !pack(vfeat, vfeatp, false)) {
         c->err = "order-3/4 visit outside the packed range";
         return false;
     }
@@ -1543,18 +1555,28 @@ int mlip_flops(const mlip_ctx* c, double* out7) {
                     // per term: complex product 6, then 4 (adjoint: two FMAs
                     // into G) or 5 (features: Re of the product with the
                     // output factor, one FMA); order 4 adds the a x b
-                    // product (6) per (m_a, m_b), the adjoint H x a (2) per m_a
+                    // product (6) per (m_a, m_b), the adjoint H x a (2) per m_a.
+                    // Visits of a species-paired group add their products
+                    // into a visit accumulator instead and fold it into each
+                    // of their rows, 4 (L + 1) per row
                     const double na_ = 2.0 * v.la + 1.0, nb_ = 2.0 * v.lb + 1.0;
-                    f += (adj ? 10.0 : 11.0) * cnt + (v.order == 4 ? 6.0 * na_ * nb_ : 0.0) + (adj ? 2.0 * na_ : 0.0);
+                    const bool paired = adj && g.S == kPairedRows;
+                    f += (adj ? 10.0 : 11.0) * cnt + (v.order == 4 ? 6.0 * na_ * nb_ : 0.0) +
+                         (adj && !paired ? 2.0 * na_ : 0.0) + (paired ? (v.row == 2 ? 8.0 : 4.0) * (g.L + 1) : 0.0);
                 }
             // adjoint: the energy of linear families from the G rows (Euler),
             // one row dot product per linear class present in the group
             if (adj)
                 for (const GroupQ34& g : gs) {
-                    bool seen[4] = {false, false, false, false};
-                    for (int vi = g.vbeg; vi < g.vend; ++vi)
-                        if (vs[vi].pslot < 0 && vs[vi].order >= 1 && vs[vi].order <= 4) seen[vs[vi].order - 1] = true;
-                    for (bool b : seen) f += b ? 5.0 * (g.L + 1) : 0.0;
+                    bool seen[2][4] = {{false, false, false, false}, {false, false, false, false}};
+                    for (int vi = g.vbeg; vi < g.vend; ++vi) {
+                        const VisitQ34& v = vs[vi];
+                        if (v.pslot >= 0 || v.order < 1 || v.order > 4) continue;
+                        if (v.row != 1) seen[0][v.order - 1] = true;
+                        if (g.S == kPairedRows && v.row != 0) seen[1][v.order - 1] = true;
+                    }
+                    for (int r = 0; r < 2; ++r)
+                        for (bool b : seen[r]) f += b ? 5.0 * (g.L + 1) : 0.0;
                 }
             return f;
         };

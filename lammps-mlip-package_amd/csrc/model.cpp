@@ -227,6 +227,85 @@ double lookup(const std::map<std::vector<int>, double>& c, const std::vector<int
     return it == c.end() ? 0.0 : it->second;
 }
 
+// Species pairing of the adjoint rows (two-species models): for L <=
+// kPairMaxL, rows (0, L) and (1, L) whose visits all run unrolled become one
+// row (S = kPairedRows).  A visit of row 0 and one of row 1 with the same
+// order, factors (l and species), energy class and coefficient block (the
+// same irrep seen from two families that differ only in the output species)
+// become one visit of both rows: the kernel forms its products once and
+// adds them with each family's H.  The rest keep their row.  Within the
+// paired row the visits go by energy class, and inside a class row-0 visits,
+// row-1 visits, then shared ones, so each row still sees its classes in
+// order.  `pend` arrives sorted by (S, L, class, ...).
+void pair_rows(const Model& m, std::vector<PlanVisit>& pend) {
+    if (m.n_type != 2) return;
+    auto cls = [](const VisitQ34& v) { return v.pslot >= 0 ? 4 : std::max(v.order - 1, 0); };
+    std::vector<PlanVisit> out;
+    std::vector<bool> taken(pend.size(), false);
+    for (int L = 0; L <= kPairMaxL; ++L) {
+        std::vector<size_t> r0, r1;
+        bool all_fast = true;
+        for (size_t k = 0; k < pend.size(); ++k) {
+            if (pend[k].L != L) continue;
+            if (!(pend[k].v.flags & 4)) all_fast = false;
+            (pend[k].S == 0 ? r0 : r1).push_back(k);
+        }
+        if (!all_fast || r0.empty() || r1.empty()) continue;
+        std::vector<bool> used1(r1.size(), false);
+        std::vector<PlanVisit> only0[5], only1[5], both[5];
+        for (size_t i0 : r0) {
+            const PlanVisit& a = pend[i0];
+            const int c = cls(a.v);
+            int hit = -1;
+            if (a.v.order >= 3)
+                for (size_t j = 0; j < r1.size() && hit < 0; ++j) {
+                    if (used1[j]) continue;
+                    const PlanVisit& b = pend[r1[j]];
+                    const VisitQ34 &x = a.v, &y = b.v;
+                    if (y.order != x.order || y.la != x.la || y.sa != x.sa || y.lb != x.lb || y.sb != x.sb ||
+                        y.lc != x.lc || y.sc != x.sc || cls(y) != c || b.clen != a.clen)
+                        continue;
+                    bool same = true;
+                    for (int q = 0; q < a.clen && same; ++q) same = m.coef[x.coef + q] == m.coef[y.coef + q];
+                    if (same) hit = static_cast<int>(j);
+                }
+            PlanVisit q = a;
+            q.S = kPairedRows;
+            if (hit >= 0) {
+                used1[hit] = true;
+                q.v.row = 2;
+                q.v.pslot1 = pend[r1[hit]].v.pslot;
+                q.v.itc1 = pend[r1[hit]].v.itc;
+                both[c].push_back(q);
+            } else {
+                q.v.row = 0;
+                only0[c].push_back(q);
+            }
+        }
+        for (size_t j = 0; j < r1.size(); ++j) {
+            if (used1[j]) continue;
+            PlanVisit q = pend[r1[j]];
+            q.S = kPairedRows;
+            q.v.row = 1;
+            only1[cls(q.v)].push_back(q);
+        }
+        for (size_t k : r0) taken[k] = true;
+        for (size_t k : r1) taken[k] = true;
+        for (int c = 0; c < 5; ++c) {
+            out.insert(out.end(), only0[c].begin(), only0[c].end());
+            out.insert(out.end(), only1[c].begin(), only1[c].end());
+            out.insert(out.end(), both[c].begin(), both[c].end());
+        }
+    }
+    if (out.empty()) return;
+    // unpaired rows first in their (S, L) order, then the paired rows by L
+    std::vector<PlanVisit> rest;
+    for (size_t k = 0; k < pend.size(); ++k)
+        if (!taken[k]) rest.push_back(pend[k]);
+    rest.insert(rest.end(), out.begin(), out.end());
+    pend.swap(rest);
+}
+
 // Group visits by (S, L) and balance the groups over the waves of a
 // workgroup.  The work model of a visit is its terms plus region_cost per
 // m_a (per (m_a, m_b)) region and visit_cost per visit, in units of terms.
@@ -248,30 +327,37 @@ void make_groups(const Model& m, std::vector<PlanVisit> pend, std::vector<GroupQ
     });
     for (const PlanVisit& q : pend)
         if (!(q.v.flags & 4)) dyn = true;
+    if (adjoint) pair_rows(m, pend);
     // Work model of a visit: its terms plus the per-m_a (per (m_a, m_b))
     // region and per-visit overheads, in units of terms (3 and 24 were the
     // best of a sweep at the 256k-atom benchmark model: 40.3 -> 38.8 ms);
     // of a part: plus loading and storing its G row.
     auto vcost = [&](const PlanVisit& q) {
         const long regions = q.v.order <= 2 ? 0 : (2L * q.v.la + 1) * (q.v.order == 4 ? 2L * q.v.lb + 1 : 1);
-        return q.cost + region_cost * regions + visit_cost;
+        // a paired row's visit adds its products into one or both rows:
+        // 2 (L + 1) FMAs each, about a third of a term's work per entry
+        const long fin = (q.S == kPairedRows && q.v.order >= 3) ? (q.v.row == 2 ? 2L : 1L) * (q.L + 1) / 3 : 0L;
+        return q.cost + region_cost * regions + visit_cost + fin;
     };
     struct Row {
         size_t b, e;
-        int L;
+        int L, S;
     };
     std::vector<Row> rows;
     for (size_t k = 0; k < pend.size();) {
         size_t e = k;
         while (e < pend.size() && pend[e].S == pend[k].S && pend[e].L == pend[k].L) ++e;
-        rows.push_back({k, e, pend[k].L});
+        rows.push_back({k, e, pend[k].L, pend[k].S});
         k = e;
     }
     // LDS available for the partial rows of split rows (adjoint only):
     // what the workgroup's 160 KB leave beside the a_nlm tile
     const long tile = static_cast<long>(m.n_type) * n_lmh_of(m.lmax3) * 2 * kBlockLanes;
     const long budget = adjoint ? std::max(0L, 160L * 1024 / 8 - tile) : 1L << 40;
-    auto row_red = [](int L, int np) { return np > 1 ? static_cast<long>(np) * (L + 1) * 2 * kBlockLanes : 0L; };
+    // a paired row keeps two partial rows per part
+    auto row_red = [](int L, int np, int S) {
+        return np > 1 ? static_cast<long>(np) * (L + 1) * 2 * kBlockLanes * (S == kPairedRows ? 2 : 1) : 0L;
+    };
     struct Part {
         size_t row, b, e;
         long cost;
@@ -375,7 +461,7 @@ void make_groups(const Model& m, std::vector<PlanVisit> pend, std::vector<GroupQ
         for (size_t r = 0; r < rows.size(); ++r) {
             const Row& R = rows[r];
             if (static_cast<size_t>(nparts[r]) >= R.e - R.b || nparts[r] >= kWavesPerWg) continue;
-            const long red_new = red_used - row_red(R.L, nparts[r]) + row_red(R.L, nparts[r] + 1);
+            const long red_new = red_used - row_red(R.L, nparts[r], R.S) + row_red(R.L, nparts[r] + 1, R.S);
             if (red_new > budget) continue;
             ++nparts[r];
             std::vector<Part> p2;
@@ -389,7 +475,7 @@ void make_groups(const Model& m, std::vector<PlanVisit> pend, std::vector<GroupQ
             }
         }
         if (cand == rows.size()) break;
-        red_used += row_red(rows[cand].L, nparts[cand] + 1) - row_red(rows[cand].L, nparts[cand]);
+        red_used += row_red(rows[cand].L, nparts[cand] + 1, rows[cand].S) - row_red(rows[cand].L, nparts[cand], rows[cand].S);
         ++nparts[cand];
         build(parts);
         best = assign(parts, wave);
@@ -406,14 +492,16 @@ void make_groups(const Model& m, std::vector<PlanVisit> pend, std::vector<GroupQ
             g.L = rows[r].L;
             g.part = part;
             g.nparts = np;
-            g.red = (adjoint && np > 1) ? red + part * (rows[r].L + 1) * 2 * kBlockLanes : 0;
+            g.red = (adjoint && np > 1)
+                        ? red + part * (rows[r].L + 1) * 2 * kBlockLanes * (g.S == kPairedRows ? 2 : 1)
+                        : 0;
             g.vbeg = static_cast<int>(visits.size());
             for (size_t j = P.b; j < P.e; ++j) visits.push_back(pend[j].v);
             g.vend = static_cast<int>(visits.size());
             g.wave = wave[pi];
             groups.push_back(g);
         }
-        if (adjoint && np > 1) red += static_cast<int>(row_red(rows[r].L, np));
+        if (adjoint && np > 1) red += static_cast<int>(row_red(rows[r].L, np, rows[r].S));
     }
     red_total = std::max(red_total, red);
 }

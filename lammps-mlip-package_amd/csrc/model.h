@@ -79,11 +79,21 @@ struct VisitQ34 {
     int32_t flags;     // bit0: compute d here; bit1: accumulate G; bit2: unrolled signature
     int32_t coef;      // dense coefficient block offset
     int32_t dcoef;     // dense pair-summed block offset (flags bit0)
+    // species-paired adjoint groups (GroupQ34::S == kPairedRows): the rows
+    // the visit adds into (0: G_{0,L}, 1: G_{1,L}, 2: both, with the
+    // coefficient block both families share); pslot1 / itc1 are the second
+    // family's H source when row == 2
+    int32_t row;
+    int32_t pslot1, itc1;
 };
 
 // Device form of a VisitQ34 (16 bytes, one scalar load): sig packs order,
 // l_a, l_b, l_c, s_a, s_b, s_c and flags in 4 bits each (low to high); the
-// feature-pass visits read their coefficients at coef (dcoef == coef there).
+// feature-pass visits read their coefficients at coef (dcoef == coef there)
+// and keep pslot / itc.  Adjoint visits carry the H source of their family
+// as h = pslot >= 0 ? pslot : -1 - itc in the pslot field (and the second
+// family's in the itc field for a visit of both paired rows); flags bit0
+// marks a visit of row 1 of a paired group, bit3 one of both rows.
 struct VisitPack {
     int32_t sig;
     int32_t coef;
@@ -97,6 +107,15 @@ struct VisitPack {
 // G_{S,L,m}.  A heavy (S, L) row is split into nparts sub-groups on
 // different waves; each writes its partial row to LDS at `red` (doubles
 // from the partial area) and the wave of part 0 sums them in part order.
+// S == kPairedRows: rows (0, L) and (1, L) of a two-species model handled
+// together, so that the visits both rows have in common (same factors, same
+// coefficients, different families) form their products once.
+constexpr int kPairedRows = 2;
+// largest L whose two species rows are paired (the kernel holds both rows and
+// a visit accumulator, 6 (L + 1) doubles): at 256k atoms of the order-4
+// benchmark model the order-3/4 kernel takes 38.1 ms unpaired, 36.8 / 35.1 /
+// 38.6 ms with rows up to L = 4 / 5 / 6 paired (at 6 the registers spill)
+constexpr int kPairMaxL = 5;
 struct GroupQ34 {
     int32_t S, L;
     int32_t vbeg, vend;

@@ -4,7 +4,9 @@
 // Workgroup = kWavesPerWg waves on one (block of 64 same-type centres,
 // radial index n).  The a_nlm tile (l <= lmax3, both species) sits in LDS as
 // [s][lm][lane](re, im); each wave owns (S, L) output groups (or balanced
-// parts of heavy ones) and keeps G_{S,L,m<=0} in registers.  Visits with
+// parts of heavy ones) and keeps G_{S,L,m<=0} in registers.  For L <=
+// kPairMaxL the rows of both species form one group, whose visits common
+// to the two rows form their products once (q34_group_pair).  Visits with
 // fully unrolled signatures run in one kernel, the rest in a second
 // general-loop kernel (only launched for models that have such visits), so
 // the common kernel's register budget allows more waves per SIMD.  Reference: compute_anlm_uniq_products and the
@@ -183,7 +185,9 @@ __device__ __forceinline__ void factor(const double (&r)[LB + 1], const double (
     }
 }
 
-template <int L, int LA, int LB, bool DOG, bool DOD>
+// SC: the a factor carries H (G accumulation); without it the visit adds
+// the unscaled products into a visit accumulator (species-paired groups)
+template <int L, int LA, int LB, bool DOG, bool DOD, bool SC>
 __device__ __forceinline__ void visit3u(const Tile& T, int S, int sa, int sb, cptr<double> cg,
                                         cptr<double> cd, double H, double (&gr)[L + 1],
                                         double (&gi)[L + 1], double& d) {
@@ -198,7 +202,7 @@ __device__ __forceinline__ void visit3u(const Tile& T, int S, int sa, int sb, cp
         // next a factor requested one m_a ahead
         double nxr = 0.0, nxi = 0.0;
         if (ia < 2 * LA) T.get(sa, LA, ma + 1, nxr, nxi);
-        if (DOG) {
+        if (DOG && SC) {
             xr *= H;
             xi *= H;
         }
@@ -234,7 +238,7 @@ __device__ __forceinline__ void visit3u(const Tile& T, int S, int sa, int sb, cp
     }
 }
 
-template <int L, int LA, int LB, int LC, bool DOG, bool DOD>
+template <int L, int LA, int LB, int LC, bool DOG, bool DOD, bool SC>
 __device__ __forceinline__ void visit4u(const Tile& T, int S, int sa, int sb, int sc, cptr<double> cg,
                                         cptr<double> cd, double H, double (&gr)[L + 1],
                                         double (&gi)[L + 1], double& d) {
@@ -247,7 +251,7 @@ __device__ __forceinline__ void visit4u(const Tile& T, int S, int sa, int sb, in
         const int ma = ia - LA;
         double xr, xi;
         T.get(sa, LA, ma, xr, xi);
-        if (DOG) {
+        if (DOG && SC) {
             xr *= H;
             xi *= H;
         }
@@ -287,17 +291,17 @@ __device__ __forceinline__ void visit4u(const Tile& T, int S, int sa, int sb, in
 
 // Dispatch on (l_a, l_b[, l_c]) for a fixed output L; returns false for a
 // signature outside the unrolled set (the caller then runs the general loop).
-template <int L, bool DOG, bool DOD>
+template <int L, bool DOG, bool DOD, bool SC>
 struct Unrolled;
 
 #define MLIP_CASE3(A, B) \
-    case (A) * 16 + (B): visit3u<LV, A, B, DOG, DOD>(T, S, v.sa, v.sb, cg, cd, H, gr, gi, d); return true;
+    case (A) * 16 + (B): visit3u<LV, A, B, DOG, DOD, SC>(T, S, v.sa, v.sb, cg, cd, H, gr, gi, d); return true;
 #define MLIP_CASE4(A, B, C)                                                                       \
     case ((A) * 16 + (B)) * 16 + (C):                                                             \
-        visit4u<LV, A, B, C, DOG, DOD>(T, S, v.sa, v.sb, v.sc, cg, cd, H, gr, gi, d); return true;
+        visit4u<LV, A, B, C, DOG, DOD, SC>(T, S, v.sa, v.sb, v.sc, cg, cd, H, gr, gi, d); return true;
 #define MLIP_UNROLLED(LL, CASES3, CASES4)                                                                     \
-    template <bool DOG, bool DOD>                                                                             \
-    struct Unrolled<LL, DOG, DOD> {                                                                           \
+    template <bool DOG, bool DOD, bool SC>                                                                    \
+    struct Unrolled<LL, DOG, DOD, SC> {                                                                       \
         static constexpr int LV = LL;                                                                         \
         __device__ static bool run(const Tile& T, const VDec& v, int S, int order, int la, int lb, int lc,     \
                                    cptr<double> cg, cptr<double> cd, double H, double (&gr)[LL + 1],          \
@@ -318,7 +322,7 @@ MLIP_UNROLLED(5, MLIP_SIG3_L5(MLIP_CASE3), MLIP_NO_CASES)
 MLIP_UNROLLED(6, MLIP_SIG3_L6(MLIP_CASE3), MLIP_NO_CASES)
 MLIP_UNROLLED(7, MLIP_SIG3_L7(MLIP_CASE3), MLIP_NO_CASES)
 MLIP_UNROLLED(8, MLIP_SIG3_L8(MLIP_CASE3), MLIP_NO_CASES)
-template <int L, bool DOG, bool DOD>
+template <int L, bool DOG, bool DOD, bool SC>
 struct Unrolled {
     __device__ static bool run(const Tile&, const VDec&, int, int, int, int, int, cptr<double>, cptr<double>,
                                double, double (&)[L + 1], double (&)[L + 1], double&) {
@@ -341,6 +345,171 @@ __device__ __forceinline__ double row_dot(const Tile& T, int S, const double (&g
         sum = fma(k == 0 ? 1.0 : 2.0, gr[k] * ar - gi[k] * ai, sum);
     }
     return sum;
+}
+
+// &G[blk, n, s = 0, lm = 0, re, lane]; entry (s, lm, c) lies at
+// ((s n_lmh + lm) 2 + c) kBlock from it.  Formed after a group's visits and
+// made opaque, so that the compiler keeps one pointer instead of hoisting
+// every entry's address out of the visit loop (where they would hold
+// registers across all visits).
+__device__ __forceinline__ double* g_row_base(const DevModel& M, const DevSystem& S, int blk, int n, int lane) {
+    double* p = S.G + a_index(M, blk, n, 0, 0, 0, lane);
+    asm volatile("" : "+v"(p));
+    return p;
+}
+
+// H of an adjoint visit from its packed source h: the polynomial adjoint at
+// rank h of the slot's compact row (h >= 0), else the linear weight of
+// family -1 - h at radial index n
+__device__ __forceinline__ double h_value(const DevModel& M, const DevSystem& S, int64_t prow, int n, int h) {
+    return h >= 0 ? S.H[prow + h] : as_const(M.weight)[n * M.n_gtinv + (-1 - h)];
+}
+
+// energy weight of a linear class (order q = c + 1): 1 / q of the row's dot
+// product gained across it
+__device__ __forceinline__ double class_weight(int c) {
+    return c == 0 ? 1.0 : c == 1 ? 0.5 : c == 2 ? (1.0 / 3.0) : 0.25;
+}
+
+// fused order-1/2 adjoint term into one row: G_{L,-k} += H C_k a_{s_a,L,+k}
+// (order 1: G_{0,0} += H C)
+template <int L>
+__device__ __forceinline__ void fused12(const Tile& T, int order, int sa, double H, cptr<double> c1,
+                                        double (&gr)[L + 1], double (&gi)[L + 1]) {
+    if (order == 1) {
+        gr[0] = fma(H, c1[0], gr[0]);
+        return;
+    }
+#pragma unroll
+    for (int k = 0; k <= L; ++k) {
+        double yr, yi;
+        T.get(sa, L, k, yr, yi);
+        const double hc = H * c1[k];
+        gr[k] = fma(hc, yr, gr[k]);
+        gi[k] = fma(hc, yi, gi[k]);
+    }
+}
+
+// Species-paired adjoint group: rows G_{0,L} and G_{1,L} in registers.  A
+// visit's products (unscaled) go into the visit accumulator t, which is then
+// added into its row with its family's H, or into both rows with the two
+// families' H where the visit is shared; fused order-1/2 visits add into
+// their row directly.  Each row keeps its own energy classes.
+template <int L>
+__device__ __forceinline__ void q34_group_pair(const Tile& T, const DevModel& M, const DevSystem& S,
+                                               const GroupQ34& g, int blk, int n, double* red, double& eacc) {
+    double g0r[L + 1], g0i[L + 1], g1r[L + 1], g1i[L + 1];
+#pragma unroll
+    for (int k = 0; k <= L; ++k) g0r[k] = g0i[k] = g1r[k] = g1i[k] = 0.0;
+    const int slot = blk * kBlock + T.lane;
+    const int tsp = uniform(S.block_type[blk]);
+    const int64_t prow = (int64_t)slot * M.n_pr + pcol(M, tsp, n, 0);
+    cptr<VisitPack> vis = as_const(M.vadj);
+    const int vb = uniform(g.vbeg), ve = uniform(g.vend);
+    int cls0 = -1, cls1 = -1;
+    double dprev0 = 0.0, dprev1 = 0.0;
+    auto close0 = [&](int c) {
+        if (c >= 0 && c <= 3) {
+            const double D = row_dot<L>(T, 0, g0r, g0i);
+            eacc = fma(class_weight(c), D - dprev0, eacc);
+            dprev0 = D;
+        }
+    };
+    auto close1 = [&](int c) {
+        if (c >= 0 && c <= 3) {
+            const double D = row_dot<L>(T, 1, g1r, g1i);
+            eacc = fma(class_weight(c), D - dprev1, eacc);
+            dprev1 = D;
+        }
+    };
+    VisitPack nxt{};
+    if (vb < ve) nxt = vis[vb];
+    for (int vi = vb; vi < ve; ++vi) {
+        const VisitPack cur = nxt;
+        if (vi + 1 < ve) nxt = vis[vi + 1];
+        const VDec v = decode(cur);
+        const int order = v.order;
+        if (order == 0) continue;  // marker of a fused row no family adds to
+        const int row = (v.flags & 1) ? 1 : (v.flags & 8) ? 2 : 0;
+        const int h0 = v.pslot;
+        const int c = h0 >= 0 ? 4 : order - 1;
+        if (row != 1 && c != cls0) {
+            close0(cls0);
+            cls0 = c;
+        }
+        if (row != 0 && c != cls1) {
+            close1(cls1);
+            cls1 = c;
+        }
+        const double H0 = h_value(M, S, prow, n, h0);
+        cptr<double> cg = as_const(M.coef) + uniform(v.coef);
+        if (order <= 2) {
+            if (row == 0)
+                fused12<L>(T, order, uniform(v.sa), H0, cg, g0r, g0i);
+            else
+                fused12<L>(T, order, uniform(v.sa), H0, cg, g1r, g1i);
+            continue;
+        }
+        double tr[L + 1], ti[L + 1];
+#pragma unroll
+        for (int k = 0; k <= L; ++k) tr[k] = ti[k] = 0.0;
+        double dd = 0.0;
+        Unrolled<L, true, false, false>::run(T, v, 0, order, uniform(v.la), uniform(v.lb), uniform(v.lc), cg, cg,
+                                             1.0, tr, ti, dd);
+        if (row != 1) {
+#pragma unroll
+            for (int k = 0; k <= L; ++k) {
+                g0r[k] = fma(H0, tr[k], g0r[k]);
+                g0i[k] = fma(H0, ti[k], g0i[k]);
+            }
+        }
+        if (row != 0) {
+            const double H1 = row == 2 ? h_value(M, S, prow, n, uniform(v.itc)) : H0;
+#pragma unroll
+            for (int k = 0; k <= L; ++k) {
+                g1r[k] = fma(H1, tr[k], g1r[k]);
+                g1i[k] = fma(H1, ti[k], g1i[k]);
+            }
+        }
+    }
+    close0(cls0);
+    close1(cls1);
+    if (uniform(g.nparts) == 1) {
+        // a fused row (L <= q12_fuse) is written, any other added to
+        const bool first = L <= M.q12_fuse;
+        double* G0 = g_row_base(M, S, blk, n, T.lane);
+        double* G1 = G0 + static_cast<size_t>(M.n_lmh) * 2 * kBlock;
+#pragma unroll
+        for (int k = 0; k <= L; ++k) {
+            const int lm = L * (L + 1) / 2 + L - k;  // lmh(L, -k)
+            double* a0 = G0 + (2 * lm) * kBlock;
+            double* b0 = G0 + (2 * lm + 1) * kBlock;
+            double* a1 = G1 + (2 * lm) * kBlock;
+            double* b1 = G1 + (2 * lm + 1) * kBlock;
+            if (first) {
+                *a0 = g0r[k];
+                *b0 = g0i[k];
+                *a1 = g1r[k];
+                *b1 = g1i[k];
+            } else {
+                *a0 += g0r[k];
+                *b0 += g0i[k];
+                *a1 += g1r[k];
+                *b1 += g1i[k];
+            }
+        }
+    } else {
+        // partial rows: row 0, then row 1
+        double* pr = red + uniform(g.red);
+        double* pr1 = pr + (L + 1) * 2 * kBlock;
+#pragma unroll
+        for (int k = 0; k <= L; ++k) {
+            pr[(2 * k) * kBlock + T.lane] = g0r[k];
+            pr[(2 * k + 1) * kBlock + T.lane] = g0i[k];
+            pr1[(2 * k) * kBlock + T.lane] = g1r[k];
+            pr1[(2 * k + 1) * kBlock + T.lane] = g1i[k];
+        }
+    }
 }
 
 template <int L, bool FEAT, bool DYN>
@@ -380,7 +549,6 @@ __device__ __forceinline__ void q34_group(const Tile& T, const DevModel& M, cons
         const bool fast = (flags & 4) != 0;
         if (fast == DYN) continue;  // the other kernel's visit
         if (order == 0) continue;   // marker of a fused row no family adds to
-        const int itc = uniform(v.itc);
         double d = 0.0;
         if (!FEAT) {
             const int c = pslot >= 0 ? 4 : order - 1;
@@ -392,8 +560,7 @@ __device__ __forceinline__ void q34_group(const Tile& T, const DevModel& M, cons
         if (!FEAT && order <= 2) {
             // fused order-1/2 adjoint: G_{S,L,-k} += H C_k a_{s_a,L,+k}
             // (order 1: G_{S,0,0} += H C)
-            const double H =
-                pslot >= 0 ? S.H[prow + pslot] : as_const(M.weight)[n * M.n_gtinv + itc];
+            const double H = h_value(M, S, prow, n, pslot);
             cptr<double> c1 = as_const(M.coef) + uniform(v.coef);
             if (order == 1) {
                 gr[0] = fma(H, c1[0], gr[0]);
@@ -420,20 +587,19 @@ __device__ __forceinline__ void q34_group(const Tile& T, const DevModel& M, cons
                 else
                     visit4<L, false, true>(T, M, v, Sx, 0.0, gr, gi, d);
             } else {
-                Unrolled<L, false, true>::run(T, v, Sx, order, la, lb, lc, cg, cd, 0.0, gr, gi, d);
+                Unrolled<L, false, true, true>::run(T, v, Sx, order, la, lb, lc, cg, cd, 0.0, gr, gi, d);
             }
             S.D[prow + pslot] = d;
             continue;
         }
-        const double H =
-            pslot >= 0 ? S.H[prow + pslot] : as_const(M.weight)[n * M.n_gtinv + itc];
+        const double H = h_value(M, S, prow, n, pslot);
         if constexpr (DYN) {
             if (order == 3)
                 visit3<L, true, false>(T, M, v, Sx, H, gr, gi, d);
             else
                 visit4<L, true, false>(T, M, v, Sx, H, gr, gi, d);
         } else {
-            Unrolled<L, true, false>::run(T, v, Sx, order, la, lb, lc, cg, cd, H, gr, gi, d);
+            Unrolled<L, true, false, true>::run(T, v, Sx, order, la, lb, lc, cg, cd, H, gr, gi, d);
         }
     }
     if (!FEAT) {
@@ -469,22 +635,28 @@ __device__ __forceinline__ void q34_combine(const DevModel& M, const DevSystem& 
                                             int lane, const double* red, bool dyn) {
     const int L = uniform(g.L), Sx = uniform(g.S), np = uniform(g.nparts);
     const bool first = !dyn && L <= M.q12_fuse;  // a fused row: written, not added to
-    const double* base = red + uniform(g.red);
-    const int stride = (L + 1) * 2 * kBlock;
-    for (int k = 0; k <= L; ++k) {
-        double sr = 0.0, si = 0.0;
-        for (int j = 0; j < np; ++j) {
-            sr += base[j * stride + (2 * k) * kBlock + lane];
-            si += base[j * stride + (2 * k + 1) * kBlock + lane];
-        }
-        double& gre = S.G[a_index(M, blk, n, Sx, lmh(L, -k), 0, lane)];
-        double& gim = S.G[a_index(M, blk, n, Sx, lmh(L, -k), 1, lane)];
-        if (first) {
-            gre = sr;
-            gim = si;
-        } else {
-            gre += sr;
-            gim += si;
+    // a species-paired group keeps rows 0 and 1 one after the other per part
+    const int nrows = Sx == kPairedRows ? 2 : 1;
+    const int rowlen = (L + 1) * 2 * kBlock;
+    const int stride = rowlen * nrows;
+    for (int r = 0; r < nrows; ++r) {
+        const double* base = red + uniform(g.red) + r * rowlen;
+        const int So = nrows == 2 ? r : Sx;
+        for (int k = 0; k <= L; ++k) {
+            double sr = 0.0, si = 0.0;
+            for (int j = 0; j < np; ++j) {
+                sr += base[j * stride + (2 * k) * kBlock + lane];
+                si += base[j * stride + (2 * k + 1) * kBlock + lane];
+            }
+            double& gre = S.G[a_index(M, blk, n, So, lmh(L, -k), 0, lane)];
+            double& gim = S.G[a_index(M, blk, n, So, lmh(L, -k), 1, lane)];
+            if (first) {
+                gre = sr;
+                gim = si;
+            } else {
+                gre += sr;
+                gim += si;
+            }
         }
     }
 }
@@ -521,6 +693,24 @@ __global__ __launch_bounds__(kQ34Threads) void k_q34(DevModel M, DevSystem S) {
         if (uniform(g.wave) != wave) continue;
         const int L = uniform(g.L);
         if (!DYN && L > kMaxLUnrolled) continue;
+        if (!FEAT && uniform(g.S) == kPairedRows) {
+            // species-paired rows hold unrolled visits only: this kernel's
+            if constexpr (!FEAT && !DYN) {
+                switch (L) {
+                    case 0: q34_group_pair<0>(T, M, S, g, blk, n, red, eacc); break;
+                    case 1: if constexpr (kPairMaxL >= 1) q34_group_pair<1>(T, M, S, g, blk, n, red, eacc); break;
+                    case 2: if constexpr (kPairMaxL >= 2) q34_group_pair<2>(T, M, S, g, blk, n, red, eacc); break;
+                    case 3: if constexpr (kPairMaxL >= 3) q34_group_pair<3>(T, M, S, g, blk, n, red, eacc); break;
+                    case 4: if constexpr (kPairMaxL >= 4) q34_group_pair<4>(T, M, S, g, blk, n, red, eacc); break;
+                    case 5: if constexpr (kPairMaxL >= 5) q34_group_pair<5>(T, M, S, g, blk, n, red, eacc); break;
+                    case 6: if constexpr (kPairMaxL >= 6) q34_group_pair<6>(T, M, S, g, blk, n, red, eacc); break;
+                    case 7: if constexpr (kPairMaxL >= 7) q34_group_pair<7>(T, M, S, g, blk, n, red, eacc); break;
+                    case 8: if constexpr (kPairMaxL >= 8) q34_group_pair<8>(T, M, S, g, blk, n, red, eacc); break;
+                    default: break;
+                }
+            }
+            continue;
+        }
         switch (L) {
             case 0: q34_group<0, FEAT, DYN>(T, M, S, g, blk, n, red, eacc); break;
             case 1: q34_group<1, FEAT, DYN>(T, M, S, g, blk, n, red, eacc); break;
@@ -547,6 +737,7 @@ __global__ __launch_bounds__(kQ34Threads) void k_q34(DevModel M, DevSystem S) {
                 const GroupQ34 g = groups[gi];
                 if (uniform(g.wave) != wave || uniform(g.nparts) == 1 || uniform(g.part) != 0) continue;
                 if (!DYN && uniform(g.L) > kMaxLUnrolled) continue;
+                if (DYN && uniform(g.S) == kPairedRows) continue;  // the unrolled kernel's
                 q34_combine(M, S, g, blk, n, T.lane, red, DYN);
             }
         }
