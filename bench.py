@@ -204,7 +204,7 @@ def timed_steps(rs, warmup, steps, tag):
     plan = rs.autotune_q34(reps=2)
     rs.compute()
     rs.sync()
-    rs.tuning = {"force_threads": nt, "anlm_kernel": rs.pair.anlm_kernel(), "q34_plan_work_model": list(plan)}
+    rs.tuning = {"force_threads": nt, "anlm_kernel": rs.pair.anlm_kernel(), "q34_plan_work_model": list(plan[:2]), "q34_paired_rows_lmax": plan[2]}
     log("%s: warmup done, force kernel %d threads per workgroup, a_nlm kernel %s, order-3/4 plan %s"
         % (tag, nt, rs.pair.anlm_kernel(), plan))
     rs.profile(True)

@@ -194,11 +194,17 @@ int mlip_get_anlm_kernel(const mlip_ctx* ctx);
 
 /* Order-3/4 workgroup plan: regroups the visits over the 12 waves of the
  * order-3/4 kernel with each of a few work models (terms per scheduling
- * region and per visit), times the adjoint kernel `reps` times on the current
- * resident structure or batch with each, and keeps the fastest; plan2
- * receives its two constants.  Results do not depend on the plan beyond the
- * summation order of split rows. */
-int mlip_autotune_q34(mlip_ctx* ctx, int reps, int* plan2);
+ * region and per visit) and each of three limits on the angular momentum
+ * up to which the two species' rows are paired, times the adjoint kernel
+ * `reps` times on the current resident structure or batch with each, and
+ * keeps the fastest; plan3 receives its two constants and the pairing limit
+ * (-1: no paired rows).  Results do not depend on the plan beyond the
+ * summation order of split rows and of the paired rows' products. */
+int mlip_autotune_q34(mlip_ctx* ctx, int reps, int* plan3);
+
+/* Sets the order-3/4 workgroup plan directly (the three values
+ * mlip_autotune_q34 returns), e.g. to reuse a plan timed before. */
+int mlip_set_q34_plan(mlip_ctx* ctx, int region_cost, int visit_cost, int pair_lmax);
 
 /* Sets the force kernel's launch shape directly (64, 128, 192 or 256
  * threads per workgroup; 192 by default).  Results do not depend on it

@@ -1418,21 +1418,34 @@ int mlip_autotune_q34(mlip_ctx* c, int reps, int* plan2) {
     }
     Model& m = c->model;
     if (!m.has_order34 || m.pair_model) {
-        if (plan2) plan2[0] = m.plan_region, plan2[1] = m.plan_visit;
+        if (plan2) plan2[0] = m.plan_region, plan2[1] = m.plan_visit, plan2[2] = m.plan_pair;
         return MLIP_OK;
     }
     // the work models tried (terms per scheduling region, per visit); the
     // structure's buffers hold the last evaluation's a_nlm and H, and the
     // timed launches only rewrite G and the order-3/4 energies, which the
     // next evaluation writes again
-    const int cand[][2] = {{3, 24}, {0, 0}, {2, 8}, {3, 12}, {3, 48}, {6, 24}, {1, 24}, {4, 32}};
+    // work models (terms per scheduling region, per visit) times the largest
+    // L of species-paired rows (pairing forms fewer products but leaves
+    // fewer, larger parts to balance over the waves)
+    const int work[][2] = {{3, 24}, {0, 0}, {2, 8}, {3, 12}, {3, 48}, {6, 24}, {1, 24}, {4, 32}};
+    const int pairs[] = {kPairMaxL, kPairMaxL - 1, kPairMaxL - 2};
+    int cand[24][3];
+    int ncand = 0;
+    for (int pl : pairs)
+        for (const auto& w : work) {
+            cand[ncand][0] = w[0];
+            cand[ncand][1] = w[1];
+            cand[ncand][2] = std::max(pl, -1);
+            ++ncand;
+        }
     double best = 1e300;
     int bi = 0;
     hipEvent_t a, b;
     if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return MLIP_ERR_DEVICE;
     int rc = MLIP_OK;
-    for (int k = 0; k < static_cast<int>(sizeof cand / sizeof cand[0]) && rc == MLIP_OK; ++k) {
-        plan_q34(m, cand[k][0], cand[k][1]);
+    for (int k = 0; k < ncand && rc == MLIP_OK; ++k) {
+        plan_q34(m, cand[k][0], cand[k][1], cand[k][2]);
         if (!upload_q34_plan(c) || !hip_ok(c, hipStreamSynchronize(c->stream), "plan upload")) {
             rc = MLIP_ERR_DEVICE;
             break;
@@ -1454,11 +1467,24 @@ int mlip_autotune_q34(mlip_ctx* c, int reps, int* plan2) {
     }
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
-    plan_q34(m, cand[bi][0], cand[bi][1]);
+    plan_q34(m, cand[bi][0], cand[bi][1], cand[bi][2]);
     if (!upload_q34_plan(c) || !hip_ok(c, hipStreamSynchronize(c->stream), "plan upload")) return MLIP_ERR_DEVICE;
     drop_graph(c);
-    if (plan2) plan2[0] = cand[bi][0], plan2[1] = cand[bi][1];
+    if (plan2) plan2[0] = cand[bi][0], plan2[1] = cand[bi][1], plan2[2] = cand[bi][2];
     return rc;
+}
+
+int mlip_set_q34_plan(mlip_ctx* c, int region_cost, int visit_cost, int pair_lmax) {
+    if (!c || region_cost < 0 || visit_cost < 0 || pair_lmax < -1 || region_cost > 1000 || visit_cost > 10000) {
+        if (c) c->err = "q34 plan: work-model constants must be >= 0, the pairing limit >= -1";
+        return MLIP_ERR_ARG;
+    }
+    Model& m = c->model;
+    if (!m.has_order34 || m.pair_model) return MLIP_OK;
+    plan_q34(m, region_cost, visit_cost, pair_lmax);
+    if (!upload_q34_plan(c) || !hip_ok(c, hipStreamSynchronize(c->stream), "plan upload")) return MLIP_ERR_DEVICE;
+    drop_graph(c);
+    return MLIP_OK;
 }
 
 int mlip_set_force_threads(mlip_ctx* c, int threads) {

@@ -228,7 +228,7 @@ double lookup(const std::map<std::vector<int>, double>& c, const std::vector<int
 }
 
 // Species pairing of the adjoint rows (two-species models): for L <=
-// kPairMaxL, rows (0, L) and (1, L) whose visits all run unrolled become one
+// min(kPairMaxL, plan_pair), rows (0, L) and (1, L) whose visits all run unrolled become one
 // row (S = kPairedRows).  A visit of row 0 and one of row 1 with the same
 // order, factors (l and species), energy class and coefficient block (the
 // same irrep seen from two families that differ only in the output species)
@@ -242,7 +242,7 @@ void pair_rows(const Model& m, std::vector<PlanVisit>& pend) {
     auto cls = [](const VisitQ34& v) { return v.pslot >= 0 ? 4 : std::max(v.order - 1, 0); };
     std::vector<PlanVisit> out;
     std::vector<bool> taken(pend.size(), false);
-    for (int L = 0; L <= kPairMaxL; ++L) {
+    for (int L = 0; L <= std::min(kPairMaxL, m.plan_pair); ++L) {
         std::vector<size_t> r0, r1;
         bool all_fast = true;
         for (size_t k = 0; k < pend.size(); ++k) {
@@ -1102,16 +1102,17 @@ bool load_model(const std::string& path, const GtinvTable& table, Model& m, std:
     m.coef.clear();
     m.types.assign(m.n_type, TypeTables());
     for (int t = 0; t < m.n_type; ++t) build_type_tables(m, t, cmap);
-    plan_q34(m, m.plan_region, m.plan_visit);
+    plan_q34(m, m.plan_region, m.plan_visit, m.plan_pair);
     m.has_feat34 = false;
     for (const auto& tt : m.types)
         if (!tt.visits_feat.empty()) m.has_feat34 = true;
     return true;
 }
 
-void plan_q34(Model& m, int region_cost, int visit_cost) {
+void plan_q34(Model& m, int region_cost, int visit_cost, int pair_lmax) {
     m.plan_region = region_cost;
     m.plan_visit = visit_cost;
+    m.plan_pair = pair_lmax;
     m.q34_red = 0;
     m.q34_dyn_adj = m.q34_dyn_feat = false;
     for (TypeTables& tt : m.types) {

@@ -226,11 +226,12 @@ struct Model {
     // region and per visit); 3 and 24 were the best of a sweep at the 256k
     // benchmark model, mlip_autotune_q34 times alternatives on the device
     int plan_region = 3, plan_visit = 24;
+    int plan_pair = kPairMaxL;  // largest L of species-paired rows (-1: none)
 };
 
 // (Re)group the order-3/4 visits of every centre type over the waves of the
 // order-3/4 workgroup with the given work model.
-void plan_q34(Model& m, int region_cost, int visit_cost);
+void plan_q34(Model& m, int region_cost, int visit_cost, int pair_lmax);
 
 // Parse a potential file (format of mlip_model.h:45-101) and build every
 // table.  Returns false with a message instead of exiting.

@@ -70,6 +70,7 @@ def library():
     L.mlip_flops.argtypes = [vp, dp]
     L.mlip_random_structures.argtypes = [c_int, c_int, c_int, c_double, c_double, ctypes.c_uint64, ip, dp, dp, ip,
                                          ctypes.c_int64]
+    L.mlip_set_q34_plan.argtypes = [vp, c_int, c_int, c_int]
     L.mlip_set_force_threads.argtypes = [vp, c_int]
     L.mlip_flops_reference.argtypes = [vp, dp]
     i64 = ctypes.c_int64

@@ -187,10 +187,15 @@ class _ResidentBase:
 
     def autotune_q34(self, reps=2):
         """Order-3/4 workgroup plan timed on the device; returns its two
-        work-model constants."""
-        v = np.zeros(2, dtype=np.int32)
+        work-model constants and the largest L of species-paired rows."""
+        v = np.zeros(3, dtype=np.int32)
         self.pair._check(self.lib.mlip_autotune_q34(self.pair.ctx, int(reps), _p(v, ctypes.c_int)))
         return tuple(int(x) for x in v)
+
+    def set_q34_plan(self, region_cost, visit_cost, pair_lmax):
+        """Order-3/4 workgroup plan by hand (what autotune_q34 returns)."""
+        self.pair._check(self.lib.mlip_set_q34_plan(self.pair.ctx, int(region_cost), int(visit_cost),
+                                                    int(pair_lmax)))
 
     def set_force_threads(self, threads):
         self.pair._check(self.lib.mlip_set_force_threads(self.pair.ctx, int(threads)))

@@ -191,7 +191,8 @@ def test_autotune_q34_plan():
     base = _eval(rs)
     plan = rs.autotune_q34(reps=2)
     print("q34 plan work model", plan, flush=True)
-    assert plan in [(3, 24), (0, 0), (2, 8), (3, 12), (3, 48), (6, 24), (1, 24), (4, 32)]
+    work = [(3, 24), (0, 0), (2, 8), (3, 12), (3, 48), (6, 24), (1, 24), (4, 32)]
+    assert plan[:2] in work and plan[2] in (5, 4, 3)
     r = _eval(rs)
     assert abs(r["energy"] - base["energy"]) <= 1e-13 * abs(base["energy"])
     assert np.abs(r["forces"] - base["forces"]).max() <= 1e-12 * np.abs(base["forces"]).max()
