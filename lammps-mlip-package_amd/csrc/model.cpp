@@ -387,7 +387,7 @@ void make_groups(const Model& m, std::vector<PlanVisit> pend, std::vector<GroupQ
         }
     };
     // longest-processing-time assignment, then moves and swaps that lower
-    // the most loaded wave
+    // the most loaded wave; returns the wave loads, heaviest first
     auto assign = [&](const std::vector<Part>& parts, std::vector<int>& wave) {
         std::vector<size_t> order(parts.size());
         for (size_t k = 0; k < order.size(); ++k) order[k] = k;
@@ -441,7 +441,8 @@ void make_groups(const Model& m, std::vector<PlanVisit> pend, std::vector<GroupQ
                 wave[bk2] = hi;
             }
         }
-        return *std::max_element(load.begin(), load.end());
+        std::sort(load.begin(), load.end(), std::greater<long>());
+        return load;
     };
     std::vector<int> nparts(rows.size(), 1);
     auto build = [&](std::vector<Part>& parts) {
@@ -451,12 +452,14 @@ void make_groups(const Model& m, std::vector<PlanVisit> pend, std::vector<GroupQ
     std::vector<Part> parts;
     std::vector<int> wave;
     build(parts);
-    long best = assign(parts, wave);
+    std::vector<long> best = assign(parts, wave);
     long red_used = 0;
-    // split one row one part further while that lowers the most loaded
-    // wave and the partial rows fit in LDS
+    // split one row one part further while that lowers the wave loads
+    // (heaviest first, compared in order: with two rows at the maximum, the
+    // first split leaves the maximum and lowers the count of waves at it)
+    // and the partial rows fit in LDS
     for (;;) {
-        long cand_best = best;
+        std::vector<long> cand_best = best;
         size_t cand = rows.size();
         for (size_t r = 0; r < rows.size(); ++r) {
             const Row& R = rows[r];
@@ -467,7 +470,7 @@ void make_groups(const Model& m, std::vector<PlanVisit> pend, std::vector<GroupQ
             std::vector<Part> p2;
             std::vector<int> w2;
             build(p2);
-            const long mx = assign(p2, w2);
+            const std::vector<long> mx = assign(p2, w2);
             --nparts[r];
             if (mx < cand_best) {
                 cand_best = mx;
