@@ -29,6 +29,7 @@ struct DevModel {
     int n_type, n_fn, n_gtinv, n_lmh, n_lmh3, maxl, lmax3, n_pitc, n_p;
     int has_q34;            // order-3/4 invariants present (k_q34 runs)
     int q12_fuse;           // order-1/2 adjoint rows L <= q12_fuse are the order-3/4 kernel's (-1: none)
+    int q12_fuse_hi;        // ... and the rows lmax3 < L <= q12_fuse_hi (a_nlm from HBM)
     double cutoff;
     double beta[kMaxFnPair], mu[kMaxFnPair];
     const double* coef;     // dense coefficient blocks
@@ -86,6 +87,11 @@ struct DevModel {
     int32_t hadj_beg[2], hadj_end[2], hfeat_beg[2], hfeat_end[2];
     int lmaxh, n_lmhh;
 };
+
+// order-1/2 adjoint row L runs inside the order-3/4 kernel
+__host__ __device__ inline bool q12_fused(const DevModel& m, int L) {
+    return L <= m.q12_fuse || (L > m.lmax3 && L <= m.q12_fuse_hi);
+}
 
 struct DevSystem {
     int nall;      // positions (owned atoms first; ghosts / images after)

@@ -123,6 +123,7 @@ bool upload_model(mlip_ctx* c) {
         for (int k = 0; k < 2; ++k) dm.prange[t][k] = m.prange[t][k];
     dm.has_q34 = m.has_order34 ? 1 : 0;
     dm.q12_fuse = m.q12_fuse;
+    dm.q12_fuse_hi = m.q12_fuse_hi;
     dm.q34_red = m.q34_red;
     dm.pair_model = m.pair_model ? 1 : 0;
     for (int a = 0; a < 2; ++a)
@@ -1553,7 +1554,7 @@ int mlip_flops(const mlip_ctx* c, double* out7) {
             for (int l = 0; l <= m.maxl; ++l)
                 if (&v - tt.visits_q12.data() >= tt.q12_off[l]) L = l;
             if (v.pslot >= 0) q12 += v.order == 1 ? 2.0 : 6.0 * (L + 1);
-            if (L > m.q12_fuse) q12 += v.order == 1 ? 2.0 : 14.0 * (L + 1);
+            if (!q12_fused(c->dm, L)) q12 += v.order == 1 ? 2.0 : 14.0 * (L + 1);
         }
         out7[1] += na * q12;
         auto terms = [&](const std::vector<GroupQ34>& gs, const std::vector<VisitQ34>& vs, bool adj) {

@@ -125,8 +125,8 @@ template <int L, int LMAX, bool FEAT>
 struct Q12Levels {
     __device__ static void run(const DevModel& M, const DevSystem& S, int blk, int n, int lane, int t,
                                double& e, double* tile) {
-        // adjoint levels at or below q12_fuse run in the order-3/4 kernel
-        if (L <= M.maxl && (FEAT || L > M.q12_fuse)) q12_level<L, FEAT>(M, S, blk, n, lane, t, e, tile);
+        // fused adjoint levels run in the order-3/4 kernel
+        if (L <= M.maxl && (FEAT || !q12_fused(M, L))) q12_level<L, FEAT>(M, S, blk, n, lane, t, e, tile);
         Q12Levels<L + 1, LMAX, FEAT>::run(M, S, blk, n, lane, t, e, tile);
     }
 };

@@ -540,7 +540,7 @@ void build_type_tables(Model& m, int t, const std::vector<std::map<std::vector<i
                 for (int m0 = -L; m0 <= L; ++m0) push_coef(lookup(cm, {m0, -m0}));
             }
             by_l[L].push_back(v);
-            if (L <= m.q12_fuse) {
+            if (L <= m.q12_fuse || (L > m.lmax3 && L <= m.q12_fuse_hi)) {
                 // the same family's adjoint in the order-3/4 kernel, one visit
                 // per factor position p: dd/da_{sp[p],L,-k} = C a_{other,L,+k}
                 // with C = C(-k, k) (p = 0) or C(k, -k) (p = 1)
@@ -723,7 +723,8 @@ void build_type_tables(Model& m, int t, const std::vector<std::map<std::vector<i
     // kernel skips it): rows no family adds to get an empty marker visit,
     // so their G is written as zero
     for (int S = 0; S < m.n_type; ++S)
-        for (int L = 0; L <= m.q12_fuse; ++L) {
+        for (int L = 0; L <= std::max(m.q12_fuse, m.q12_fuse_hi); ++L) {
+            if (!(L <= m.q12_fuse || (L > m.lmax3 && L <= m.q12_fuse_hi))) continue;
             bool any = false;
             for (const Pending& q : adj) any = any || (q.S == S && q.L == L);
             if (!any) {
@@ -1092,7 +1093,11 @@ bool load_model(const std::string& path, const GtinvTable& table, Model& m, std:
             m.lmax3 = std::max(m.lmax3, ir.l.back());
         }
     m.n_lmh3 = n_lmh_of(m.lmax3);
+    // order-1/2 adjoint rows inside the order-3/4 kernel: those within its
+    // LDS tile and unrolled signatures (l <= min(lmax3, 8)), and those above
+    // the tile up to kMaxLQ12 (a_nlm read from HBM)
     m.q12_fuse = m.has_order34 ? std::min(m.lmax3, kUnrollCap3) : -1;
+    m.q12_fuse_hi = m.has_order34 ? std::min(m.maxl, kMaxLQ12) : -1;
     m.lmaxh = -1;
     for (const Irrep& ir : m.irreps)
         if (ir.l.size() >= 5) m.lmaxh = std::max(m.lmaxh, ir.l.back());

@@ -209,6 +209,9 @@ struct Model {
     // order-1/2 adjoint rows L <= q12_fuse run inside the order-3/4 adjoint
     // kernel, on its LDS tile (-1: none; the order-1/2 kernel takes the rest)
     int q12_fuse = -1;
+    // ... and so do the rows above the order-3/4 tile, lmax3 < L <= q12_fuse_hi
+    // (their a_nlm read from HBM)
+    int q12_fuse_hi = -1;
     // radial-only style (des_type pair): features f = tc * n_fn + n
     bool pair_model = false;
     std::array<std::array<int, 2>, 2> pair_tc{};   // [centre type][neighbor type] -> tc

@@ -32,6 +32,27 @@ struct Tile {
     const double2* As;  // [s][lm][lane] (re, im): one 16-byte LDS read per entry
     int nlmh3;
     int lane;
+    const double* Ag;   // a_nlm of the (block, n) in HBM: [s][lm][re, im][lane]
+    int nlmh;
+    // the row (s, l) of a level above the LDS tile (fused order-1/2 rows
+    // l > lmax3) in HBM: entry m <= 0 at (l - |m|) 2 kBlock from the
+    // returned base.  The base is opaque, so a visit keeps one pointer rather
+    // than every entry's address hoisted out of the visit loop.
+    __device__ __forceinline__ const double* rowg(int s, int l) const {
+        const double* p = Ag + (static_cast<size_t>(s * nlmh + l * (l + 1) / 2) * 2) * kBlock + lane;
+        asm volatile("" : "+v"(p));
+        return p;
+    }
+    __device__ __forceinline__ static void entryg(const double* row, int l, int m, double& re, double& im) {
+        const int am = m < 0 ? -m : m;
+        re = row[(2 * (l - am)) * kBlock];
+        im = row[(2 * (l - am) + 1) * kBlock];
+        if (m > 0) {
+            const double sg = (m & 1) ? -1.0 : 1.0;
+            re *= sg;
+            im *= -sg;
+        }
+    }
     __device__ __forceinline__ void get(int s, int l, int m, double& re, double& im) const {
         const int am = m < 0 ? -m : m;
         const int idx = s * nlmh3 + l * (l + 1) / 2 + l - am;
@@ -336,12 +357,16 @@ struct Unrolled {
 // factor positions this is q times its energy (Euler).
 template <int L>
 __device__ __forceinline__ double row_dot(const Tile& T, int S, const double (&gr)[L + 1],
-                                          const double (&gi)[L + 1]) {
+                                          const double (&gi)[L + 1], bool glob = false) {
     double sum = 0.0;
+    const double* rg = glob ? T.rowg(S, L) : nullptr;
 #pragma unroll
     for (int k = 0; k <= L; ++k) {
         double ar, ai;
-        T.get(S, L, -k, ar, ai);
+        if (glob)
+            Tile::entryg(rg, L, -k, ar, ai);
+        else
+            T.get(S, L, -k, ar, ai);
         sum = fma(k == 0 ? 1.0 : 2.0, gr[k] * ar - gi[k] * ai, sum);
     }
     return sum;
@@ -630,11 +655,100 @@ __device__ __forceinline__ void q34_group(const Tile& T, const DevModel& M, cons
     }
 }
 
+// A fused order-1/2 row above the LDS tile (lmax3 < L <= q12_fuse_hi): its
+// visits are order 1/2 only and read a_{s_a,L,+k} from HBM (each a 64-lane
+// coalesced line); energies by the same Euler classes, G written like the
+// other fused rows (first write, or a partial row of a split row).  One
+// instance serves every such L (entries k <= L of LM-long arrays), which
+// keeps the kernel's code, and the scheduler's register choices for the
+// other rows, as they are.
+template <int LM>
+__device__ __forceinline__ void q34_group_glob(const Tile& T, const DevModel& M, const DevSystem& S,
+                                               const GroupQ34& g, int blk, int n, double* red, double& eacc) {
+    const int L = uniform(g.L);
+    double gr[LM + 1], gi[LM + 1];
+    const int Sx = uniform(g.S);
+#pragma unroll
+    for (int k = 0; k <= LM; ++k) gr[k] = gi[k] = 0.0;
+    const int slot = blk * kBlock + T.lane;
+    const int tsp = uniform(S.block_type[blk]);
+    const int64_t prow = (int64_t)slot * M.n_pr + pcol(M, tsp, n, 0);
+    cptr<VisitPack> vis = as_const(M.vadj);
+    const int vb = uniform(g.vbeg), ve = uniform(g.vend);
+    int cls_cur = -1;
+    double dprev = 0.0;
+    auto close_class = [&](int c) {
+        if (c >= 0 && c <= 3) {
+            const double* rg = T.rowg(Sx, L);
+            double D = 0.0;
+#pragma unroll
+            for (int k = 0; k <= LM; ++k) {
+                if (k <= L) {
+                    double ar, ai;
+                    Tile::entryg(rg, L, -k, ar, ai);
+                    D = fma(k == 0 ? 1.0 : 2.0, gr[k] * ar - gi[k] * ai, D);
+                }
+            }
+            eacc = fma(class_weight(c), D - dprev, eacc);
+            dprev = D;
+        }
+    };
+    for (int vi = vb; vi < ve; ++vi) {
+        const VDec v = decode(vis[vi]);
+        const int order = v.order;
+        if (order == 0 || order > 2) continue;
+        const int h = v.pslot;
+        const int c = h >= 0 ? 4 : order - 1;
+        if (c != cls_cur) {
+            close_class(cls_cur);
+            cls_cur = c;
+        }
+        const double H = h_value(M, S, prow, n, h);
+        cptr<double> c1 = as_const(M.coef) + uniform(v.coef);
+        if (order == 1) {
+            gr[0] = fma(H, c1[0], gr[0]);
+            continue;
+        }
+        const double* rg = T.rowg(uniform(v.sa), L);
+#pragma unroll
+        for (int k = 0; k <= LM; ++k) {
+            if (k <= L) {
+                double yr, yi;
+                Tile::entryg(rg, L, k, yr, yi);
+                const double hc = H * c1[k];
+                gr[k] = fma(hc, yr, gr[k]);
+                gi[k] = fma(hc, yi, gi[k]);
+            }
+        }
+    }
+    close_class(cls_cur);
+    if (uniform(g.nparts) == 1) {
+        double* Gs = g_row_base(M, S, blk, n, T.lane) + static_cast<size_t>(Sx) * M.n_lmh * 2 * kBlock +
+                     static_cast<size_t>(L * (L + 1) / 2) * 2 * kBlock;
+#pragma unroll
+        for (int k = 0; k <= LM; ++k) {
+            if (k <= L) {
+                Gs[(2 * (L - k)) * kBlock] = gr[k];  // lmh(L, -k) = L (L + 1) / 2 + L - k
+                Gs[(2 * (L - k) + 1) * kBlock] = gi[k];
+            }
+        }
+    } else {
+        double* pr = red + uniform(g.red);
+#pragma unroll
+        for (int k = 0; k <= LM; ++k) {
+            if (k <= L) {
+                pr[(2 * k) * kBlock + T.lane] = gr[k];
+                pr[(2 * k + 1) * kBlock + T.lane] = gi[k];
+            }
+        }
+    }
+}
+
 // Sum of the partial rows of a split (S, L) row, in part order, into G.
 __device__ __forceinline__ void q34_combine(const DevModel& M, const DevSystem& S, const GroupQ34& g, int blk, int n,
                                             int lane, const double* red, bool dyn) {
     const int L = uniform(g.L), Sx = uniform(g.S), np = uniform(g.nparts);
-    const bool first = !dyn && L <= M.q12_fuse;  // a fused row: written, not added to
+    const bool first = !dyn && q12_fused(M, L);  // a fused row: written, not added to
     // a species-paired group keeps rows 0 and 1 one after the other per part
     const int nrows = Sx == kPairedRows ? 2 : 1;
     const int rowlen = (L + 1) * 2 * kBlock;
@@ -683,7 +797,7 @@ __global__ __launch_bounds__(kQ34Threads) void k_q34(DevModel M, DevSystem S) {
     double* red = As + 2 * M.n_type * per_s;  // partial rows of split groups
     __syncthreads();
     const int wave = uniform(threadIdx.x >> 6);
-    Tile T{As2, M.n_lmh3, static_cast<int>(threadIdx.x & 63)};
+    Tile T{As2, M.n_lmh3, static_cast<int>(threadIdx.x & 63), S.A + a_index(M, blk, n, 0, 0, 0, 0), M.n_lmh};
     cptr<GroupQ34> groups = as_const(FEAT ? M.gfeat : M.gadj);
     const int gb = FEAT ? M.gfeat_beg[t] : M.gadj_beg[t];
     const int ge = FEAT ? M.gfeat_end[t] : M.gadj_end[t];
@@ -692,7 +806,14 @@ __global__ __launch_bounds__(kQ34Threads) void k_q34(DevModel M, DevSystem S) {
         const GroupQ34 g = groups[gi];
         if (uniform(g.wave) != wave) continue;
         const int L = uniform(g.L);
-        if (!DYN && L > kMaxLUnrolled) continue;
+        // above the unrolled signatures the first kernel has only fused
+        // order-1/2 rows (adjoint)
+        if (!DYN && L > kMaxLUnrolled && (FEAT || L <= M.lmax3)) continue;
+        if (!FEAT && L > M.lmax3) {
+            // fused order-1/2 row above the tile: the first kernel's
+            if constexpr (!FEAT && !DYN) q34_group_glob<kMaxLQ12>(T, M, S, g, blk, n, red, eacc);
+            continue;
+        }
         if (!FEAT && uniform(g.S) == kPairedRows) {
             // species-paired rows hold unrolled visits only: this kernel's
             if constexpr (!FEAT && !DYN) {
@@ -736,7 +857,8 @@ __global__ __launch_bounds__(kQ34Threads) void k_q34(DevModel M, DevSystem S) {
             for (int gi = gb; gi < ge; ++gi) {
                 const GroupQ34 g = groups[gi];
                 if (uniform(g.wave) != wave || uniform(g.nparts) == 1 || uniform(g.part) != 0) continue;
-                if (!DYN && uniform(g.L) > kMaxLUnrolled) continue;
+                if (!DYN && uniform(g.L) > kMaxLUnrolled && uniform(g.L) <= M.lmax3) continue;
+                if (DYN && uniform(g.L) > M.lmax3) continue;  // the first kernel's
                 if (DYN && uniform(g.S) == kPairedRows) continue;  // the unrolled kernel's
                 q34_combine(M, S, g, blk, n, T.lane, red, DYN);
             }
