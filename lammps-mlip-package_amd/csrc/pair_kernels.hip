@@ -51,7 +51,51 @@ __device__ __forceinline__ const YTab<float>* ytab_of<float>(const DevModel& M) 
     return M.yt32;
 }
 
-__device__ __forceinline__ void sin_cos(double a, double* s, double* c) { sincos(a, s, c); }
+// sin and cos of the cutoff argument a = pi r / r_c, 0 < r < r_c, so a lies
+// in (0, pi) and needs no range reduction: t = a (a <= pi/2) or pi - a
+// (exact by Sterbenz, plus the low part of pi), sin a = sin t, cos a = +-cos t,
+// each a Taylor series in t^2 through t^21 and t^22 (remainder below 2e-18 for
+// |t| <= pi/2; Horner's rounding, about one ulp, dominates).  The coefficients
+// (-1)^k / (2k+1)! and (-1)^k / (2k)!, k >= 1, are read from LDS through a
+// base the compiler cannot take as constant, so they are loaded where the
+// series runs: the library routine's reduction constants, materialised once
+// per thread and held across the pair loop, were spilled to scratch.
+constexpr int kTrigSin = 10, kTrigCos = 11;
+constexpr int kTrigN = kTrigSin + kTrigCos;
+typedef const double __attribute__((address_space(3))) lds_cdouble;
+
+__device__ __forceinline__ double trig_coef(int i) {
+    const int k = i < kTrigSin ? i + 1 : i - kTrigSin + 1;
+    const int n = i < kTrigSin ? 2 * k + 1 : 2 * k;
+    double f = 1.0;
+    for (int j = 2; j <= n; ++j) f *= static_cast<double>(j);
+    return ((k & 1) ? -1.0 : 1.0) / f;
+}
+
+// q: the coefficients in LDS, as an opaque LDS pointer (lds_opaque).
+__device__ __forceinline__ lds_cdouble* lds_opaque(const double* p) {
+    lds_cdouble* q = (lds_cdouble*)p;
+    asm volatile("" : "+v"(q));
+    return q;
+}
+
+__device__ __forceinline__ void sin_cos(double a, lds_cdouble* q, double* s, double* c) {
+    constexpr double kPiHi = 3.141592653589793;       // pi rounded to double
+    constexpr double kPiLo = 1.2246467991473532e-16;  // pi - kPiHi
+    const bool up = a > 0.5 * kPiHi;
+    const double t = up ? (kPiHi - a) + kPiLo : a;
+    const double u = t * t;
+    double ps = q[kTrigSin - 1];
+#pragma unroll
+    for (int k = kTrigSin - 2; k >= 0; --k) ps = fma(ps, u, q[k]);
+    double pc = q[kTrigN - 1];
+#pragma unroll
+    for (int k = kTrigN - 2; k >= kTrigSin; --k) pc = fma(pc, u, q[k]);
+    const double st = fma(t * u, ps, t);
+    const double ct = fma(u, pc, 1.0);
+    *s = st;
+    *c = up ? -ct : ct;
+}
 
 struct PairDel {
     double dx, dy, dz, r;
@@ -534,6 +578,7 @@ struct ForceLds {
     T alm[KL1][KL1];
     T ainv[KL1][KL1];
     T grad[KL1][KL1];
+    double trig[kTrigN + 1];  // sin / cos series coefficients, then pi / r_c
 };
 
 // Register budget for occupancy: FP64 at most 168 VGPRs (3 waves per SIMD;
@@ -578,6 +623,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NF > 16 ? 1 
             C.grad[l][mm] = yt->grad[l][mm];
         }
         if (threadIdx.x <= KL1) C.diag[threadIdx.x] = yt->diag[threadIdx.x];
+        if (threadIdx.x < kTrigN) C.trig[threadIdx.x] = trig_coef(threadIdx.x);
+        if (threadIdx.x == kTrigN) C.trig[kTrigN] = kPiRef / M.cutoff;
         if (threadIdx.x < kMaxFn) {
             C.beta[threadIdx.x] = threadIdx.x < nfn ? M.beta[threadIdx.x] : 0.0;
             C.mu[threadIdx.x] = threadIdx.x < nfn ? M.mu[threadIdx.x] : 0.0;
@@ -585,21 +632,24 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NF > 16 ? 1 
     }
     __syncthreads();
     const int p0 = S.poff[atom], p1 = S.pend[atom];
-    const double pir = kPiRef / M.cutoff;
     double v0 = 0, v1 = 0, v2 = 0, v3 = 0, v4 = 0, v5 = 0;
     for (int64_t p = p0 + threadIdx.x; p < p1; p += NT) {
         const PairDel pg = pair_del(M, S, atom, p);
         double fx = 0.0, fy = 0.0, fz = 0.0;
+        double ex = pg.dx, ey = pg.dy, ez = pg.dz;  // del for the virial
         if (pg.valid) {
             const int spec = S.elem[S.pnbr[p]];
             const T* Gsp = Gs + (size_t)spec * nlmh * NF * 2;
             const double ird = 1.0 / pg.r;
             const T x = static_cast<T>(pg.dx * ird), y = static_cast<T>(pg.dy * ird),
                     z = static_cast<T>(pg.dz * ird);
-            // radial functions and the cutoff function in FP64 in both modes
+            // radial functions and the cutoff function in FP64 in both modes; pi / r_c
+            // and the series coefficients are read from LDS here rather than held
+            lds_cdouble* tq = lds_opaque(C.trig);
+            const double pir = tq[kTrigN];
             const double arg = pir * pg.r;
             double sa, ca;
-            sin_cos(arg, &sa, &ca);
+            sin_cos(arg, tq, &sa, &ca);
             const double fc = 0.5 * (ca + 1.0);
             const double fcd = -0.5 * pir * sa;
             T gn[NF], gd[NF];
@@ -729,22 +779,37 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NF > 16 ? 1 
                 cr = pr;
                 ci = pi;
             }
-            // dE/d(del) = er * rhat + t / r
-            const double ir = ird;
-            fx = er * (pg.dx * ir) + tx * ir;
-            fy = er * (pg.dy * ir) + ty * ir;
-            fz = er * (pg.dz * ir) + tz * ir;
+            // dE/d(del) = er * rhat + t / r.  In FP64, rhat is (x, y, z) itself and del
+            // is r rhat within an ulp, so of the pair's geometry only r stays live across
+            // the column loop (del and 1/r held there were spilled to scratch); the mixed
+            // mode, whose unit vector is FP32, keeps del.
+            if constexpr (sizeof(T) == 8) {
+                double rl = pg.r;
+                asm volatile("" : "+v"(rl));  // 1/r recomputed here, not kept from above
+                const double ir = 1.0 / rl;
+                fx = er * xd + tx * ir;
+                fy = er * yd + ty * ir;
+                fz = er * zd + tz * ir;
+                ex = xd * rl;
+                ey = yd * rl;
+                ez = zd * rl;
+            } else {
+                const double ir = ird;
+                fx = er * (pg.dx * ir) + tx * ir;
+                fy = er * (pg.dy * ir) + ty * ir;
+                fz = er * (pg.dz * ir) + tz * ir;
+            }
         }
         S.gpair[3 * p] = fx;
         S.gpair[3 * p + 1] = fy;
         S.gpair[3 * p + 2] = fz;
         // virial: del (x) F_centre with F_centre = -dE/d(del)
-        v0 -= pg.dx * fx;
-        v1 -= pg.dy * fy;
-        v2 -= pg.dz * fz;
-        v3 -= pg.dx * fy;
-        v4 -= pg.dx * fz;
-        v5 -= pg.dy * fz;
+        v0 -= ex * fx;
+        v1 -= ey * fy;
+        v2 -= ez * fz;
+        v3 -= ex * fy;
+        v4 -= ex * fz;
+        v5 -= ey * fz;
     }
     // fixed-order reduction: butterfly within each wave, then over waves
     for (int off = 32; off > 0; off >>= 1) {
