@@ -238,10 +238,12 @@ int mlip_calibration_copy(mlip_ctx* ctx, int64_t nbytes, int reps);
  * evaluation to host.  what = 0: a_nlm [slot order], 1: polynomial features
  * (per slot a compact row of the features its species sees, in the engine's
  * column order), 2: per-centre energies, 3: the adjoint G (layout of a_nlm),
- * 4: the quadratic-form products Hq (layout of 1), 5: the per-species
- * quadratic forms, 6: seven numbers (row pitch; per species the offset and
- * dimension of its form and its slot blocks); returns the number of doubles
- * written. */
+ * 4: the quadratic-form GEMM's output (layout of 1): the products D W, or,
+ * for a model without cubic terms, whose GEMM epilogue forms the adjoint
+ * weights, H = lin + D W; 5: the per-species quadratic forms, 6: eight
+ * numbers (row pitch; per species the offset and dimension of its form and
+ * its slot blocks; 1 if the epilogue forms H), 7: the linear weights per
+ * species column ([2][row pitch]); returns the number of doubles written. */
 int64_t mlip_debug_fetch(mlip_ctx* ctx, int what, double* out, int64_t cap);
 
 /* Model description: n_type, n_fn, n_gtinv, n_des, n_coeff, maxl, lmax3,

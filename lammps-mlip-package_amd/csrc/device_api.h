@@ -61,6 +61,10 @@ struct DevModel {
     const int32_t* c3_off;  // cubic terms transposed by target: offsets ([species][n_pr + 1])
     const int32_t* c3_bc;   // 2 per entry (columns of the same species)
     const double* c3_w;
+    // 1 where the model has no cubic terms: the quadratic-form GEMM's
+    // epilogue forms H = lin + D W and per column tile partial energies
+    // sum d (lin + (D W) / 2), which launch_poly sums into Ep
+    int poly_fused;
     // per-atom energy split of the reference CPU style (each polynomial term
     // of dE/da weighted by 1 / its degree in a): linear weights divided by
     // the family's order (n_des), the degree (family order) of each compact
@@ -166,6 +170,8 @@ hipError_t launch_qhigh(const DevModel& m, const DevSystem& s, bool feat, hipStr
 // Hq = D W per species on the matrix cores (slot blocks grouped by species,
 // nblk_type[t] blocks of species t in order)
 hipError_t launch_poly_gemm(const DevModel& m, const DevSystem& s, const int* nblk_type, hipStream_t st);
+// Ep from the fused GEMM epilogue's partial energies (poly_fused models)
+hipError_t launch_poly_sum(const DevModel& m, const DevSystem& s, hipStream_t st);
 // threads per workgroup of the force kernel: 64, 128, 192 (default) or 256
 hipError_t launch_force(const DevModel& m, const DevSystem& s, int precision, int threads, hipStream_t st);
 // per-structure energy + virial (7 doubles each) of a batch; soff = atom offsets

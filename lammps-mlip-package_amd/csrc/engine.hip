@@ -253,6 +253,7 @@ bool upload_model(mlip_ctx* c) {
               upload(c, c->d_fq12, fq12.data(), fq12.size(), "order-1/2 feature list") &&
               upload(c, c->d_zlist, zlist.data(), zlist.size(), "feature zero list");
     if (!ok) return false;
+    dm.poly_fused = (!m.pair_model && m.n_p > 0 && c3w.empty()) ? 1 : 0;
     dm.zlist = c->d_zlist.p;
     dm.fq12 = c->d_fq12.p;
     if (m.pair_model) {
@@ -1726,7 +1727,9 @@ int64_t mlip_debug_fetch(mlip_ctx* c, int what, double* out, int64_t cap) {
         src = c->G.p;
         n = nslot * m.n_fn * m.n_type * m.n_lmh * 2;
     } else if (what == 4) {
-        src = c->D.p ? c->Hq.p : nullptr;
+        // the quadratic-form GEMM's output: D W, or lin + D W (= H) where its
+        // epilogue forms H (no cubic terms)
+        src = c->D.p ? (c->dm.poly_fused ? c->H.p : c->Hq.p) : nullptr;
         n = nslot * m.n_p;
     } else if (what == 5) {
         src = c->d_wmat.p;
@@ -1734,13 +1737,16 @@ int64_t mlip_debug_fetch(mlip_ctx* c, int what, double* out, int64_t cap) {
     } else if (what == 6) {
         // layout of the per-species products: row pitch, then per species
         // (offset of its quadratic form, its dimension, its slot blocks)
-        const double v[7] = {static_cast<double>(c->dm.n_pr), static_cast<double>(c->wofs[0]),
+        const double v[8] = {static_cast<double>(c->dm.n_pr), static_cast<double>(c->wofs[0]),
                              static_cast<double>(c->wdim[0]), static_cast<double>(c->nblk_type[0]),
                              static_cast<double>(c->wofs[1]), static_cast<double>(c->wdim[1]),
-                             static_cast<double>(c->nblk_type[1])};
-        if (cap < 7) return -1;
-        std::copy(v, v + 7, out);
-        return 7;
+                             static_cast<double>(c->nblk_type[1]), static_cast<double>(c->dm.poly_fused)};
+        if (cap < 8) return -1;
+        std::copy(v, v + 8, out);
+        return 8;
+    } else if (what == 7) {
+        src = c->d_linp.p;  // linear weights per species column ([2][row pitch])
+        n = static_cast<size_t>(2) * c->dm.n_pr;
     }
     if (!src || static_cast<int64_t>(n) > cap) return -1;
     if (hipStreamSynchronize(c->stream) != hipSuccess) return -1;

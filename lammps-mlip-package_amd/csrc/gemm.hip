@@ -33,10 +33,20 @@ constexpr int kGThreads = 256;
 // (measured 2.48 -> 2.35 ms per evaluation at 256k against a pad of 1)
 constexpr int kGPad = 16;
 
-__global__ __launch_bounds__(kGThreads) void k_gemm_rows(const double* __restrict__ A, int lda,
+// EPI: the polynomial step of a model without cubic terms in the epilogue,
+// C = H = lin + A B and epart[row * ldep + nt] = sum over the tile's columns
+// of A[row][col] (lin + (A B) / 2) (A holds the features d, K = N), each row's
+// partial in a fixed order: four columns per lane, a butterfly over the 16
+// lanes of the row, then the two column waves.  Held to 4 waves per SIMD
+// (128 registers with the accumulators): unconstrained, the compiler issues
+// every epilogue load at once (108 VGPRs), which halves the occupancy of
+// the product loop (2.35 -> 2.97 ms at 256k).
+template <bool EPI>
+__global__ __launch_bounds__(kGThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_gemm_rows(const double* __restrict__ A, int lda,
                                                          const double* __restrict__ B, int ldb,
                                                          double* __restrict__ C, int ldc, int K, int N,
-                                                         int mtiles, int ntiles) {
+                                                         int mtiles, int ntiles, const double* __restrict__ lin,
+                                                         double* __restrict__ epart, int ldep) {
     __shared__ double As[kGK][kGM + kGPad];  // [k][row]
     __shared__ double Bs[kGK][kGN + kGPad];  // [k][column]
     // XCD-aware order: workgroups are dealt round-robin over the 8 XCDs, and
@@ -97,6 +107,32 @@ __global__ __launch_bounds__(kGThreads) void k_gemm_rows(const double* __restric
                 for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
         }
     }
+    if constexpr (EPI) {
+        __shared__ double esum[2][kGM];  // [column wave][row of the tile]
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int rt = wr * 32 + i * 16 + (lane >> 4) + 4 * r;
+                const size_t row = static_cast<size_t>(m0 + rt);
+                double e = 0.0;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const int col = n0 + wc * 64 + j * 16 + (lane & 15);
+                    if (col < N) {
+                        const double la = lin[col], hq = acc[i][j][r];
+                        C[row * ldc + col] = la + hq;
+                        e = fma(A[row * lda + col], la + 0.5 * hq, e);
+                    }
+                }
+#pragma unroll
+                for (int o = 1; o < 16; o <<= 1) e += __shfl_xor(e, o);
+                if ((lane & 15) == 0) esum[wc][rt] = e;
+            }
+        __syncthreads();
+        if (tid < kGM) epart[static_cast<size_t>(m0 + tid) * ldep + nt] = esum[0][tid] + esum[1][tid];
+        return;
+    }
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -111,6 +147,19 @@ __global__ __launch_bounds__(kGThreads) void k_gemm_rows(const double* __restric
         }
 }
 
+// Ep[slot] = the slot's partial energies of the fused epilogue, in column
+// tile order (no tile for a species without polynomial features: 0)
+__global__ __launch_bounds__(256) void k_ep_sum(DevModel M, DevSystem S) {
+    const int slot = blockIdx.x * blockDim.x + threadIdx.x;
+    if (slot >= S.nblocks * kBlock) return;
+    const int t = S.block_type[slot / kBlock];
+    const int nt = (M.wdim[t] + kGN - 1) / kGN;
+    const double* ep = S.Hq + static_cast<size_t>(slot) * M.n_pr;
+    double e = 0.0;
+    for (int k = 0; k < nt; ++k) e += ep[k];
+    S.Ep[slot] = e;
+}
+
 }  // namespace
 
 hipError_t launch_poly_gemm(const DevModel& m, const DevSystem& s, const int* nblk_type, hipStream_t st) {
@@ -122,12 +171,25 @@ hipError_t launch_poly_gemm(const DevModel& m, const DevSystem& s, const int* nb
         if (nb == 0 || w == 0) continue;
         const int mtiles = nb * kBlock / kGM, ntiles = (w + kGN - 1) / kGN;
         const int grid = 8 * ntiles * ((mtiles + 7) / 8);
-        hipLaunchKernelGGL(k_gemm_rows, dim3(grid), dim3(kGThreads), 0, st, s.D + r0 * m.n_pr, m.n_pr,
-                           m.wmat + m.wofs[t], w, s.Hq + r0 * m.n_pr, m.n_pr, w, w, mtiles, ntiles);
+        if (m.poly_fused)
+            hipLaunchKernelGGL(k_gemm_rows<true>, dim3(grid), dim3(kGThreads), 0, st, s.D + r0 * m.n_pr, m.n_pr,
+                               m.wmat + m.wofs[t], w, s.H + r0 * m.n_pr, m.n_pr, w, w, mtiles, ntiles,
+                               m.lin_p + static_cast<size_t>(t) * m.n_pr, s.Hq + r0 * m.n_pr, m.n_pr);
+        else
+            hipLaunchKernelGGL(k_gemm_rows<false>, dim3(grid), dim3(kGThreads), 0, st, s.D + r0 * m.n_pr, m.n_pr,
+                               m.wmat + m.wofs[t], w, s.Hq + r0 * m.n_pr, m.n_pr, w, w, mtiles, ntiles,
+                               nullptr, nullptr, 0);
         const hipError_t e = launched();
         if (e != hipSuccess) return e;
     }
     return b0 == s.nblocks ? hipSuccess : hipErrorInvalidValue;
+}
+
+hipError_t launch_poly_sum(const DevModel& m, const DevSystem& s, hipStream_t st) {
+    const int nslot = s.nblocks * kBlock;
+    if (nslot == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_ep_sum, dim3((nslot + 255) / 256), dim3(256), 0, st, m, s);
+    return launched();
 }
 
 }  // namespace mlip

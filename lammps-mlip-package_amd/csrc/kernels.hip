@@ -6,7 +6,8 @@
 //   k_q12    order-1/2 invariants: polynomial features or adjoint G
 //   k_q34    order-3/4 invariants: polynomial features or adjoint G,
 //            a_nlm tile of 64 same-species centres resident in LDS
-//   k_gemm_rows  Hq = D W (quadratic form of the polynomial model, FP64 MFMA; gemm.hip)
+//   k_gemm_rows  Hq = D W (quadratic form of the polynomial model, FP64 MFMA; gemm.hip),
+//                or, without cubic terms, H and the polynomial energy in its epilogue
 //   k_poly   E_i polynomial part and H = dE_i/dd                 (:321-343,481-521)
 //   k_force  per-pair dE_i/d(del) from G and dphi/d(del)          (:161-256, pair_kernels.hip)
 //   k_gather atomics-free per-position force sums, deterministic
@@ -487,6 +488,7 @@ hipError_t launch_q12(const DevModel& m, const DevSystem& s, bool feat, hipStrea
 hipError_t launch_poly(const DevModel& m, const DevSystem& s, hipStream_t st) {
     const int nslot = s.nblocks * kBlock;
     if (nslot == 0 || m.n_p == 0) return hipSuccess;
+    if (m.poly_fused) return launch_poly_sum(m, s, st);  // H and the partials came from the GEMM
     hipLaunchKernelGGL(k_poly, dim3((nslot + 3) / 4), dim3(256), 0, st, m, s);
     return launched();
 }
