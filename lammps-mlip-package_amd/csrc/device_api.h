@@ -30,6 +30,8 @@ struct DevModel {
     int has_q34;            // order-3/4 invariants present (k_q34 runs)
     int q12_fuse;           // order-1/2 adjoint rows L <= q12_fuse are the order-3/4 kernel's (-1: none)
     int q12_fuse_hi;        // ... and the rows lmax3 < L <= q12_fuse_hi (a_nlm from HBM)
+    int q12_adj_levels;     // 1 if some level is left to the order-1/2 adjoint kernel (else it
+                            // is not launched and its energies Eq12 are not read)
     double cutoff;
     double beta[kMaxFnPair], mu[kMaxFnPair];
     const double* coef;     // dense coefficient blocks

@@ -124,6 +124,8 @@ bool upload_model(mlip_ctx* c) {
     dm.has_q34 = m.has_order34 ? 1 : 0;
     dm.q12_fuse = m.q12_fuse;
     dm.q12_fuse_hi = m.q12_fuse_hi;
+    for (int L = 0; L <= m.maxl; ++L)
+        if (!q12_fused(dm, L)) dm.q12_adj_levels = 1;
     dm.q34_red = m.q34_red;
     dm.pair_model = m.pair_model ? 1 : 0;
     for (int a = 0; a < 2; ++a)

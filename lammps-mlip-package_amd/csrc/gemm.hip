@@ -61,7 +61,9 @@ __global__ __launch_bounds__(kGThreads) __attribute__((amdgpu_waves_per_eu(4))) 
     const int wr = wave >> 1, wc = wave & 1;
     // global -> register staging roles
     const int arow = tid >> 2, ak = (tid & 3) * 4;   // A: 64 rows x 16 k, 4 per thread
-    const int bk = tid >> 4, bc = (tid & 15) * 8;    // B: 16 k x 128 columns, 8 per thread
+    // B: 16 k x 128 columns, 8 per thread, 16 apart (a row's 16 lanes store
+    // consecutive doubles: no LDS bank conflicts; global reads stay 128-byte runs)
+    const int bk = tid >> 4, bc = tid & 15;
     const double* arow_p = A + (size_t)(m0 + arow) * lda;
     double ra[4], rb[8];
     auto load = [&](int kb) {
@@ -74,7 +76,7 @@ __global__ __launch_bounds__(kGThreads) __attribute__((amdgpu_waves_per_eu(4))) 
         const double* brow = B + (size_t)k * ldb;
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
-            const int c = n0 + bc + q;
+            const int c = n0 + bc + 16 * q;
             rb[q] = (k < K && c < N) ? brow[c] : 0.0;
         }
     };
@@ -90,7 +92,7 @@ __global__ __launch_bounds__(kGThreads) __attribute__((amdgpu_waves_per_eu(4))) 
 #pragma unroll
         for (int q = 0; q < 4; ++q) As[ak + q][arow] = ra[q];
 #pragma unroll
-        for (int q = 0; q < 8; ++q) Bs[bk][bc + q] = rb[q];
+        for (int q = 0; q < 8; ++q) Bs[bk][bc + 16 * q] = rb[q];
         __syncthreads();
         if (kt + 1 < nk) load((kt + 1) * kGK);
 #pragma unroll

@@ -367,7 +367,7 @@ __global__ __launch_bounds__(256) void k_energy_slot(DevModel M, DevSystem S) {
     } else if (S.atom_of_slot[slot] >= 0) {
         e = M.n_p > 0 ? S.Ep[slot] : 0.0;
         for (int n = 0; n < M.n_fn; ++n) {
-            e += S.Eq12[(size_t)slot * M.n_fn + n];
+            if (M.q12_adj_levels) e += S.Eq12[(size_t)slot * M.n_fn + n];
             if (M.has_q34)
                 for (int w = 0; w < kWavesPerWg; ++w) e += S.Eq34[((size_t)slot * M.n_fn + n) * kWavesPerWg + w];
         }
@@ -480,6 +480,7 @@ hipError_t launch_q12(const DevModel& m, const DevSystem& s, bool feat, hipStrea
         const int stage = (wmax > 0 && lds <= kQ12MaxStage) ? 1 : 0;
         hipLaunchKernelGGL(k_q12<true>, dim3(s.nblocks, m.n_fn), dim3(64), stage ? lds : 0, st, m, s, stage);
     } else {
+        if (!m.q12_adj_levels) return hipSuccess;  // every level runs in the order-3/4 kernel
         hipLaunchKernelGGL(k_q12<false>, dim3(s.nblocks, m.n_fn), dim3(64), 0, st, m, s, 0);
     }
     return launched();
