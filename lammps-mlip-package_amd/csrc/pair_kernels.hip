@@ -602,17 +602,19 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NF > 16 ? 1 
     if (atom < 0) return;
     const int nfn = M.n_fn, ns = M.n_type, nlmh = M.n_lmh, maxl = M.maxl;
     const int blk = slot / kBlock, lane = slot % kBlock;
+    // four tile entries per step, read without a branch (padding columns
+    // n >= n_fn read an in-range entry and store zero), so that their HBM /
+    // L2 reads are in flight together
+#pragma unroll 4
     for (int k = threadIdx.x; GL && k < ns * nlmh * NF; k += NT) {
         const int n = k % NF;
         const int lm = (k / NF) % nlmh;
         const int s = k / (NF * nlmh);
-        T gr = T(0), gi = T(0);
-        if (n < nfn) {
-            gr = static_cast<T>(S.G[a_index(M, blk, n, s, lm, 0, lane)]);
-            gi = static_cast<T>(S.G[a_index(M, blk, n, s, lm, 1, lane)]);
-        }
-        Gs[2 * k] = gr;
-        Gs[2 * k + 1] = gi;
+        const int nr = n < nfn ? n : nfn - 1;
+        const double gr = S.G[a_index(M, blk, nr, s, lm, 0, lane)];
+        const double gi = S.G[a_index(M, blk, nr, s, lm, 1, lane)];
+        Gs[2 * k] = n < nfn ? static_cast<T>(gr) : T(0);
+        Gs[2 * k + 1] = n < nfn ? static_cast<T>(gi) : T(0);
     }
     {
         const YTab<T>* yt = ytab_of<T>(M);
