@@ -170,7 +170,8 @@ hipError_t launch_poly(const DevModel& m, const DevSystem& s, hipStream_t st);
 // of the energy of linear families (qhigh.hip); no-op for models without them
 hipError_t launch_qhigh(const DevModel& m, const DevSystem& s, bool feat, hipStream_t st);
 // Hq = D W per species on the matrix cores (slot blocks grouped by species,
-// nblk_type[t] blocks of species t in order)
+// nblk_type[t] blocks of species t in order); for poly_fused models H =
+// lin + D W and per column tile partial energies (in the Hq rows) instead
 hipError_t launch_poly_gemm(const DevModel& m, const DevSystem& s, const int* nblk_type, hipStream_t st);
 // Ep from the fused GEMM epilogue's partial energies (poly_fused models)
 hipError_t launch_poly_sum(const DevModel& m, const DevSystem& s, hipStream_t st);

@@ -631,7 +631,9 @@ bool run_pipeline(mlip_ctx* c) {
         if (!ok) return false;
         // Hq = D W per species: slot blocks are grouped by species and a
         // species' row holds only the features it sees, so each species is
-        // one dense product on its rows and its own quadratic form
+        // one dense product on its rows and its own quadratic form.  Without
+        // cubic terms the product's epilogue also forms H and the energy
+        // partials, and "poly" only sums those; otherwise k_poly does both
         run("poly_gemm", [&] { return launch_poly_gemm(m, s, c->nblk_type, st); });
         run("poly", [&] { return launch_poly(m, s, st); });
     }
