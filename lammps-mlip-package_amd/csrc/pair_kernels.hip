@@ -150,7 +150,7 @@ struct AnlmLds {
     T ux[NTH], uy[NTH], uz[NTH];
     double rr[NTH], fc[NTH];
     int spec[NTH];
-    alignas(16) double g[kAnlmJ][GW];
+    alignas(16) double g[kAnlmJ][GW + 2];  // rows 16 B longer: a column's stores on distinct banks
 };
 
 // Radial indices per round of the feature epilogue: a of both species
@@ -197,7 +197,12 @@ __global__ __launch_bounds__(NTH) void k_anlm(DevModel M, DevSystem S, int ngrou
 #pragma unroll
     for (int k = 0; k < NPT; ++k) a0r[k] = a0i[k] = a1r[k] = a1i[k] = 0.0;
     // phase-1 role
-    const int j1 = tid / kAnlmParts, part = tid % kAnlmParts;
+    // lanes 16 apart share a column m: a 16-lane group takes one m for the
+    // chunk's 16 neighbors, so a wave's lanes run at most four different
+    // column lengths (the waves of the short columns finish early instead of
+    // every wave waiting for m = 0), the recursion constants are broadcasts,
+    // and a group's Y stores fall on distinct banks (rows n_lmh apart)
+    const int j1 = tid % kAnlmJ, part = tid / kAnlmJ;
     const double pir = kPiRef / M.cutoff;
     __syncthreads();
     for (int b0 = p0; b0 < p1; b0 += kAnlmThreads) {
