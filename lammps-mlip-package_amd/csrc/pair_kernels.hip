@@ -234,7 +234,10 @@ __global__ __launch_bounds__(NTH) void k_anlm(DevModel M, DevSystem S, int ngrou
                 }
                 const T x = L.ux[q], y = L.uy[q], z = L.uz[q];
                 T2* yrow = ys + j1 * nlmh;
-                for (int m = part; m <= maxl; m += kAnlmParts) {
+                // columns in boustrophedon order over the parts (part, then
+                // 2 P - 1 - part, ...): the second round's short columns go to
+                // the last parts, whose waves hold the short first-round ones
+                for (int r = 0, m = part; m <= maxl; ++r, m = (r & 1) ? (r + 1) * kAnlmParts - 1 - part : r * kAnlmParts + part) {
                     // (-1)^m conj(u)^m by binary powering
                     T pr = (m & 1) ? T(-1) : T(1), pi = T(0);
                     T br = x, bi = -y;
