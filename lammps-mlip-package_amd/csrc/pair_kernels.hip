@@ -254,6 +254,9 @@ __global__ __launch_bounds__(NTH) void k_anlm(DevModel M, DevSystem S, int ngrou
                     T q2 = T(0), q1 = L.diag[m];
                     int idx = m * (m + 1) / 2;  // lmh(m, -m)
                     yrow[idx] = T2{q1 * pr, q1 * pi};
+                    // two steps per trip: half the loop's exec-mask and index
+                    // work, and no register rotation of q1, q2
+#pragma unroll 2
                     for (int l = m + 1; l <= maxl; ++l) {
                         const T qn = L.alm[l][m] * (z * q1 - L.ainv[l][m] * q2);
                         idx += l + 1;  // lmh(l, -m) = lmh(l-1, -m) + l + 1
