@@ -155,11 +155,17 @@ struct AnlmLds {
 
 // Radial indices per round of the feature epilogue: a of both species
 // (FP64 pairs) and the round's feature segments must fit the Y_lm chunk
-// buffer they reuse, kAnlmJ rows of n_lmh entries of kAnlmBufEntry bytes in
-// both precisions (an FP32 chunk uses half of it; sized for double2, the
-// mixed mode's epilogue takes 7 radial indices per round at l_max 12, as
-// FP64 does, instead of 3, and runs half the rounds and their barriers).
-constexpr int kAnlmBufEntry = 16;
+// buffer they reuse, kAnlmJ rows of n_lmh entries of anlm_buf_entry bytes.
+// The compact tables (l_max <= 12) use 16-byte entries in both precisions
+// (an FP32 chunk uses half: the mixed mode's epilogue then takes 7 radial
+// indices per round at l_max 12, as FP64 does, instead of 3, and runs half
+// the rounds and their barriers, with 4 workgroups per CU either way); the
+// full tables keep entries of T2, since a larger buffer there would cost a
+// workgroup per CU.
+template <typename T, int KL1>
+constexpr int anlm_buf_entry() {
+    return KL1 == kSmallL1 ? 16 : static_cast<int>(sizeof(typename Vec2<T>::type));
+}
 __host__ __device__ inline int anlm_feature_round(int n_lmh, int w, int t2_bytes) {
     const int buf = kAnlmJ * n_lmh * t2_bytes;
     return buf / (2 * n_lmh * 16 + w * 8);
@@ -336,7 +342,7 @@ __global__ __launch_bounds__(NTH) void k_anlm(DevModel M, DevSystem S, int ngrou
     const int t = S.block_type[blk];
     const int lo = M.prange[t][0], w = M.prange[t][1] - lo;
     const int f0 = M.fq12_off[t], nfe = M.fq12_off[t + 1] - f0;
-    const int R = anlm_feature_round(nlmh, w, kAnlmBufEntry);
+    const int R = anlm_feature_round(nlmh, w, anlm_buf_entry<T, KL1>());
     const int nw0 = blockIdx.y * ngroup * NPT, nw1 = min(nw0 + ngroup * NPT, nfn);
     double2* as = reinterpret_cast<double2*>(anlm_dyn);
     for (int r0 = nw0; r0 < nw1; r0 += R) {
@@ -944,8 +950,8 @@ int anlm_npt(const DevModel& m, int nth, int* ngroup, int* passes) {
 template <int NTH, typename T, int KL1, int GW>
 hipError_t launch_anlm_nth(const DevModel& m, const DevSystem& s, int npt, int ngroup, int passes, int feat,
                            hipStream_t st) {
-    static_assert(sizeof(typename Vec2<T>::type) <= kAnlmBufEntry, "a Y_lm chunk fits the buffer");
-    const size_t lds = static_cast<size_t>(kAnlmBufEntry) * kAnlmJ * m.n_lmh;
+    static_assert(sizeof(typename Vec2<T>::type) <= anlm_buf_entry<T, KL1>(), "a Y_lm chunk fits the buffer");
+    const size_t lds = static_cast<size_t>(anlm_buf_entry<T, KL1>()) * kAnlmJ * m.n_lmh;
     if (lds + sizeof(AnlmLds<T, NTH, KL1, GW>) > lds_limit()) return hipErrorInvalidConfiguration;
     const dim3 grid(slot_grid(s), passes), block(NTH);
     switch (npt) {
@@ -1109,9 +1115,11 @@ bool anlm_has_mfma(const DevModel& m) { return anlm_mfma_fits(m); }
 bool anlm_forms_features(const DevModel& m, int precision, int kind) {
     if (kind == kAnlmMfma && anlm_mfma_fits(m)) return false;
     if (m.pair_model || m.n_p <= 0) return false;
-    (void)precision;  // the chunk buffer has the same size in both precisions
+    // the smallest buffer either table size may use (T2 entries): a round
+    // that fits it fits the compact tables' larger one too
+    const int t2 = precision == kMixed ? 8 : 16;
     for (int t = 0; t < m.n_type && t < 2; ++t)
-        if (anlm_feature_round(m.n_lmh, m.prange[t][1] - m.prange[t][0], kAnlmBufEntry) < 1) return false;
+        if (anlm_feature_round(m.n_lmh, m.prange[t][1] - m.prange[t][0], t2) < 1) return false;
     return true;
 }
 
