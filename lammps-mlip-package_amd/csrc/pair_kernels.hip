@@ -462,7 +462,9 @@ __global__ __launch_bounds__(kAmThreads) void k_anlm_mfma(DevModel M, DevSystem 
             for (int t = 0; t < NTW; ++t) acc[s][i][t] = dbl4{0.0, 0.0, 0.0, 0.0};
     const int p0 = S.poff[atom], p1 = S.pend[atom];
     const double pir = kPiRef / M.cutoff;
-    const int j1 = tid >> 4, part = tid & 15;  // phase 1: neighbor of the chunk, part within it
+    // phase 1: lanes 16 apart share a column m (part), the 16 lanes of a
+    // group take the chunk's 16 neighbors (j1), as in k_anlm
+    const int j1 = tid & 15, part = tid >> 4;
     __syncthreads();
     for (int b0 = p0; b0 < p1; b0 += kAmThreads) {
         const int nb = min(kAmThreads, p1 - b0);
@@ -519,6 +521,7 @@ __global__ __launch_bounds__(kAmThreads) void k_anlm_mfma(DevModel M, DevSystem 
                         int idx = m * (m + 1) / 2;  // lmh(m, -m)
                         yrow[2 * idx] = static_cast<double>(q1 * pr);
                         yrow[2 * idx + 1] = static_cast<double>(q1 * pi);
+#pragma unroll 2
                         for (int l = m + 1; l <= maxl; ++l) {
                             const T qn = L.alm[l][m] * (z * q1 - L.ainv[l][m] * q2);
                             idx += l + 1;  // lmh(l, -m) = lmh(l-1, -m) + l + 1
